@@ -33,8 +33,14 @@ static inline int64_t fidx(const int dm[3], int i, int j, int k, int c)
 }
 
 /* The reference indexes l = dm+1 / r = dm+1 when x - floor(x) rounds up to
- * 1.0 (x = -tiny), reading past the array; here such an index wraps to 1. */
-static inline int wrap1(int i, int n) { return i > n ? i - n : i; }
+ * 1.0 (x = -tiny), reading past the array; here such an index wraps to 1.
+ * Indices outside [1, n] otherwise only come from non-finite positions and
+ * are clamped to 1 (the reference reads out of bounds there too). */
+static inline int wrap1(int i, int n)
+{
+    i = i > n ? i - n : i;
+    return (i >= 1 && i <= n) ? i : 1;
+}
 
 /* sources/interpolations.F90:104-226 */
 void orc_interpolate_lin(const int dm[3], const double xl[3], const double xs[3],
@@ -214,6 +220,7 @@ int orc_integrate_rk5(int mode, int m, const int dm[3], const double *params,
 
     while (t < tmx) {                                                      /* :129 */
         if (max_steps > 0 && cnt[0] >= max_steps) { status = PTC_STATUS_MAXSTEPS; break; }
+        if (!isfinite(dt)) { status = PTC_STATUS_NONFINITE; break; }
         if (!(dt > 0.0) || t + dt == t) { status = PTC_STATUS_DTSMALL; break; }
 
         for (int j = 0; j < 6; j++) {                                      /* :131-138 */
@@ -362,6 +369,7 @@ int orc_integrate_dp853(int mode, int m, const int dm[3], const double *params,
 
     while (t < tmx) {                                                      /* :361 */
         if (max_steps > 0 && cnt[0] >= max_steps) { status = PTC_STATUS_MAXSTEPS; break; }
+        if (!isfinite(dt)) { status = PTC_STATUS_NONFINITE; break; }
         if (!(dt > 0.0) || t + dt == t) { status = PTC_STATUS_DTSMALL; break; }
 
         RHS(si, k1);                                                       /* :365-416 */

@@ -1,0 +1,223 @@
+"""particles_amd -- MI355X-native integrator for the test-particle hot path
+of gkowal/particles.
+
+Python host mirror of the reference's integration interface, over the C-ABI
+library ``libparticles_amd.so`` (include/particles_amd.h):
+
+  * ``Fields``      -- initialize_fields / finalize_fields
+                       (reference sources/fields.F90:51-101): one field
+                       snapshot uploaded to HBM once;
+  * ``integrate``   -- the particle loop of sources/particles.F90:337-356
+                       calling integrate_rk5 / integrate_dp853
+                       (sources/integrations.F90:57,224) per particle,
+                       with the same arguments and array layouts;
+  * ``Particles``   -- the same integration as a device-resident batch that
+                       advances in segments (snapshot intervals).
+
+Arrays follow the reference's Fortran layouts: fields (dm1, dm2, dm3, 3),
+state (8, m, np), counter (3, np) with (all, accepted, rejected) steps.
+The library must be built (``__graft_entry__.build()``); importing the
+package without it raises, there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+
+import numpy as np
+
+__all__ = ["Fields", "Particles", "integrate", "set_option", "get_option", "PtcError",
+           "METHODS", "MODES", "STATUS", "library_path"]
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+library_path = os.path.join(_HERE, "libparticles_amd.so")
+
+# method strings of sources/particles.F90:338,347
+METHODS = {"rk5": 0, "dop853": 1, "dp853": 1, "rk8": 1}
+MODES = {"vb": 0, "eb": 1}
+PRECS = {"fp64": 0, "fp32": 1}
+STATUS = {-1: "running", 0: "done", 1: "max_steps", 2: "nonfinite", 3: "dt_small"}
+
+
+class PtcError(RuntimeError):
+    """A nonzero return code of the C-ABI, with ptc_last_error()'s message."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[ptc error {code}] {msg}")
+        self.code = code
+
+
+def _load():
+    if not os.path.exists(library_path):
+        raise ImportError(f"{library_path} is not built; run __graft_entry__.build() "
+                          "(make -C particles_amd/csrc)")
+    lib = C.CDLL(library_path)
+    vp, i32, i64, dbl = C.c_void_p, C.c_int, C.c_int64, C.c_double
+    sig = {
+        "ptc_fields_create": ([vp, vp, vp, i32, i32, C.POINTER(vp)], i32),
+        "ptc_fields_destroy": ([vp], i32),
+        "ptc_integrate": ([vp, i32, i64, i32, vp, i32, vp, vp, vp, vp], i32),
+        "ptc_particles_create": ([vp, i32, i64, i32, vp, i32, vp, vp, C.POINTER(vp)], i32),
+        "ptc_particles_advance": ([vp, dbl], i32),
+        "ptc_particles_download": ([vp, vp, vp, vp], i32),
+        "ptc_particles_totals": ([vp, vp], i32),
+        "ptc_particles_last_kernel_ms": ([vp, C.POINTER(dbl)], i32),
+        "ptc_particles_destroy": ([vp], i32),
+        "ptc_set_option": ([C.c_char_p, C.c_char_p], i32),
+        "ptc_get_option": ([C.c_char_p, C.c_char_p, i32], i32),
+        "ptc_last_error": ([], C.c_char_p),
+        "ptc_version": ([], C.c_char_p),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    return lib
+
+
+_lib = _load()
+
+
+def _check(rc: int):
+    if rc != 0:
+        raise PtcError(rc, _lib.ptc_last_error().decode())
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def version() -> str:
+    return _lib.ptc_version().decode()
+
+
+def set_option(key: str, value) -> None:
+    """Process-wide tuning switch (block, grid_blocks, layout, max_steps)."""
+    _check(_lib.ptc_set_option(key.encode(), str(value).encode()))
+
+
+def get_option(key: str) -> str:
+    buf = C.create_string_buffer(128)
+    _check(_lib.ptc_get_option(key.encode(), buf, 128))
+    return buf.value.decode()
+
+
+class Fields:
+    """One field snapshot resident in HBM (reference fields.F90 module
+    arrays uu, bb).  `f1` is U (mode "vb", the reference's formulation) or E
+    (mode "eb"); arrays are (dm1, dm2, dm3, 3) float64."""
+
+    def __init__(self, f1: np.ndarray, bb: np.ndarray, mode: str = "vb", prec: str = "fp64"):
+        f1 = np.asfortranarray(f1, dtype=np.float64)
+        bb = np.asfortranarray(bb, dtype=np.float64)
+        if f1.ndim != 4 or f1.shape[3] != 3 or f1.shape != bb.shape:
+            raise ValueError(f"fields must both be (dm1, dm2, dm3, 3), got {f1.shape} and {bb.shape}")
+        self.dm = np.ascontiguousarray(f1.shape[:3], dtype=np.int32)
+        self.mode, self.prec = mode, prec
+        h = C.c_void_p()
+        _check(_lib.ptc_fields_create(_ptr(self.dm), _ptr(f1), _ptr(bb), MODES[mode], PRECS[prec],
+                                      C.byref(h)))
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if self._h:
+            _check(_lib.ptc_fields_destroy(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+def _prep(params, time, state):
+    params = np.ascontiguousarray(params, dtype=np.float64)
+    time = np.ascontiguousarray(time, dtype=np.float64)
+    state = np.asfortranarray(state, dtype=np.float64)
+    if state.ndim != 3 or state.shape[0] != 8:
+        raise ValueError(f"state must be (8, m, np), got {state.shape}")
+    if state.shape[1] != time.shape[0]:
+        raise ValueError(f"state has {state.shape[1]} snapshots but time has {time.shape[0]}")
+    return params, time, state
+
+
+def integrate(fields: Fields, method: str, params, time, state):
+    """The particle loop of sources/particles.F90:337-356: integrates every
+    particle from t = 0 to params[2] with integrate_rk5 ('rk5') or
+    integrate_dp853 ('dop853', 'dp853', 'rk8').  Returns (state, counter,
+    status): state (8, m, np) with the snapshots filled in, counter (3, np)
+    int64 = (all, accepted, rejected) steps, status (np,) int32."""
+    params, time, state = _prep(params, time, state)
+    state = state.copy(order="F")
+    np_ = state.shape[2]
+    counter = np.zeros((3, np_), dtype=np.int64, order="F")
+    status = np.zeros(np_, dtype=np.int32)
+    _check(_lib.ptc_integrate(fields.handle, METHODS[method], np_, state.shape[1], _ptr(params),
+                              params.size, _ptr(time), _ptr(state), _ptr(counter), _ptr(status)))
+    return state, counter, status
+
+
+class Particles:
+    """Device-resident particle batch advanced in segments; the result of
+    advancing to t1 < t2 < ... < tmax equals one uninterrupted integrate()."""
+
+    def __init__(self, fields: Fields, method: str, params, time, state):
+        params, time, state = _prep(params, time, state)
+        self.fields = fields  # keeps the field handle alive
+        self.m, self.np = state.shape[1], state.shape[2]
+        h = C.c_void_p()
+        _check(_lib.ptc_particles_create(fields.handle, METHODS[method], self.np, self.m,
+                                         _ptr(params), params.size, _ptr(time), _ptr(state),
+                                         C.byref(h)))
+        self._h = h
+
+    def advance(self, t_stop: float = math.inf) -> float:
+        """Integrates until t >= min(t_stop, tmax); returns the integrator
+        kernel's duration in ms (HIP events on the batch's stream)."""
+        _check(_lib.ptc_particles_advance(self._h, float(t_stop)))
+        ms = C.c_double()
+        _check(_lib.ptc_particles_last_kernel_ms(self._h, C.byref(ms)))
+        return ms.value
+
+    def totals(self):
+        """dict of summed counters, reduced on the device."""
+        t = np.zeros(5, dtype=np.int64)
+        _check(_lib.ptc_particles_totals(self._h, _ptr(t)))
+        return dict(all=int(t[0]), accepted=int(t[1]), rejected=int(t[2]), rhs=int(t[3]),
+                    running=int(t[4]))
+
+    def download(self):
+        state = np.zeros((8, self.m, self.np), dtype=np.float64, order="F")
+        counter = np.zeros((3, self.np), dtype=np.int64, order="F")
+        status = np.zeros(self.np, dtype=np.int32)
+        _check(_lib.ptc_particles_download(self._h, _ptr(state), _ptr(counter), _ptr(status)))
+        return state, counter, status
+
+    def close(self):
+        if self._h:
+            _check(_lib.ptc_particles_destroy(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

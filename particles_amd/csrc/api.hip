@@ -1,0 +1,410 @@
+// C-ABI of the integrator (include/particles_amd.h): handles, validation,
+// host<->device transposition and launch bookkeeping.  The compute lives in
+// integrator.hip.
+#include "internal.hpp"
+
+#include "particles_amd.h"
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+using namespace ptc;
+
+struct ptc_fields {
+    DeviceFields d;
+};
+
+struct ptc_particles {
+    const ptc_fields *fields;
+    int method;
+    StepParams sp;
+    DeviceParticles d;
+    void *block;          // one allocation holding every per-particle array
+    double *d_time;
+    int64_t *d_totals;
+    hipStream_t stream;
+    hipEvent_t ev0, ev1;
+    double last_ms;
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char *what)
+{
+    return fail(e == hipErrorOutOfMemory ? PTC_ENOMEM : PTC_EHIP, "%s: %s", what,
+                hipGetErrorString(e));
+}
+
+#define HIP_TRY(expr)                                           \
+    do {                                                        \
+        hipError_t e_ = (expr);                                 \
+        if (e_ != hipSuccess) return hip_fail(e_, #expr);       \
+    } while (0)
+
+struct Options {
+    int block = 256;
+    int grid_blocks = 0;
+    int layout = LAYOUT_AOS;
+    int64_t max_steps = 0;
+};
+
+std::mutex g_opt_mu;
+Options g_opt;
+
+Options options()
+{
+    std::lock_guard<std::mutex> lk(g_opt_mu);
+    return g_opt;
+}
+
+void ok() { g_err.clear(); }
+
+}  // namespace
+
+extern "C" {
+
+const char *ptc_last_error(void) { return g_err.c_str(); }
+
+const char *ptc_version(void) { return "particles_amd 0.1 (gfx950)"; }
+
+int ptc_set_option(const char *key, const char *value)
+{
+    if (!key || !value) return fail(PTC_EARG, "ptc_set_option: null key or value");
+    std::lock_guard<std::mutex> lk(g_opt_mu);
+    char *end = nullptr;
+    if (!strcmp(key, "block")) {
+        long v = strtol(value, &end, 10);
+        if (*end || v < 64 || v > 256 || v % 64)
+            return fail(PTC_EARG, "block must be 64, 128, 192 or 256 (got %s)", value);
+        g_opt.block = (int)v;
+    } else if (!strcmp(key, "grid_blocks")) {
+        long v = strtol(value, &end, 10);
+        if (*end || v < 0 || v > (1 << 20)) return fail(PTC_EARG, "bad grid_blocks %s", value);
+        g_opt.grid_blocks = (int)v;
+    } else if (!strcmp(key, "layout")) {
+        if (!strcmp(value, "soa")) g_opt.layout = LAYOUT_SOA;
+        else if (!strcmp(value, "aos")) g_opt.layout = LAYOUT_AOS;
+        else return fail(PTC_EARG, "layout must be soa or aos (got %s)", value);
+    } else if (!strcmp(key, "max_steps")) {
+        long long v = strtoll(value, &end, 10);
+        if (*end || v < 0) return fail(PTC_EARG, "bad max_steps %s", value);
+        g_opt.max_steps = v;
+    } else {
+        return fail(PTC_EARG, "unknown option %s", key);
+    }
+    ok();
+    return PTC_OK;
+}
+
+int ptc_get_option(const char *key, char *buf, int len)
+{
+    if (!key || !buf || len <= 0) return fail(PTC_EARG, "ptc_get_option: bad arguments");
+    Options o = options();
+    if (!strcmp(key, "block")) snprintf(buf, len, "%d", o.block);
+    else if (!strcmp(key, "grid_blocks")) snprintf(buf, len, "%d", o.grid_blocks);
+    else if (!strcmp(key, "layout")) snprintf(buf, len, "%s", o.layout == LAYOUT_SOA ? "soa" : "aos");
+    else if (!strcmp(key, "max_steps")) snprintf(buf, len, "%lld", (long long)o.max_steps);
+    else return fail(PTC_EARG, "unknown option %s", key);
+    ok();
+    return PTC_OK;
+}
+
+int ptc_fields_create(const int dm[3], const double *f1, const double *bb, int mode, int prec,
+                      ptc_fields **out)
+{
+    if (!out) return fail(PTC_EARG, "ptc_fields_create: out is null");
+    *out = nullptr;
+    if (!dm || !f1 || !bb) return fail(PTC_EARG, "ptc_fields_create: null dm/field pointer");
+    if (dm[0] < 2 || dm[1] < 2 || dm[2] < 2)
+        return fail(PTC_EARG, "ptc_fields_create: dimensions must be >= 2 (got %d %d %d)", dm[0],
+                    dm[1], dm[2]);
+    if (mode != PTC_MODE_VB && mode != PTC_MODE_EB) return fail(PTC_EARG, "bad field mode %d", mode);
+    if (prec != PTC_FP64) return fail(PTC_EARG, "field precision %d not supported", prec);
+    const int64_t nodes = (int64_t)dm[0] * dm[1] * dm[2];
+    if (nodes > (int64_t)0xFFFFFFFFll)
+        return fail(PTC_EARG, "ptc_fields_create: %lld nodes exceed 32-bit node offsets",
+                    (long long)nodes);
+    const Options o = options();
+
+    double *soa = nullptr;
+    HIP_TRY(hipMalloc(&soa, sizeof(double) * 6 * nodes));
+    hipError_t e = hipMemcpy(soa, f1, sizeof(double) * 3 * nodes, hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(soa + 3 * nodes, bb, sizeof(double) * 3 * nodes, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)hipFree(soa);
+        return hip_fail(e, "ptc_fields_create: upload");
+    }
+    void *data = soa;
+    if (o.layout == LAYOUT_AOS) {
+        double *aos = nullptr;
+        e = hipMalloc(&aos, sizeof(double) * 6 * nodes);
+        if (e == hipSuccess) e = launch_to_aos(soa, nodes, aos, PTC_FP64, nullptr);
+        if (e == hipSuccess) e = hipDeviceSynchronize();
+        (void)hipFree(soa);
+        if (e != hipSuccess) {
+            if (aos) (void)hipFree(aos);
+            return hip_fail(e, "ptc_fields_create: interleave");
+        }
+        data = aos;
+    }
+    ptc_fields *f = new ptc_fields;
+    f->d.dm[0] = dm[0];
+    f->d.dm[1] = dm[1];
+    f->d.dm[2] = dm[2];
+    f->d.mode = mode;
+    f->d.prec = prec;
+    f->d.layout = o.layout;
+    f->d.nodes = nodes;
+    f->d.data = data;
+    *out = f;
+    ok();
+    return PTC_OK;
+}
+
+int ptc_fields_destroy(ptc_fields *f)
+{
+    if (!f) return PTC_OK;
+    hipError_t e = hipFree(f->d.data);
+    delete f;
+    if (e != hipSuccess) return hip_fail(e, "ptc_fields_destroy");
+    ok();
+    return PTC_OK;
+}
+
+static int build_step_params(const ptc_fields *f, int m, const double *params, int nparams,
+                             StepParams &sp)
+{
+    if (!params || nparams < 17) return fail(PTC_EARG, "params must have at least 17 entries");
+    for (int i = 0; i < 17; i++)
+        if (!std::isfinite(params[i])) return fail(PTC_EARG, "params(%d) is not finite", i + 1);
+    sp.qom = params[0];
+    sp.vun = params[1];
+    sp.tmx = params[2];
+    sp.dtm = params[4];
+    sp.atol = params[5];
+    sp.rtol = params[6];
+    sp.fcmn = params[7];
+    sp.fcmx = params[8];
+    sp.safe = params[9];
+    sp.beta = params[10];
+    sp.expo = 2.0e-01 * sp.beta - 1.25e-01;  // integrations.F90:348
+    for (int i = 0; i < 3; i++) {
+        sp.xl[i] = params[11 + i];
+        sp.xs[i] = params[14 + i];
+        sp.dm[i] = f->d.dm[i];
+        sp.dmd[i] = (double)f->d.dm[i];
+        sp.inv_dm[i] = 1.0 / (double)f->d.dm[i];
+    }
+    // cases where the reference would never return (no step-size floor)
+    if (!(sp.dtm > 0.0)) return fail(PTC_EARG, "dtmax = params(5) must be > 0 (got %g)", sp.dtm);
+    if (sp.xs[0] == 0.0 || sp.xs[1] == 0.0 || sp.xs[2] == 0.0)
+        return fail(PTC_EARG, "domain size params(15:17) must be nonzero");
+    if (sp.vun == 0.0) return fail(PTC_EARG, "velocity unit params(2) must be nonzero");
+    sp.m = m;
+    sp.max_steps = options().max_steps;
+    return PTC_OK;
+}
+
+int ptc_particles_create(const ptc_fields *f, int method, int64_t np, int m, const double *params,
+                         int nparams, const double *time, const double *state, ptc_particles **out)
+{
+    if (!out) return fail(PTC_EARG, "ptc_particles_create: out is null");
+    *out = nullptr;
+    if (!f) return fail(PTC_EARG, "ptc_particles_create: fields handle is null");
+    if (method != PTC_RK5 && method != PTC_DOP853) return fail(PTC_EARG, "bad method %d", method);
+    if (np < 0 || np > (int64_t)0x7F000000) return fail(PTC_EARG, "bad particle count %lld", (long long)np);
+    if (m < 1) return fail(PTC_EARG, "need at least one snapshot (m = %d)", m);
+    if (np > 0 && (!time || !state)) return fail(PTC_EARG, "null time/state");
+    StepParams sp;
+    int rc = build_step_params(f, m, params, nparams, sp);
+    if (rc) return rc;
+
+    ptc_particles *p = new ptc_particles();
+    p->fields = f;
+    p->method = method;
+    p->sp = sp;
+    p->d.np = np;
+    p->last_ms = 0.0;
+
+    const int64_t n = np > 0 ? np : 1;
+    const size_t bytes = sizeof(double) * (9 * n + (size_t)8 * m * n) + sizeof(int32_t) * 3 * n +
+                         sizeof(int64_t) * 4 * n;
+    hipError_t e = hipMalloc(&p->block, bytes);
+    if (e != hipSuccess) { delete p; return hip_fail(e, "ptc_particles_create: state"); }
+    char *c = (char *)p->block;
+    double *dx = (double *)c;
+    for (int k = 0; k < 6; k++) p->d.x[k] = dx + k * n;
+    p->d.t = dx + 6 * n;
+    p->d.dt = dx + 7 * n;
+    p->d.errold = dx + 8 * n;
+    p->d.out = dx + 9 * n;
+    c += sizeof(double) * (9 * n + (size_t)8 * m * n);
+    p->d.cnt = (int64_t *)c;
+    p->d.nrhs = p->d.cnt + 3 * n;
+    c += sizeof(int64_t) * 4 * n;
+    p->d.nout = (int32_t *)c;
+    p->d.flags = p->d.nout + n;
+    p->d.status = p->d.nout + 2 * n;
+    p->d.order = nullptr;
+
+    e = hipMalloc(&p->d_time, sizeof(double) * m + 64 * sizeof(int64_t));
+    if (e != hipSuccess) { (void)hipFree(p->block); delete p; return hip_fail(e, "ptc_particles_create: time"); }
+    p->d.time = p->d_time;
+    p->d_totals = (int64_t *)(p->d_time + m);
+    p->d.qcount = (int32_t *)(p->d_totals + 8);
+
+    auto cleanup = [&](hipError_t err, const char *what) {
+        (void)hipFree(p->d_time);
+        (void)hipFree(p->block);
+        delete p;
+        return hip_fail(err, what);
+    };
+    e = hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) return cleanup(e, "hipStreamCreate");
+    if ((e = hipEventCreate(&p->ev0)) != hipSuccess || (e = hipEventCreate(&p->ev1)) != hipSuccess)
+        return cleanup(e, "hipEventCreate");
+
+    // initial state: SoA copy of state(1:6,1,:), full state(8,m,np) image
+    if (np > 0) {
+        std::vector<double> h(9 * (size_t)np);
+        for (int64_t i = 0; i < np; i++) {
+            const double *s = state + (size_t)8 * m * i;
+            for (int k = 0; k < 6; k++) h[k * np + i] = s[k];
+            h[6 * np + i] = 0.0;        // t
+            h[7 * np + i] = 0.0;        // dt
+            h[8 * np + i] = 1.0e-04;    // errold (integrations.F90:349)
+        }
+        e = hipMemcpy(dx, h.data(), sizeof(double) * 9 * np, hipMemcpyHostToDevice);
+        if (e == hipSuccess)
+            e = hipMemcpy(p->d.out, state, sizeof(double) * 8 * m * np, hipMemcpyHostToDevice);
+        std::vector<int32_t> hi(3 * (size_t)np);
+        for (int64_t i = 0; i < np; i++) {
+            hi[i] = 2;                            // next snapshot (1-based)
+            hi[np + i] = 0;                       // flags
+            hi[2 * np + i] = PTC_STATUS_RUNNING;  // status
+        }
+        if (e == hipSuccess)
+            e = hipMemcpy(p->d.nout, hi.data(), sizeof(int32_t) * 3 * np, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemset(p->d.cnt, 0, sizeof(int64_t) * 4 * np);
+        if (e == hipSuccess)
+            e = hipMemcpy(p->d_time, time, sizeof(double) * m, hipMemcpyHostToDevice);
+        if (e != hipSuccess) return cleanup(e, "ptc_particles_create: upload");
+    }
+    *out = p;
+    ok();
+    return PTC_OK;
+}
+
+int ptc_particles_advance(ptc_particles *p, double t_stop)
+{
+    if (!p) return fail(PTC_EARG, "ptc_particles_advance: null handle");
+    if (std::isnan(t_stop)) return fail(PTC_EARG, "ptc_particles_advance: t_stop is NaN");
+    p->last_ms = 0.0;
+    if (p->d.np == 0) { ok(); return PTC_OK; }
+    const Options o = options();
+    p->sp.max_steps = o.max_steps;
+    LaunchConfig lc{o.block, o.grid_blocks};
+    HIP_TRY(launch_zero_i32(p->d.qcount, 1, p->stream));
+    HIP_TRY(hipEventRecord(p->ev0, p->stream));
+    HIP_TRY(launch_integrate(p->method, p->fields->d, p->sp, p->d, t_stop, lc, p->stream));
+    HIP_TRY(hipEventRecord(p->ev1, p->stream));
+    HIP_TRY(hipStreamSynchronize(p->stream));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, p->ev0, p->ev1));
+    p->last_ms = ms;
+    ok();
+    return PTC_OK;
+}
+
+int ptc_particles_download(ptc_particles *p, double *state, int64_t *counter, int32_t *status)
+{
+    if (!p) return fail(PTC_EARG, "ptc_particles_download: null handle");
+    const int64_t np = p->d.np;
+    if (np == 0) { ok(); return PTC_OK; }
+    HIP_TRY(hipStreamSynchronize(p->stream));
+    if (state)
+        HIP_TRY(hipMemcpy(state, p->d.out, sizeof(double) * 8 * (size_t)p->sp.m * np,
+                          hipMemcpyDeviceToHost));
+    if (counter) {
+        std::vector<int64_t> h(3 * (size_t)np);
+        HIP_TRY(hipMemcpy(h.data(), p->d.cnt, sizeof(int64_t) * 3 * np, hipMemcpyDeviceToHost));
+        for (int64_t i = 0; i < np; i++)
+            for (int k = 0; k < 3; k++) counter[3 * i + k] = h[k * np + i];
+    }
+    if (status)
+        HIP_TRY(hipMemcpy(status, p->d.status, sizeof(int32_t) * np, hipMemcpyDeviceToHost));
+    ok();
+    return PTC_OK;
+}
+
+int ptc_particles_totals(ptc_particles *p, int64_t totals[5])
+{
+    if (!p || !totals) return fail(PTC_EARG, "ptc_particles_totals: null argument");
+    for (int i = 0; i < 5; i++) totals[i] = 0;
+    if (p->d.np == 0) { ok(); return PTC_OK; }
+    HIP_TRY(launch_totals(p->d, p->d_totals, p->stream));
+    HIP_TRY(hipMemcpyAsync(totals, p->d_totals, sizeof(int64_t) * 5, hipMemcpyDeviceToHost, p->stream));
+    HIP_TRY(hipStreamSynchronize(p->stream));
+    ok();
+    return PTC_OK;
+}
+
+int ptc_particles_last_kernel_ms(const ptc_particles *p, double *ms)
+{
+    if (!p || !ms) return fail(PTC_EARG, "ptc_particles_last_kernel_ms: null argument");
+    *ms = p->last_ms;
+    ok();
+    return PTC_OK;
+}
+
+int ptc_particles_destroy(ptc_particles *p)
+{
+    if (!p) return PTC_OK;
+    (void)hipStreamSynchronize(p->stream);
+    (void)hipEventDestroy(p->ev0);
+    (void)hipEventDestroy(p->ev1);
+    (void)hipStreamDestroy(p->stream);
+    hipError_t e1 = hipFree(p->d_time);
+    hipError_t e2 = hipFree(p->block);
+    delete p;
+    if (e1 != hipSuccess) return hip_fail(e1, "ptc_particles_destroy");
+    if (e2 != hipSuccess) return hip_fail(e2, "ptc_particles_destroy");
+    ok();
+    return PTC_OK;
+}
+
+int ptc_integrate(const ptc_fields *f, int method, int64_t np, int m, const double *params,
+                  int nparams, const double *time, double *state, int64_t *counter, int32_t *status)
+{
+    ptc_particles *p = nullptr;
+    int rc = ptc_particles_create(f, method, np, m, params, nparams, time, state, &p);
+    if (rc) return rc;
+    rc = ptc_particles_advance(p, INFINITY);
+    if (!rc) rc = ptc_particles_download(p, state, counter, status);
+    std::string keep = g_err;
+    int rc2 = ptc_particles_destroy(p);
+    if (rc) { g_err = keep; return rc; }
+    return rc2;
+}
+
+}  // extern "C"

@@ -1,0 +1,73 @@
+// Internal types shared by the C-ABI layer (api.hip) and the integrator
+// kernels (integrator.hip).  Not part of the public interface.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ptc {
+
+enum Layout : int { LAYOUT_SOA = 0, LAYOUT_AOS = 1 };
+
+// Device copy of one field snapshot.
+//   SOA: six planes of dm1*dm2*dm3 values (U/E x,y,z then B x,y,z), each x
+//        fastest -- the reference's uu(:,:,:,c) / bb(:,:,:,c) planes.
+//   AOS: one array of nodes, six values per node (U/E x,y,z, B x,y,z), so
+//        the eight corners of a cell are eight 48-byte rows.
+struct DeviceFields {
+    int dm[3];
+    int mode;     // PTC_MODE_VB / PTC_MODE_EB
+    int prec;     // PTC_FP64 / PTC_FP32
+    int layout;   // Layout
+    int64_t nodes;
+    void *data;   // SOA: 6 * nodes values; AOS: nodes * 6 values
+};
+
+// Geometry and controller constants of one integration (params vector of
+// sources/particles.F90:317-333, plus derived values).
+struct StepParams {
+    double qom, vun, tmx, dtm, atol, rtol, fcmn, fcmx, safe, beta, expo;
+    double xl[3], xs[3];
+    double inv_dm[3];  // 1.0 / dm, computed like sources/interpolations.F90:141
+    double dmd[3];     // dm as double
+    int dm[3];
+    int m;             // number of snapshots
+    int64_t max_steps; // 0 = no cap
+};
+
+// Structure-of-arrays particle state resident in HBM.
+struct DeviceParticles {
+    int64_t np;
+    double *x[6];        // current state si (position, momentum)
+    double *t, *dt, *errold;
+    int32_t *nout;       // next snapshot index (1-based, as in the reference)
+    int32_t *flags;      // bit 0: last attempt rejected; bit 1: started
+    int32_t *status;     // PTC_STATUS_*
+    int64_t *cnt;        // 3 planes of np: all, accepted, rejected
+    int64_t *nrhs;       // RHS evaluations per particle (diagnostics)
+    double *out;         // state(8, m, np), Fortran order
+    const double *time;  // time(m)
+    const int32_t *order;// work-queue permutation (nullptr = identity)
+    int32_t *qcount;     // work-queue head
+};
+
+enum : int32_t { FLAG_REJECTED = 1, FLAG_STARTED = 2 };
+
+struct LaunchConfig {
+    int block;
+    int grid_blocks;   // 0 = occupancy-derived
+};
+
+// Launches the integrator on `stream`; returns a hipError_t.
+hipError_t launch_integrate(int method, const DeviceFields &f, const StepParams &sp,
+                            const DeviceParticles &p, double t_stop, const LaunchConfig &lc,
+                            hipStream_t stream);
+// Sums counters on the device: out[0..4] = all, accepted, rejected, rhs, running.
+hipError_t launch_totals(const DeviceParticles &p, int64_t *d_out, hipStream_t stream);
+// Zeroes n int32 values (graph-safe replacement for hipMemsetAsync).
+hipError_t launch_zero_i32(int32_t *ptr, int64_t n, hipStream_t stream);
+hipError_t launch_zero_i64(int64_t *ptr, int64_t n, hipStream_t stream);
+// SOA planes -> AOS nodes (and FP64 -> FP32 when prec is PTC_FP32).
+hipError_t launch_to_aos(const double *soa, int64_t nodes, void *aos, int prec, hipStream_t stream);
+
+}  // namespace ptc

@@ -1,0 +1,203 @@
+"""Device integrator (HIP, gfx950) against the reference's golden vectors and
+the CPU oracle on the same seeded inputs.
+
+Bar: the device path runs the reference's FP64 operation order without FMA
+contraction, so a particle's trajectory and step counters are bit-identical
+to the reference's unless pow() (the step-size factor) rounds differently
+from the host libm; from there on the trajectories stay within the
+tolerance written in each test (1e-8 relative in position and momentum).
+"""
+import math
+
+import numpy as np
+import pytest
+
+import particles_amd as P
+from conftest import golden_cases, load_golden
+from particles_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+PARITY_TOL = 1e-8
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def deviation(a, b):
+    """Per-particle max over snapshots of |a - b| / max(|b|, 1), for the
+    position and the momentum vector (state rows 1:3 and 4:6)."""
+    d = a[:6] - b[:6]
+    pos = np.sqrt((d[:3] ** 2).sum(0)) / np.maximum(np.sqrt((b[:3] ** 2).sum(0)), 1.0)
+    mom = np.sqrt((d[3:6] ** 2).sum(0)) / np.maximum(np.sqrt((b[3:6] ** 2).sum(0)), 1.0)
+    return np.maximum(pos, mom).max(0)
+
+
+def exact_particles(sa, ca, sb, cb):
+    return np.array([np.array_equal(sa[..., i], sb[..., i]) and np.array_equal(ca[:, i], cb[:, i])
+                     for i in range(sa.shape[2])])
+
+
+@pytest.fixture(params=["aos", "soa"])
+def layout(request):
+    old = P.get_option("layout")
+    P.set_option("layout", request.param)
+    yield request.param
+    P.set_option("layout", old)
+
+
+@pytest.mark.parametrize("case", golden_cases())
+def test_device_matches_reference_golden(case, layout):
+    g = load_golden(case)
+    say(f"[golden] {case} layout={layout} np={g['state0'].shape[2]}")
+    with P.Fields(g["uu"], g["bb"]) as f:
+        st, cnt, status = P.integrate(f, g["method"], g["params"], g["time"], g["state0"])
+    assert (status == 0).all()
+    ex = exact_particles(st, cnt, g["state"], g["counter"])
+    dev = deviation(st, g["state"])
+    say(f"  bit-exact particles {ex.sum()}/{len(ex)}, max deviation {dev.max():.3e}")
+    assert dev.max() <= PARITY_TOL
+    # initial row: gamma and the initial step (dti_guess) are exact
+    np.testing.assert_array_equal(st[6:8, 0, :], g["state"][6:8, 0, :])
+
+
+@pytest.mark.parametrize("method", ["rk5", "dop853"])
+def test_device_matches_oracle_turbulent_10_gyroperiods(oracle, method):
+    """Turbulent-box recipe (B0 = 1, dB rms 0.5, V rms 0.1, 256 modes with
+    |k| in [2 pi, 16 pi]) on a 32^3 grid, 10 gyro-periods of B0 at tol 1e-8."""
+    uu, bb = W.turbulent_vb(32, seed=1)
+    p = W.params_vector(tmax=10 * W.T0, dtmax=1.0, atol=1e-8, rtol=1e-8)
+    tg = W.time_grid(10, p[2])
+    st0 = W.initial_state(192, len(tg), seed=2, gamma_range=(1.0, 1.0e3))
+    say(f"[oracle] turbulent 32^3 {method} np=192 t=10 T0")
+    so, co, _ = oracle.integrate(method, uu, bb, p, tg, st0)
+    with P.Fields(uu, bb) as f:
+        st, cnt, status = P.integrate(f, method, p, tg, st0)
+    assert (status == 0).all()
+    ex = exact_particles(st, cnt, so, co)
+    dev = deviation(st, so)
+    say(f"  bit-exact {ex.sum()}/{len(ex)}, max deviation {dev.max():.3e}, "
+        f"accepted {cnt[1].sum()} vs {co[1].sum()}")
+    assert dev.max() <= PARITY_TOL
+
+
+def test_eb_mode_matches_oracle(oracle):
+    ee, bb = W.reconnection_eb(32, seed=3, nmodes=64)
+    p = W.params_vector(tmax=2 * W.T0, dtmax=1.0, atol=1e-8, rtol=1e-8)
+    tg = W.time_grid(4, p[2])
+    st0 = W.initial_state(128, len(tg), seed=4, gamma_range=(1.0, 1.0e2))
+    for method in ("rk5", "dop853"):
+        say(f"[eb] reconnection 32^3 {method}")
+        so, co, _ = oracle.integrate(method, ee, bb, p, tg, st0, mode="eb")
+        with P.Fields(ee, bb, mode="eb") as f:
+            st, cnt, status = P.integrate(f, method, p, tg, st0)
+        dev = deviation(st, so)
+        say(f"  max deviation {dev.max():.3e}")
+        assert (status == 0).all() and dev.max() <= PARITY_TOL
+
+
+@pytest.mark.parametrize("method", ["rk5", "dop853"])
+def test_segments_equal_one_call(method):
+    """Advancing in snapshot-interval segments continues each trajectory
+    exactly: bit-identical to one uninterrupted integration."""
+    uu, bb = W.turbulent_vb(16, seed=7, nmodes=64)
+    p = W.params_vector(tmax=3 * W.T0, dtmax=1.0, atol=1e-8, rtol=1e-8)
+    tg = W.time_grid(6, p[2])
+    st0 = W.initial_state(300, len(tg), seed=8, gamma_range=(1.0, 100.0))
+    say(f"[segments] {method}")
+    with P.Fields(uu, bb) as f:
+        s1, c1, _ = P.integrate(f, method, p, tg, st0)
+        with P.Particles(f, method, p, tg, st0) as b:
+            for t_stop in np.linspace(0, p[2], 8)[1:-1]:
+                b.advance(t_stop)
+                assert b.totals()["running"] > 0
+            b.advance()
+            tot = b.totals()
+            s2, c2, st2 = b.download()
+    assert np.array_equal(s1, s2) and np.array_equal(c1, c2)
+    assert (st2 == 0).all() and tot["running"] == 0
+    assert tot["accepted"] == c1[1].sum() and tot["rejected"] == c1[2].sum()
+    assert tot["all"] == c1[0].sum()
+
+
+def test_launch_shape_does_not_change_results():
+    """The work queue hands particles to lanes in a launch-dependent order;
+    results must not depend on it."""
+    uu, bb = W.turbulent_vb(16, seed=9, nmodes=64)
+    p = W.params_vector(tmax=1 * W.T0, dtmax=1.0, atol=1e-8, rtol=1e-8)
+    tg = W.time_grid(3, p[2])
+    st0 = W.initial_state(1000, len(tg), seed=10, gamma_range=(1.0, 1000.0))
+    outs = []
+    try:
+        for block, grid in ((256, 0), (64, 3), (128, 1)):
+            P.set_option("block", block)
+            P.set_option("grid_blocks", grid)
+            say(f"[launch] block={block} grid_blocks={grid}")
+            with P.Fields(uu, bb) as f:
+                outs.append(P.integrate(f, "dop853", p, tg, st0))
+    finally:
+        P.set_option("block", 256)
+        P.set_option("grid_blocks", 0)
+    for o in outs[1:]:
+        assert np.array_equal(o[0], outs[0][0]) and np.array_equal(o[1], outs[0][1])
+
+
+def test_nonfinite_field_and_caps():
+    """Failure handling forced from the host: a NaN field node retires the
+    particles that meet it (status 2), a step cap retires with status 1,
+    others finish normally; every launch returns."""
+    dm = (8, 8, 8)
+    uu = np.zeros(dm + (3,), order="F")
+    bb = np.zeros(dm + (3,), order="F")
+    bb[..., 2] = 1.0
+    bb[2, 4, 6, 0] = np.nan
+    p = W.params_vector(tmax=20.0, dtmax=1.0, atol=1e-8, rtol=1e-8)
+    tg = W.time_grid(2, p[2])
+    st0 = np.zeros((8, len(tg), 3), order="F")
+    st0[:6, 0, 0] = [0.27, 0.52, 0.77, 0.5, 0.0, 0.0]   # next to the NaN node
+    st0[:6, 0, 1] = [0.9, 0.1, 0.1, 0.1, 0.0, 0.0]      # far away, small orbit
+    st0[:6, 0, 2] = [0.9, 0.1, 0.2, 0.1, 0.0, 0.0]
+    say("[failure] NaN node")
+    with P.Fields(uu, bb) as f:
+        _, cnt, status = P.integrate(f, "dop853", p, tg, st0)
+        assert status.tolist() == [2, 0, 0]
+        try:
+            P.set_option("max_steps", 25)
+            _, cnt, status = P.integrate(f, "rk5", p, tg, st0)
+        finally:
+            P.set_option("max_steps", 0)
+        assert status[1] == 1 and cnt[0, 1] == 25
+        bad = p.copy()
+        bad[4] = 0.0  # dtmax <= 0 would never terminate in the reference
+        with pytest.raises(P.PtcError) as e:
+            P.integrate(f, "rk5", bad, tg, st0)
+        assert e.value.code == 1
+
+
+def test_empty_and_single_snapshot():
+    uu, bb = W.turbulent_vb(8, seed=1, nmodes=16)
+    p = W.params_vector(tmax=1.0, dtmax=0.5)
+    with P.Fields(uu, bb) as f:
+        st, cnt, status = P.integrate(f, "rk5", p, np.zeros(1), np.zeros((8, 1, 0), order="F"))
+        assert st.shape == (8, 1, 0)
+        st0 = W.initial_state(5, 1, seed=3, gamma=3.0)
+        st, cnt, status = P.integrate(f, "dop853", p, np.zeros(1), st0)
+        assert (status == 0).all() and (cnt[1] > 0).all()
+        np.testing.assert_allclose(st[6, 0], 3.0, rtol=1e-14)
+
+
+def test_energy_conserved_without_electric_field():
+    """U = 0 (E = 0): gamma is invariant; the device run keeps it to the
+    tolerance level at the snapshots (DOP853, cubic dense output)."""
+    _, bb = W.turbulent_vb(24, seed=2)
+    uu = np.zeros_like(bb, order="F")
+    p = W.params_vector(tmax=10 * W.T0, dtmax=0.5, atol=1e-10, rtol=1e-10)
+    tg = W.time_grid(10, p[2])
+    st0 = W.initial_state(256, len(tg), seed=5, gamma_range=(2.0, 100.0))
+    with P.Fields(uu, bb) as f:
+        st, _, status = P.integrate(f, "dop853", p, tg, st0)
+    g0 = st[6, 0, :]
+    rel = np.abs(st[6, 1:, :] - g0) / g0
+    say(f"[energy] max relative gamma drift {rel.max():.3e}")
+    assert (status == 0).all() and rel.max() < 1e-7
