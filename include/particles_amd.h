@@ -103,6 +103,24 @@ int ptc_particles_totals(ptc_particles *p, int64_t totals[5]);
 int ptc_particles_last_kernel_ms(const ptc_particles *p, double *ms);
 int ptc_particles_destroy(ptc_particles *p);
 
+/* Evaluates one building block of the device integrator for n inputs, so
+ * each can be checked on its own against interpolate_lin
+ * (sources/interpolations.F90:104), acceleration (sources/accelerations.F90:59)
+ * and dti_guess (sources/integrations.F90:524):
+ *   PTC_PROBE_GATHER : in pos(3,n)        -> out (6,n) = (U or E, B)
+ *   PTC_PROBE_RHS    : in vec(6,n)        -> out (6,n) = d(vec)/dt
+ *   PTC_PROBE_DTI5/8 : in vec(6,n)        -> out (n)   = initial step, order 5/8
+ *   PTC_PROBE_POW    : in (x,y)(2,n)      -> out (n)   = x**y as the step
+ *                                                       controller computes it
+ * params as for ptc_integrate (geometry, units, tolerances); host arrays. */
+#define PTC_PROBE_GATHER 0
+#define PTC_PROBE_RHS    1
+#define PTC_PROBE_DTI5   2
+#define PTC_PROBE_DTI8   3
+#define PTC_PROBE_POW    4
+int ptc_probe(const ptc_fields *f, int op, int64_t n, const double *params, int nparams,
+              const double *in, double *out);
+
 /* Tuning and behaviour switches (process-wide), e.g.
  *   "block"       threads per block (64..1024, multiple of 64), default 256
  *   "grid_blocks" blocks of the persistent kernel (0 = fill the device)
@@ -119,6 +137,12 @@ const char *ptc_last_error(void);
 
 /* Library version string. */
 const char *ptc_version(void);
+
+/* x ** y computed on the host by the same code the device step controller
+ * runs (PTC_PROBE_POW evaluates it on the device); it reproduces the host
+ * libm's pow, which the reference's real ** real calls
+ * (sources/integrations.F90:177,186,466,486,581). */
+double ptc_pow_host(double x, double y);
 
 #ifdef __cplusplus
 }

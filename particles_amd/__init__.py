@@ -64,6 +64,7 @@ def _load():
         "ptc_particles_totals": ([vp, vp], i32),
         "ptc_particles_last_kernel_ms": ([vp, C.POINTER(dbl)], i32),
         "ptc_particles_destroy": ([vp], i32),
+        "ptc_probe": ([vp, i32, i64, vp, i32, vp, vp], i32),
         "ptc_set_option": ([C.c_char_p, C.c_char_p], i32),
         "ptc_get_option": ([C.c_char_p, C.c_char_p, i32], i32),
         "ptc_last_error": ([], C.c_char_p),
@@ -140,6 +141,24 @@ class Fields:
 
     def __exit__(self, *a):
         self.close()
+
+
+_PROBES = {"gather": (0, 3, 6), "rhs": (1, 6, 6), "dti5": (2, 6, 1), "dti8": (3, 6, 1),
+           "pow": (4, 2, 1)}
+
+
+def probe(fields: Fields, op: str, inputs, params) -> np.ndarray:
+    """Evaluates one device building block per input row: "gather" (pos ->
+    U or E, B; interpolate_lin), "rhs" (vec -> d vec/dt; acceleration),
+    "dti5"/"dti8" (vec -> initial step; dti_guess), "pow" ((x, y) -> x**y as
+    the step controller computes it).  inputs is (n, k); returns (n, 6) or (n,)."""
+    code, nin, nout = _PROBES[op]
+    x = np.ascontiguousarray(inputs, dtype=np.float64).reshape(-1, nin)
+    params = np.ascontiguousarray(params, dtype=np.float64)
+    out = np.zeros((x.shape[0], nout), dtype=np.float64)
+    _check(_lib.ptc_probe(fields.handle, code, x.shape[0], _ptr(params), params.size, _ptr(x),
+                          _ptr(out)))
+    return out if nout > 1 else out[:, 0]
 
 
 def _prep(params, time, state):

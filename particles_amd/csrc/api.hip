@@ -2,6 +2,7 @@
 // host<->device transposition and launch bookkeeping.  The compute lives in
 // integrator.hip.
 #include "internal.hpp"
+#include "libm_pow.hpp"
 
 #include "particles_amd.h"
 
@@ -85,6 +86,8 @@ extern "C" {
 const char *ptc_last_error(void) { return g_err.c_str(); }
 
 const char *ptc_version(void) { return "particles_amd 0.1 (gfx950)"; }
+
+double ptc_pow_host(double x, double y) { return ptc_libm_pow(x, y); }
 
 int ptc_set_option(const char *key, const char *value)
 {
@@ -389,6 +392,32 @@ int ptc_particles_destroy(ptc_particles *p)
     delete p;
     if (e1 != hipSuccess) return hip_fail(e1, "ptc_particles_destroy");
     if (e2 != hipSuccess) return hip_fail(e2, "ptc_particles_destroy");
+    ok();
+    return PTC_OK;
+}
+
+int ptc_probe(const ptc_fields *f, int op, int64_t n, const double *params, int nparams,
+              const double *in, double *out)
+{
+    if (!f) return fail(PTC_EARG, "ptc_probe: fields handle is null");
+    if (op < PTC_PROBE_GATHER || op > PTC_PROBE_POW) return fail(PTC_EARG, "ptc_probe: bad op %d", op);
+    if (n < 0) return fail(PTC_EARG, "ptc_probe: negative count");
+    if (n == 0) { ok(); return PTC_OK; }
+    if (!in || !out) return fail(PTC_EARG, "ptc_probe: null in/out");
+    StepParams sp;
+    int rc = build_step_params(f, 1, params, nparams, sp);
+    if (rc) return rc;
+    const int nin = op == PTC_PROBE_GATHER ? 3 : op == PTC_PROBE_POW ? 2 : 6;
+    const int nout = (op == PTC_PROBE_GATHER || op == PTC_PROBE_RHS) ? 6 : 1;
+    double *d = nullptr;
+    HIP_TRY(hipMalloc(&d, sizeof(double) * (nin + nout) * n));
+    hipError_t e = hipMemcpy(d, in, sizeof(double) * nin * n, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = launch_probe(f->d, sp, op, n, d, d + nin * n, nullptr);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e == hipSuccess)
+        e = hipMemcpy(out, d + nin * n, sizeof(double) * nout * n, hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    if (e != hipSuccess) return hip_fail(e, "ptc_probe");
     ok();
     return PTC_OK;
 }

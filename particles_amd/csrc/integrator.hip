@@ -11,9 +11,10 @@
 // The arithmetic restates the reference per particle (gkowal/particles,
 // sources/integrations.F90, accelerations.F90, interpolations.F90) in its
 // operation order; this file is compiled with -ffp-contract=off so every
-// FP64 add/mul/div/sqrt rounds like the reference's own build.  Results then
-// match the reference bit for bit except where pow() (the step-size factor)
-// rounds differently from the host libm.
+// FP64 add/mul/div/sqrt rounds like the reference's own build, and the
+// step-size powers use ptc_libm_pow (libm_pow.hpp), which reproduces the
+// host libm's pow bit for bit.  Trajectories and step counters then match
+// the reference bit for bit.
 //
 // Redundant evaluations of the reference are not repeated, which leaves every
 // result unchanged:
@@ -24,6 +25,7 @@
 //   * dti_guess evaluates f at a trial point and never uses it (:576);
 //   * a rejected attempt reuses f(si) (stage 1 does not depend on dt).
 #include "internal.hpp"
+#include "libm_pow.hpp"
 
 #include "particles_amd.h"
 
@@ -179,7 +181,7 @@ __device__ __forceinline__ double dti_guess(const StepParams &P, int iord, const
         dtf = 1.0e-03 * dti;
         if (dtf < 1.0e-06) dtf = 1.0e-06;
     } else {
-        dtf = pow(1.0e-02 / d2, 1.0 / (double)iord);
+        dtf = ptc_libm_pow(1.0e-02 / d2, 1.0 / (double)iord);
     }
     dti = 1.0e+02 * dti;
     if (dtf < dti) dti = dtf;
@@ -586,12 +588,12 @@ integrate_kernel(const DeviceFields F, const StepParams P, const DeviceParticles
 #pragma unroll
                         for (int c = 0; c < 6; c++) si[c] = sf[c];
                         c_acc++;
-                        const double q = P.safe * pow(del, -0.20);
+                        const double q = P.safe * ptc_libm_pow(del, -0.20);
                         dtn = dt * (P.fcmx < q ? P.fcmx : q);
                         need_rhs = true;
                     } else {
                         c_rej++;
-                        const double q = P.safe * pow(del, -0.25);
+                        const double q = P.safe * ptc_libm_pow(del, -0.25);
                         dtn = dt * (P.fcmn > q ? P.fcmn : q);
                     }
                 } else {
@@ -624,7 +626,7 @@ integrate_kernel(const DeviceFields F, const StepParams P, const DeviceParticles
 #pragma unroll
                         for (int c = 0; c < 6; c++) si[c] = sf[c];
                         c_acc++;
-                        double q = P.safe * pow(err, P.expo) * pow(errold, P.beta);
+                        double q = P.safe * ptc_libm_pow(err, P.expo) * ptc_libm_pow(errold, P.beta);
                         if (q < P.fcmn) q = P.fcmn;
                         if (flags & FLAG_REJECTED) dtn = dt * (q < 1.0 ? q : 1.0);
                         else dtn = dt * (q < P.fcmx ? q : P.fcmx);
@@ -632,7 +634,7 @@ integrate_kernel(const DeviceFields F, const StepParams P, const DeviceParticles
                         flags &= ~FLAG_REJECTED;
                     } else {
                         c_rej++;
-                        const double q = P.safe * pow(err, P.expo);
+                        const double q = P.safe * ptc_libm_pow(err, P.expo);
                         dtn = dt * (P.fcmn > q ? P.fcmn : q);
                         flags |= FLAG_REJECTED;
                     }
@@ -665,6 +667,50 @@ integrate_kernel(const DeviceFields F, const StepParams P, const DeviceParticles
             need_rhs = false;
         }
     }
+}
+
+// Single-call probes of the device building blocks, one input per thread:
+//   op 0: gather at pos(3)          -> f(6)  (U or E, then B)
+//   op 1: RHS at vec(6)             -> acc(6)
+//   op 2: dti_guess, order 5, vec(6) -> dt(1) (op 3: order 8)
+//   op 4: step-size power x^y, (x, y) -> (1)
+template <int MODE, int LAYOUT>
+__global__ void probe_kernel(const DeviceFields F, const StepParams P, int op, int64_t n,
+                             const double *__restrict__ in, double *__restrict__ out)
+{
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double v[6], r[6];
+    if (op == 0) {
+#pragma unroll
+        for (int k = 0; k < 3; k++) v[k] = in[3 * i + k];
+        gather<LAYOUT>(F, P, v, r);
+#pragma unroll
+        for (int k = 0; k < 6; k++) out[6 * i + k] = r[k];
+    } else if (op == 1) {
+#pragma unroll
+        for (int k = 0; k < 6; k++) v[k] = in[6 * i + k];
+        rhs<MODE, LAYOUT>(F, P, v, r);
+#pragma unroll
+        for (int k = 0; k < 6; k++) out[6 * i + k] = r[k];
+    } else if (op == 2 || op == 3) {
+#pragma unroll
+        for (int k = 0; k < 6; k++) v[k] = in[6 * i + k];
+        rhs<MODE, LAYOUT>(F, P, v, r);
+        out[i] = dti_guess(P, op == 2 ? 5 : 8, v, r);
+    } else if (op == 4) {
+        out[i] = ptc_libm_pow(in[2 * i], in[2 * i + 1]);
+    }
+}
+
+template <int MODE, int LAYOUT>
+hipError_t launch_probe_t(const DeviceFields &f, const StepParams &sp, int op, int64_t n,
+                          const double *in, double *out, hipStream_t stream)
+{
+    const int64_t blocks = (n + 255) / 256;
+    hipLaunchKernelGGL((probe_kernel<MODE, LAYOUT>), dim3((unsigned)blocks), dim3(256), 0, stream, f,
+                       sp, op, n, in, out);
+    return hipGetLastError();
 }
 
 __global__ void totals_kernel(const DeviceParticles S, unsigned long long *out)
@@ -762,6 +808,20 @@ hipError_t launch_integrate(int method, const DeviceFields &f, const StepParams 
     DISPATCH(PTC_DOP853, PTC_MODE_EB, LAYOUT_AOS)
 #undef DISPATCH
     return hipErrorInvalidValue;
+}
+
+hipError_t launch_probe(const DeviceFields &f, const StepParams &sp, int op, int64_t n,
+                        const double *in, double *out, hipStream_t stream)
+{
+    if (n <= 0) return hipSuccess;
+    if (f.prec != PTC_FP64 || op < 0 || op > 4) return hipErrorInvalidValue;
+    if (f.mode == PTC_MODE_VB && f.layout == LAYOUT_SOA)
+        return launch_probe_t<PTC_MODE_VB, LAYOUT_SOA>(f, sp, op, n, in, out, stream);
+    if (f.mode == PTC_MODE_VB && f.layout == LAYOUT_AOS)
+        return launch_probe_t<PTC_MODE_VB, LAYOUT_AOS>(f, sp, op, n, in, out, stream);
+    if (f.mode == PTC_MODE_EB && f.layout == LAYOUT_SOA)
+        return launch_probe_t<PTC_MODE_EB, LAYOUT_SOA>(f, sp, op, n, in, out, stream);
+    return launch_probe_t<PTC_MODE_EB, LAYOUT_AOS>(f, sp, op, n, in, out, stream);
 }
 
 hipError_t launch_totals(const DeviceParticles &p, int64_t *d_out, hipStream_t stream)

@@ -62,6 +62,9 @@ struct LaunchConfig {
 hipError_t launch_integrate(int method, const DeviceFields &f, const StepParams &sp,
                             const DeviceParticles &p, double t_stop, const LaunchConfig &lc,
                             hipStream_t stream);
+// Evaluates one building block per input on the device (ptc_probe).
+hipError_t launch_probe(const DeviceFields &f, const StepParams &sp, int op, int64_t n,
+                        const double *in, double *out, hipStream_t stream);
 // Sums counters on the device: out[0..4] = all, accepted, rejected, rhs, running.
 hipError_t launch_totals(const DeviceParticles &p, int64_t *d_out, hipStream_t stream);
 // Zeroes n int32 values (graph-safe replacement for hipMemsetAsync).
