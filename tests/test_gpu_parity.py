@@ -236,9 +236,10 @@ def test_empty_and_single_snapshot():
         np.testing.assert_allclose(st[6, 0], 3.0, rtol=1e-14)
 
 
-def test_energy_conserved_without_electric_field():
-    """U = 0 (E = 0): gamma is invariant; the device run keeps it to the
-    tolerance level at the snapshots (DOP853, cubic dense output)."""
+def test_energy_conserved_without_electric_field(oracle):
+    """U = 0 (E = 0): gamma is invariant.  The snapshots come from the cubic
+    dense output, so they hold it to the tolerance level only; the device
+    run must show exactly the oracle's (reference's) drift and stay small."""
     _, bb = W.turbulent_vb(24, seed=2)
     uu = np.zeros_like(bb, order="F")
     p = W.params_vector(tmax=10 * W.T0, dtmax=0.5, atol=1e-10, rtol=1e-10)
@@ -246,7 +247,9 @@ def test_energy_conserved_without_electric_field():
     st0 = W.initial_state(256, len(tg), seed=5, gamma_range=(2.0, 100.0))
     with P.Fields(uu, bb) as f:
         st, _, status = P.integrate(f, "dop853", p, tg, st0)
+    so, _, _ = oracle.integrate("dop853", uu, bb, p, tg, st0)
     g0 = st[6, 0, :]
     rel = np.abs(st[6, 1:, :] - g0) / g0
-    say(f"[energy] max relative gamma drift {rel.max():.3e}")
-    assert (status == 0).all() and rel.max() < 1e-7
+    rel_o = np.abs(so[6, 1:, :] - so[6, 0, :]) / so[6, 0, :]
+    say(f"[energy] max relative gamma drift {rel.max():.3e} (oracle {rel_o.max():.3e})")
+    assert (status == 0).all() and np.array_equal(rel, rel_o) and rel.max() < 1e-6
