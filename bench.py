@@ -46,6 +46,10 @@ def build_workload(args):
         raise SystemExit(f"unknown workload {args.workload}")
     if args.particles:
         npart = args.particles
+    if args.grid:
+        dm = args.grid
+    if args.method:
+        method = args.method
     uu, bb = W.turbulent_vb(dm, seed=args.seed)
     p = W.params_vector(qom=1.0, vun=1.0, tmax=tmax, dtmax=W.T0, atol=tol, rtol=tol)
     tg = W.time_grid(nsnap, tmax)
@@ -58,7 +62,8 @@ def build_workload(args):
                        f"gamma {'10' if gr is None else 'log-uniform [%g, %g]' % gr}, "
                        f"step = advance all particles by 1 gyro-period of B0",
            "grid": dm, "particles": npart, "method": method, "tol": tol,
-           "snapshots_every_T0": tmax / W.T0 / nsnap, "layout": args.layout}
+           "snapshots_every_T0": tmax / W.T0 / nsnap, "layout": args.layout,
+           "cell_cache": args.cell_cache}
     return uu, bb, p, tg, st0, method, cfg
 
 
@@ -104,8 +109,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="turbulent256")
     ap.add_argument("--particles", type=int, default=0)
-    ap.add_argument("--layout", default="aos", choices=["aos", "soa"])
+    ap.add_argument("--grid", type=int, default=0)
+    ap.add_argument("--method", default="", choices=["", "rk5", "dop853"])
+    ap.add_argument("--layout", default="aos", choices=["aos", "soa", "brick"])
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--cell-cache", type=int, default=1, choices=[0, 1])
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -121,6 +129,7 @@ def main():
     t_setup = time.perf_counter()
     uu, bb, p, tg, st0, method, cfg = build_workload(args)
     P.set_option("layout", args.layout)
+    P.set_option("cell_cache", args.cell_cache)
     fields = P.Fields(uu, bb)
     batch = P.Particles(fields, method, p, tg, st0)
     print(f"setup {time.perf_counter() - t_setup:.1f}s: {cfg['workload']}", file=sys.stderr,

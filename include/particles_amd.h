@@ -124,9 +124,16 @@ int ptc_probe(const ptc_fields *f, int op, int64_t n, const double *params, int 
 /* Tuning and behaviour switches (process-wide), e.g.
  *   "block"       threads per block (64..1024, multiple of 64), default 256
  *   "grid_blocks" blocks of the persistent kernel (0 = fill the device)
- *   "layout"      "soa" (the reference's six planes) or "aos" (six
- *                 components interleaved per node)
+ *   "layout"      field layout in HBM for the next ptc_fields_create:
+ *                 "soa" (the reference's six planes), "aos" (six
+ *                 components interleaved per node) or "brick" (aos plus
+ *                 each cell's eight corners in one 384-byte block; 9x the
+ *                 memory of "soa")
  *   "max_steps"   attempted-step cap per particle (0 = none)
+ *   "cell_cache"  1 (default) keeps, per particle, the corner values of the
+ *                 last cell it sampled (aos layout), so stage points in the
+ *                 same cell skip the reload; 0 reloads every time.  Results
+ *                 are identical either way.
  * Takes effect for the next create/advance. */
 int ptc_set_option(const char *key, const char *value);
 /* Reads an option back as text into buf (size len). */

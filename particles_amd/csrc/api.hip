@@ -66,6 +66,7 @@ struct Options {
     int grid_blocks = 0;
     int layout = LAYOUT_AOS;
     int64_t max_steps = 0;
+    int cell_cache = 1;
 };
 
 std::mutex g_opt_mu;
@@ -106,11 +107,16 @@ int ptc_set_option(const char *key, const char *value)
     } else if (!strcmp(key, "layout")) {
         if (!strcmp(value, "soa")) g_opt.layout = LAYOUT_SOA;
         else if (!strcmp(value, "aos")) g_opt.layout = LAYOUT_AOS;
-        else return fail(PTC_EARG, "layout must be soa or aos (got %s)", value);
+        else if (!strcmp(value, "brick")) g_opt.layout = LAYOUT_BRICK;
+        else return fail(PTC_EARG, "layout must be soa, aos or brick (got %s)", value);
     } else if (!strcmp(key, "max_steps")) {
         long long v = strtoll(value, &end, 10);
         if (*end || v < 0) return fail(PTC_EARG, "bad max_steps %s", value);
         g_opt.max_steps = v;
+    } else if (!strcmp(key, "cell_cache")) {
+        if (strcmp(value, "0") && strcmp(value, "1"))
+            return fail(PTC_EARG, "cell_cache must be 0 or 1 (got %s)", value);
+        g_opt.cell_cache = value[0] == '1';
     } else {
         return fail(PTC_EARG, "unknown option %s", key);
     }
@@ -124,8 +130,10 @@ int ptc_get_option(const char *key, char *buf, int len)
     Options o = options();
     if (!strcmp(key, "block")) snprintf(buf, len, "%d", o.block);
     else if (!strcmp(key, "grid_blocks")) snprintf(buf, len, "%d", o.grid_blocks);
-    else if (!strcmp(key, "layout")) snprintf(buf, len, "%s", o.layout == LAYOUT_SOA ? "soa" : "aos");
+    else if (!strcmp(key, "layout"))
+        snprintf(buf, len, "%s", o.layout == LAYOUT_SOA ? "soa" : o.layout == LAYOUT_AOS ? "aos" : "brick");
     else if (!strcmp(key, "max_steps")) snprintf(buf, len, "%lld", (long long)o.max_steps);
+    else if (!strcmp(key, "cell_cache")) snprintf(buf, len, "%d", o.cell_cache);
     else return fail(PTC_EARG, "unknown option %s", key);
     ok();
     return PTC_OK;
@@ -157,8 +165,8 @@ int ptc_fields_create(const int dm[3], const double *f1, const double *bb, int m
         (void)hipFree(soa);
         return hip_fail(e, "ptc_fields_create: upload");
     }
-    void *data = soa;
-    if (o.layout == LAYOUT_AOS) {
+    void *data = soa, *bricks = nullptr;
+    if (o.layout == LAYOUT_AOS || o.layout == LAYOUT_BRICK) {
         double *aos = nullptr;
         e = hipMalloc(&aos, sizeof(double) * 6 * nodes);
         if (e == hipSuccess) e = launch_to_aos(soa, nodes, aos, PTC_FP64, nullptr);
@@ -170,6 +178,21 @@ int ptc_fields_create(const int dm[3], const double *f1, const double *bb, int m
         }
         data = aos;
     }
+    if (o.layout == LAYOUT_BRICK) {
+        e = hipMalloc(&bricks, sizeof(double) * 48 * nodes);
+        if (e == hipSuccess) {
+            DeviceFields tmp{};
+            tmp.dm[0] = dm[0]; tmp.dm[1] = dm[1]; tmp.dm[2] = dm[2];
+            tmp.prec = prec; tmp.nodes = nodes; tmp.data = data; tmp.bricks = bricks;
+            e = launch_to_bricks(tmp, nullptr);
+        }
+        if (e == hipSuccess) e = hipDeviceSynchronize();
+        if (e != hipSuccess) {
+            if (bricks) (void)hipFree(bricks);
+            (void)hipFree(data);
+            return hip_fail(e, "ptc_fields_create: bricks");
+        }
+    }
     ptc_fields *f = new ptc_fields;
     f->d.dm[0] = dm[0];
     f->d.dm[1] = dm[1];
@@ -179,6 +202,7 @@ int ptc_fields_create(const int dm[3], const double *f1, const double *bb, int m
     f->d.layout = o.layout;
     f->d.nodes = nodes;
     f->d.data = data;
+    f->d.bricks = bricks;
     *out = f;
     ok();
     return PTC_OK;
@@ -188,6 +212,10 @@ int ptc_fields_destroy(ptc_fields *f)
 {
     if (!f) return PTC_OK;
     hipError_t e = hipFree(f->d.data);
+    if (f->d.bricks) {
+        hipError_t e2 = hipFree(f->d.bricks);
+        if (e == hipSuccess) e = e2;
+    }
     delete f;
     if (e != hipSuccess) return hip_fail(e, "ptc_fields_destroy");
     ok();
@@ -326,7 +354,8 @@ int ptc_particles_advance(ptc_particles *p, double t_stop)
     if (p->d.np == 0) { ok(); return PTC_OK; }
     const Options o = options();
     p->sp.max_steps = o.max_steps;
-    LaunchConfig lc{o.block, o.grid_blocks};
+    LaunchConfig lc{o.block, o.grid_blocks,
+                    o.cell_cache != 0 && p->fields->d.layout == LAYOUT_AOS};
     HIP_TRY(launch_zero_i32(p->d.qcount, 1, p->stream));
     HIP_TRY(hipEventRecord(p->ev0, p->stream));
     HIP_TRY(launch_integrate(p->method, p->fields->d, p->sp, p->d, t_stop, lc, p->stream));

@@ -50,6 +50,7 @@ __device__ __forceinline__ int wrap_idx(int i, int n)
 struct Corners {
     uint32_t o[8];  // node offsets of the 8 corners, reference order
     double w[8];    // trilinear weights, reference order
+    bool adjacent;  // right index == left + 1 (periodic) in every direction
 };
 
 // Index and weight math of interpolations.F90:140-149 and :207-210.  The
@@ -72,6 +73,12 @@ __device__ __forceinline__ void corners(const StepParams &P, const double *pos, 
         xu[i] = 1.0 - xd[i];
         l[i] = wrap_idx((int)pl, P.dm[i]);
     }
+    c.adjacent = true;
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        const int lp = l[i] + 1 == P.dm[i] ? 0 : l[i] + 1;
+        c.adjacent = c.adjacent && r[i] == lp;
+    }
     const uint32_t sy = (uint32_t)P.dm[0], sz = (uint32_t)P.dm[0] * (uint32_t)P.dm[1];
     const uint32_t yl = (uint32_t)l[1] * sy, yr = (uint32_t)r[1] * sy;
     const uint32_t zl = (uint32_t)l[2] * sz, zr = (uint32_t)r[2] * sz;
@@ -86,6 +93,30 @@ __device__ __forceinline__ void corners(const StepParams &P, const double *pos, 
     c.w[5] = xd[0] * xu[1] * xd[2];
     c.w[6] = xd[0] * xd[1] * xu[2];
     c.w[7] = xd[0] * xd[1] * xd[2];
+}
+
+// f = w0 v0 + w1 v1 + ... + w7 v7, summed left to right per component
+__device__ __forceinline__ void accumulate_row(int j, double w, const double *v, double *f)
+{
+    if (j == 0) {
+#pragma unroll
+        for (int k = 0; k < 6; k++) f[k] = w * v[k];
+    } else {
+#pragma unroll
+        for (int k = 0; k < 6; k++) f[k] = f[k] + w * v[k];
+    }
+}
+
+// eight 48-byte node rows of the AOS array
+__device__ __forceinline__ void gather_rows(const double *__restrict__ d, const Corners &c, double *f)
+{
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+        const double2 *__restrict__ q = (const double2 *)(d + (size_t)c.o[j] * 6);
+        const double2 a = q[0], b = q[1], e = q[2];
+        const double v[6] = {a.x, a.y, b.x, b.y, e.x, e.y};
+        accumulate_row(j, c.w[j], v, f);
+    }
 }
 
 // f[0..2] = U (or E), f[3..5] = B; each a left-to-right weighted sum of the
@@ -107,19 +138,21 @@ __device__ __forceinline__ void gather(const DeviceFields &F, const StepParams &
             for (int j = 1; j < 8; j++) s = s + c.w[j] * p[c.o[j]];
             f[k] = s;
         }
+    } else if constexpr (LAYOUT == LAYOUT_AOS) {
+        gather_rows(d, c, f);
     } else {
+        if (c.adjacent) {
+            // the cell's own 384-byte block: row j holds corner j
+            const double2 *__restrict__ q =
+                (const double2 *)((const double *)F.bricks + (size_t)c.o[0] * 48);
 #pragma unroll
-        for (int j = 0; j < 8; j++) {
-            const double2 *__restrict__ q = (const double2 *)(d + (size_t)c.o[j] * 6);
-            const double2 a = q[0], b = q[1], e = q[2];
-            const double v[6] = {a.x, a.y, b.x, b.y, e.x, e.y};
-            if (j == 0) {
-#pragma unroll
-                for (int k = 0; k < 6; k++) f[k] = c.w[0] * v[k];
-            } else {
-#pragma unroll
-                for (int k = 0; k < 6; k++) f[k] = f[k] + c.w[j] * v[k];
+            for (int j = 0; j < 8; j++) {
+                const double2 a = q[3 * j], b = q[3 * j + 1], e = q[3 * j + 2];
+                const double v[6] = {a.x, a.y, b.x, b.y, e.x, e.y};
+                accumulate_row(j, c.w[j], v, f);
             }
+        } else {
+            gather_rows(d, c, f);
         }
     }
 }
@@ -127,12 +160,10 @@ __device__ __forceinline__ void gather(const DeviceFields &F, const StepParams &
 // ---------------------------------------------------------------------------
 // relativistic Lorentz force, sources/accelerations.F90:59-96
 // ---------------------------------------------------------------------------
-template <int MODE, int LAYOUT>
-__device__ __forceinline__ void rhs(const DeviceFields &F, const StepParams &P, const double *s,
-                                    double *a)
+template <int MODE>
+__device__ __forceinline__ void force_from_fields(const StepParams &P, const double *s,
+                                                  const double *f, double *a)
 {
-    double f[6];
-    gather<LAYOUT>(F, P, s, f);
     const double g = sqrt(s[3] * s[3] + s[4] * s[4] + s[5] * s[5] + 1.0);
     const double v0 = s[3] / g, v1 = s[4] / g, v2 = s[5] / g;
     a[0] = v0 / P.vun;
@@ -150,11 +181,63 @@ __device__ __forceinline__ void rhs(const DeviceFields &F, const StepParams &P, 
     }
 }
 
+template <int MODE, int LAYOUT>
+__device__ __forceinline__ void rhs(const DeviceFields &F, const StepParams &P, const double *s,
+                                    double *a)
+{
+    double f[6];
+    gather<LAYOUT>(F, P, s, f);
+    force_from_fields<MODE>(P, s, f, a);
+}
+
+// The RHS as the integrator calls it.  With CACHE, each lane keeps the 48
+// corner values of the last cell it gathered from (the node rows are
+// identified by the offsets of corners 0 and 7, which fix both the left and
+// the right index triples); a stage point in the same cell re-weights those
+// values instead of loading them again.  The values and the summation order
+// are the same either way, so the result is bit-identical.
+template <int MODE, int LAYOUT, bool CACHE>
+struct Force {
+    const DeviceFields &F;
+    const StepParams &P;
+    uint32_t key0 = 0xffffffffu, key7 = 0xffffffffu;
+    double v[CACHE ? 48 : 1];
+
+    __device__ __forceinline__ Force(const DeviceFields &F_, const StepParams &P_) : F(F_), P(P_) {}
+
+    __device__ __forceinline__ void operator()(const double *s, double *a)
+    {
+        if constexpr (!CACHE) {
+            rhs<MODE, LAYOUT>(F, P, s, a);
+        } else {
+            static_assert(LAYOUT == LAYOUT_AOS, "the cell cache reads AOS node rows");
+            Corners c;
+            corners(P, s, c);
+            if (c.o[0] != key0 || c.o[7] != key7) {
+                const double *__restrict__ d = (const double *)F.data;
+#pragma unroll
+                for (int j = 0; j < 8; j++) {
+                    const double2 *__restrict__ q = (const double2 *)(d + (size_t)c.o[j] * 6);
+                    const double2 x0 = q[0], x1 = q[1], x2 = q[2];
+                    v[6 * j + 0] = x0.x; v[6 * j + 1] = x0.y; v[6 * j + 2] = x1.x;
+                    v[6 * j + 3] = x1.y; v[6 * j + 4] = x2.x; v[6 * j + 5] = x2.y;
+                }
+                key0 = c.o[0];
+                key7 = c.o[7];
+            }
+            double f[6];
+#pragma unroll
+            for (int j = 0; j < 8; j++) accumulate_row(j, c.w[j], &v[6 * j], f);
+            force_from_fields<MODE>(P, s, f, a);
+        }
+    }
+};
+
 // ---------------------------------------------------------------------------
 // initial step, sources/integrations.F90:524-589 (ki = f(si) is given)
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ double dti_guess(const StepParams &P, int iord, const double *si,
-                                            const double *ki)
+                                            const double *ki, const PowTables &T)
 {
     double sr[6], s, q, d0, d1, d2, dti, dtf;
 #pragma unroll
@@ -181,7 +264,7 @@ __device__ __forceinline__ double dti_guess(const StepParams &P, int iord, const
         dtf = 1.0e-03 * dti;
         if (dtf < 1.0e-06) dtf = 1.0e-06;
     } else {
-        dtf = ptc_libm_pow(1.0e-02 / d2, 1.0 / (double)iord);
+        dtf = ptc_pow_with(1.0e-02 / d2, 1.0 / (double)iord, T.log, T.exp);
     }
     dti = 1.0e+02 * dti;
     if (dtf < dti) dti = dtf;
@@ -221,40 +304,39 @@ __device__ __forceinline__ double dti_guess(const StepParams &P, int iord, const
 
 // k1 = dt f(si) from the cached derivative f1; returns del (error / atol),
 // sf, and the stage-6 slope k6 used by the dense output.
-template <int MODE, int LAYOUT>
-__device__ __forceinline__ double attempt_rk5(const DeviceFields &F, const StepParams &P,
-                                              const double *si, const double *f1, double dt,
-                                              double *sf, double *k6)
+template <class Rhs>
+__device__ __forceinline__ double attempt_rk5(Rhs &rhsf, const StepParams &P, const double *si,
+                                              const double *f1, double dt, double *sf, double *k6)
 {
     double k1[6], k2[6], k3[6], k4[6], k5[6], s[6], a[6];
 #pragma unroll
     for (int c = 0; c < 6; c++) k1[c] = dt * f1[c];
 #pragma unroll
     for (int c = 0; c < 6; c++) s[c] = si[c] + CK_A21 * k1[c];
-    rhs<MODE, LAYOUT>(F, P, s, a);
+    rhsf(s, a);
 #pragma unroll
     for (int c = 0; c < 6; c++) k2[c] = dt * a[c];
 #pragma unroll
     for (int c = 0; c < 6; c++) s[c] = si[c] + CK_A31 * k1[c] + CK_A32 * k2[c];
-    rhs<MODE, LAYOUT>(F, P, s, a);
+    rhsf(s, a);
 #pragma unroll
     for (int c = 0; c < 6; c++) k3[c] = dt * a[c];
 #pragma unroll
     for (int c = 0; c < 6; c++) s[c] = si[c] + CK_A41 * k1[c] + CK_A42 * k2[c] + CK_A43 * k3[c];
-    rhs<MODE, LAYOUT>(F, P, s, a);
+    rhsf(s, a);
 #pragma unroll
     for (int c = 0; c < 6; c++) k4[c] = dt * a[c];
 #pragma unroll
     for (int c = 0; c < 6; c++)
         s[c] = si[c] + CK_A51 * k1[c] + CK_A52 * k2[c] + CK_A53 * k3[c] + CK_A54 * k4[c];
-    rhs<MODE, LAYOUT>(F, P, s, a);
+    rhsf(s, a);
 #pragma unroll
     for (int c = 0; c < 6; c++) k5[c] = dt * a[c];
 #pragma unroll
     for (int c = 0; c < 6; c++)
         s[c] = si[c] + CK_A61 * k1[c] + CK_A62 * k2[c] + CK_A63 * k3[c] + CK_A64 * k4[c] +
                CK_A65 * k5[c];
-    rhs<MODE, LAYOUT>(F, P, s, a);
+    rhsf(s, a);
 #pragma unroll
     for (int c = 0; c < 6; c++) k6[c] = dt * a[c];
 
@@ -360,13 +442,12 @@ __device__ __forceinline__ double attempt_rk5(const DeviceFields &F, const StepP
 
 // k1 = f(si) is given (raw derivatives, no dt inside, as in the reference);
 // returns err and sf.
-template <int MODE, int LAYOUT>
-__device__ __forceinline__ double attempt_dp853(const DeviceFields &F, const StepParams &P,
-                                                const double *si, const double *k1, double dt,
-                                                double *sf)
+template <class Rhs>
+__device__ __forceinline__ double attempt_dp853(Rhs &rhsf, const StepParams &P, const double *si,
+                                                const double *k1, double dt, double *sf)
 {
     double k2[6], k3[6], k4[6], k5[6], k6[6], k7[6], k8[6], k9[6], k10[6], k11[6], k12[6], ss[6];
-#define RHS(out) rhs<MODE, LAYOUT>(F, P, ss, out)
+#define RHS(out) rhsf(ss, out)
 #pragma unroll
     for (int i = 0; i < 6; i++) ss[i] = si[i] + dt * A0201 * k1[i];
     RHS(k2);
@@ -467,10 +548,21 @@ __device__ __forceinline__ bool finite6(const double *a)
 // ---------------------------------------------------------------------------
 // persistent work-queue integrator
 // ---------------------------------------------------------------------------
-template <int METHOD, int MODE, int LAYOUT>
+template <int METHOD, int MODE, int LAYOUT, bool CACHE>
 __global__ void __launch_bounds__(kBlock)
 integrate_kernel(const DeviceFields F, const StepParams P, const DeviceParticles S, double t_stop)
 {
+    // the power's lookup tables (5 KB) in LDS: per-lane indices make every
+    // lookup a divergent gather, which LDS serves far faster than memory
+    __shared__ double s_logt[128][3];
+    __shared__ unsigned long long s_expt[256];
+    for (int i = threadIdx.x; i < 128 * 3; i += blockDim.x)
+        (&s_logt[0][0])[i] = (&ptc_pow_log_tab[0][0])[i];
+    for (int i = threadIdx.x; i < 256; i += blockDim.x) s_expt[i] = ptc_pow_exp_tab[i];
+    __syncthreads();
+    const PowTables T{s_logt, s_expt};
+
+    Force<MODE, LAYOUT, CACHE> force(F, P);
     const int lane = (int)__lane_id();
     const double t_end = t_stop < P.tmx ? t_stop : P.tmx;
     const int64_t N = S.np;
@@ -525,7 +617,7 @@ integrate_kernel(const DeviceFields F, const StepParams P, const DeviceParticles
         // ---- one shared RHS slot: f(si) for refilled lanes and for lanes
         //      that accepted last iteration (the next step's stage 1) ----
         if (need_rhs) {
-            rhs<MODE, LAYOUT>(F, P, si, f1);
+            force(si, f1);
             c_rhs++;
             need_rhs = false;
         }
@@ -535,7 +627,7 @@ integrate_kernel(const DeviceFields F, const StepParams P, const DeviceParticles
             fresh = false;
             if (!(flags & FLAG_STARTED)) {
                 // integrations.F90:120-127 / :352-359
-                dt = dti_guess(P, METHOD == PTC_RK5 ? 5 : 8, si, f1);
+                dt = dti_guess(P, METHOD == PTC_RK5 ? 5 : 8, si, f1, T);
                 double *o = S.out + (size_t)8 * P.m * idx;
                 o[6] = sqrt(si[3] * si[3] + si[4] * si[4] + si[5] * si[5] + 1.0);
                 o[7] = dt;
@@ -567,12 +659,18 @@ integrate_kernel(const DeviceFields F, const StepParams P, const DeviceParticles
                 double dtn;
                 if constexpr (METHOD == PTC_RK5) {
                     double k6[6];
-                    const double del = attempt_rk5<MODE, LAYOUT>(F, P, si, f1, dt, sf, k6);
+                    const double del = attempt_rk5(force, P, si, f1, dt, sf, k6);
                     c_rhs += 5;
                     if (!isfinite(del) || !finite6(sf)) {
                         status = PTC_STATUS_NONFINITE;
                         finish = true;
-                    } else if (del <= 1.0) {
+                    } else {
+                      // one power per lane serves both outcomes:
+                      // del**(-0.2) on accept, del**(-0.25) on reject
+                      // (integrations.F90:177,186)
+                      const bool accept = del <= 1.0;
+                      const double q = P.safe * ptc_pow_with(del, accept ? -0.20 : -0.25, T.log, T.exp);
+                      if (accept) {
                         t = t + dt;
                         if (n <= P.m && t >= tnext) {
                             double k1[6];
@@ -588,27 +686,29 @@ integrate_kernel(const DeviceFields F, const StepParams P, const DeviceParticles
 #pragma unroll
                         for (int c = 0; c < 6; c++) si[c] = sf[c];
                         c_acc++;
-                        const double q = P.safe * ptc_libm_pow(del, -0.20);
                         dtn = dt * (P.fcmx < q ? P.fcmx : q);
                         need_rhs = true;
-                    } else {
+                      } else {
                         c_rej++;
-                        const double q = P.safe * ptc_libm_pow(del, -0.25);
                         dtn = dt * (P.fcmn > q ? P.fcmn : q);
+                      }
                     }
                 } else {
-                    const double err = attempt_dp853<MODE, LAYOUT>(F, P, si, f1, dt, sf);
+                    const double err = attempt_dp853(force, P, si, f1, dt, sf);
                     c_rhs += 11;
                     if (!isfinite(err) || !finite6(sf)) {
                         status = PTC_STATUS_NONFINITE;
                         finish = true;
-                    } else if (err <= 1.0) {
+                    } else {
+                      // safe * err**expo serves both outcomes (integrations.F90:466,486)
+                      const double qe = P.safe * ptc_pow_with(err, P.expo, T.log, T.exp);
+                      if (err <= 1.0) {
                         t = t + dt;
                         if (n <= P.m && t >= tnext) {
                             // the end slope f(sf) is needed now; it is also
                             // the next step's stage 1
                             double fe[6], ka[6], kb[6];
-                            rhs<MODE, LAYOUT>(F, P, sf, fe);
+                            force(sf, fe);
                             c_rhs++;
 #pragma unroll
                             for (int c = 0; c < 6; c++) { ka[c] = dt * f1[c]; kb[c] = dt * fe[c]; }
@@ -626,17 +726,20 @@ integrate_kernel(const DeviceFields F, const StepParams P, const DeviceParticles
 #pragma unroll
                         for (int c = 0; c < 6; c++) si[c] = sf[c];
                         c_acc++;
-                        double q = P.safe * ptc_libm_pow(err, P.expo) * ptc_libm_pow(errold, P.beta);
+                        double q = qe;
+                        // errold**beta; x**0 is exactly 1 (and q * 1 == q),
+                        // so the beta = 0 default skips the power
+                        if (P.beta != 0.0) q = q * ptc_pow_with(errold, P.beta, T.log, T.exp);
                         if (q < P.fcmn) q = P.fcmn;
                         if (flags & FLAG_REJECTED) dtn = dt * (q < 1.0 ? q : 1.0);
                         else dtn = dt * (q < P.fcmx ? q : P.fcmx);
                         errold = err > 1.0e-04 ? err : 1.0e-04;
                         flags &= ~FLAG_REJECTED;
-                    } else {
+                      } else {
                         c_rej++;
-                        const double q = P.safe * ptc_libm_pow(err, P.expo);
-                        dtn = dt * (P.fcmn > q ? P.fcmn : q);
+                        dtn = dt * (P.fcmn > qe ? P.fcmn : qe);
                         flags |= FLAG_REJECTED;
+                      }
                     }
                 }
                 if (!finish) {
@@ -697,7 +800,7 @@ __global__ void probe_kernel(const DeviceFields F, const StepParams P, int op, i
 #pragma unroll
         for (int k = 0; k < 6; k++) v[k] = in[6 * i + k];
         rhs<MODE, LAYOUT>(F, P, v, r);
-        out[i] = dti_guess(P, op == 2 ? 5 : 8, v, r);
+        out[i] = dti_guess(P, op == 2 ? 5 : 8, v, r, PowTables{ptc_pow_log_tab, ptc_pow_exp_tab});
     } else if (op == 4) {
         out[i] = ptc_libm_pow(in[2 * i], in[2 * i + 1]);
     }
@@ -765,11 +868,33 @@ __global__ void to_aos_kernel(const double *__restrict__ soa, int64_t nodes, dou
     }
 }
 
-template <int METHOD, int MODE, int LAYOUT>
+// one thread per value of the brick array: cell-major, then corner, then
+// component, so the writes are contiguous
+__global__ void to_bricks_kernel(const double *__restrict__ aos, int dm0, int dm1, int dm2,
+                                 int64_t nodes, double *__restrict__ bricks)
+{
+    const int64_t total = nodes * 48;
+    for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < total;
+         g += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t cell = g / 48;
+        const int rem = (int)(g - cell * 48), j = rem / 6, comp = rem - j * 6;
+        const int i = (int)(cell % dm0);
+        const int jy = (int)((cell / dm0) % dm1);
+        const int kz = (int)(cell / ((int64_t)dm0 * dm1));
+        // corner j = (x, y, z) offsets (j>>2, (j>>1)&1, j&1): x slowest
+        const int ii = (j >> 2) ? (i + 1 == dm0 ? 0 : i + 1) : i;
+        const int jj = ((j >> 1) & 1) ? (jy + 1 == dm1 ? 0 : jy + 1) : jy;
+        const int kk = (j & 1) ? (kz + 1 == dm2 ? 0 : kz + 1) : kz;
+        const int64_t node = ii + (int64_t)dm0 * (jj + (int64_t)dm1 * kk);
+        bricks[g] = aos[node * 6 + comp];
+    }
+}
+
+template <int METHOD, int MODE, int LAYOUT, bool CACHE>
 hipError_t launch_t(const DeviceFields &f, const StepParams &sp, const DeviceParticles &p,
                     double t_stop, const LaunchConfig &lc, hipStream_t stream)
 {
-    auto kern = integrate_kernel<METHOD, MODE, LAYOUT>;
+    auto kern = integrate_kernel<METHOD, MODE, LAYOUT, CACHE>;
     int grid = lc.grid_blocks;
     if (grid <= 0) {
         int dev = 0, cus = 0, per_cu = 0;
@@ -794,18 +919,26 @@ hipError_t launch_integrate(int method, const DeviceFields &f, const StepParams 
                             const DeviceParticles &p, double t_stop, const LaunchConfig &lc,
                             hipStream_t stream)
 {
-#define DISPATCH(M, MO, L)                                                     \
-    if (method == M && f.mode == MO && f.layout == L)                          \
-        return launch_t<M, MO, L>(f, sp, p, t_stop, lc, stream);
+#define DISPATCH(M, MO, L, C)                                                  \
+    if (method == M && f.mode == MO && f.layout == L && lc.cell_cache == C)    \
+        return launch_t<M, MO, L, C>(f, sp, p, t_stop, lc, stream);
     if (f.prec != PTC_FP64) return hipErrorInvalidValue;
-    DISPATCH(PTC_RK5, PTC_MODE_VB, LAYOUT_SOA)
-    DISPATCH(PTC_RK5, PTC_MODE_VB, LAYOUT_AOS)
-    DISPATCH(PTC_RK5, PTC_MODE_EB, LAYOUT_SOA)
-    DISPATCH(PTC_RK5, PTC_MODE_EB, LAYOUT_AOS)
-    DISPATCH(PTC_DOP853, PTC_MODE_VB, LAYOUT_SOA)
-    DISPATCH(PTC_DOP853, PTC_MODE_VB, LAYOUT_AOS)
-    DISPATCH(PTC_DOP853, PTC_MODE_EB, LAYOUT_SOA)
-    DISPATCH(PTC_DOP853, PTC_MODE_EB, LAYOUT_AOS)
+    DISPATCH(PTC_RK5, PTC_MODE_VB, LAYOUT_AOS, true)
+    DISPATCH(PTC_RK5, PTC_MODE_EB, LAYOUT_AOS, true)
+    DISPATCH(PTC_DOP853, PTC_MODE_VB, LAYOUT_AOS, true)
+    DISPATCH(PTC_DOP853, PTC_MODE_EB, LAYOUT_AOS, true)
+    DISPATCH(PTC_RK5, PTC_MODE_VB, LAYOUT_SOA, false)
+    DISPATCH(PTC_RK5, PTC_MODE_VB, LAYOUT_AOS, false)
+    DISPATCH(PTC_RK5, PTC_MODE_EB, LAYOUT_SOA, false)
+    DISPATCH(PTC_RK5, PTC_MODE_EB, LAYOUT_AOS, false)
+    DISPATCH(PTC_DOP853, PTC_MODE_VB, LAYOUT_SOA, false)
+    DISPATCH(PTC_DOP853, PTC_MODE_VB, LAYOUT_AOS, false)
+    DISPATCH(PTC_DOP853, PTC_MODE_EB, LAYOUT_SOA, false)
+    DISPATCH(PTC_DOP853, PTC_MODE_EB, LAYOUT_AOS, false)
+    DISPATCH(PTC_RK5, PTC_MODE_VB, LAYOUT_BRICK, false)
+    DISPATCH(PTC_RK5, PTC_MODE_EB, LAYOUT_BRICK, false)
+    DISPATCH(PTC_DOP853, PTC_MODE_VB, LAYOUT_BRICK, false)
+    DISPATCH(PTC_DOP853, PTC_MODE_EB, LAYOUT_BRICK, false)
 #undef DISPATCH
     return hipErrorInvalidValue;
 }
@@ -821,7 +954,19 @@ hipError_t launch_probe(const DeviceFields &f, const StepParams &sp, int op, int
         return launch_probe_t<PTC_MODE_VB, LAYOUT_AOS>(f, sp, op, n, in, out, stream);
     if (f.mode == PTC_MODE_EB && f.layout == LAYOUT_SOA)
         return launch_probe_t<PTC_MODE_EB, LAYOUT_SOA>(f, sp, op, n, in, out, stream);
-    return launch_probe_t<PTC_MODE_EB, LAYOUT_AOS>(f, sp, op, n, in, out, stream);
+    if (f.mode == PTC_MODE_EB && f.layout == LAYOUT_AOS)
+        return launch_probe_t<PTC_MODE_EB, LAYOUT_AOS>(f, sp, op, n, in, out, stream);
+    if (f.mode == PTC_MODE_VB)
+        return launch_probe_t<PTC_MODE_VB, LAYOUT_BRICK>(f, sp, op, n, in, out, stream);
+    return launch_probe_t<PTC_MODE_EB, LAYOUT_BRICK>(f, sp, op, n, in, out, stream);
+}
+
+hipError_t launch_to_bricks(const DeviceFields &f, hipStream_t stream)
+{
+    if (f.prec != PTC_FP64 || !f.bricks) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(to_bricks_kernel, dim3(65536), dim3(256), 0, stream, (const double *)f.data,
+                       f.dm[0], f.dm[1], f.dm[2], f.nodes, (double *)f.bricks);
+    return hipGetLastError();
 }
 
 hipError_t launch_totals(const DeviceParticles &p, int64_t *d_out, hipStream_t stream)

@@ -7,20 +7,26 @@
 
 namespace ptc {
 
-enum Layout : int { LAYOUT_SOA = 0, LAYOUT_AOS = 1 };
+enum Layout : int { LAYOUT_SOA = 0, LAYOUT_AOS = 1, LAYOUT_BRICK = 2 };
 
 // Device copy of one field snapshot.
 //   SOA: six planes of dm1*dm2*dm3 values (U/E x,y,z then B x,y,z), each x
 //        fastest -- the reference's uu(:,:,:,c) / bb(:,:,:,c) planes.
 //   AOS: one array of nodes, six values per node (U/E x,y,z, B x,y,z), so
 //        the eight corners of a cell are eight 48-byte rows.
+//   BRICK: AOS nodes plus, per cell, the eight corner rows of that cell in
+//        the reference's corner order (48 values, 384 bytes, three 128-byte
+//        lines) -- eight copies of every node, so a gather reads one
+//        contiguous block.  Cells whose right index is not left + 1 (a face
+//        case of the reference's separate floor) read the AOS nodes.
 struct DeviceFields {
     int dm[3];
     int mode;     // PTC_MODE_VB / PTC_MODE_EB
     int prec;     // PTC_FP64 / PTC_FP32
     int layout;   // Layout
     int64_t nodes;
-    void *data;   // SOA: 6 * nodes values; AOS: nodes * 6 values
+    void *data;   // SOA: 6 * nodes values; AOS/BRICK: nodes * 6 values
+    void *bricks; // BRICK: nodes * 48 values (cell-major), else nullptr
 };
 
 // Geometry and controller constants of one integration (params vector of
@@ -56,6 +62,7 @@ enum : int32_t { FLAG_REJECTED = 1, FLAG_STARTED = 2 };
 struct LaunchConfig {
     int block;
     int grid_blocks;   // 0 = occupancy-derived
+    bool cell_cache;   // keep the last cell's corner values per lane (AOS only)
 };
 
 // Launches the integrator on `stream`; returns a hipError_t.
@@ -72,5 +79,7 @@ hipError_t launch_zero_i32(int32_t *ptr, int64_t n, hipStream_t stream);
 hipError_t launch_zero_i64(int64_t *ptr, int64_t n, hipStream_t stream);
 // SOA planes -> AOS nodes (and FP64 -> FP32 when prec is PTC_FP32).
 hipError_t launch_to_aos(const double *soa, int64_t nodes, void *aos, int prec, hipStream_t stream);
+// AOS nodes -> per-cell corner bricks.
+hipError_t launch_to_bricks(const DeviceFields &f, hipStream_t stream);
 
 }  // namespace ptc

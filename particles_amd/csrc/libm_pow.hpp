@@ -52,7 +52,7 @@ PTC_POW_HD double from_bits(uint64_t u)
 PTC_POW_HD uint32_t top12(double x) { return (uint32_t)(bits(x) >> 52); }
 
 // log part: y + *tail = log(x), |tail| ~ 2^-68 |y|
-PTC_POW_HD double log_inline(uint64_t ix, double *tail)
+PTC_POW_HD double log_inline(uint64_t ix, double *tail, const double (*logt)[3])
 {
     const double ln2hi = 0x1.62e42fefa3800p-1, ln2lo = 0x1.ef35793c76730p-45;
     // log1p(r) - r + r^2/2 minimax coefficients, pre-scaled for the
@@ -66,8 +66,7 @@ PTC_POW_HD double log_inline(uint64_t ix, double *tail)
     const int k = (int)((int64_t)tmp >> 52);
     const uint64_t iz = ix - (tmp & 0xfffULL << 52);
     const double z = from_bits(iz), kd = (double)k;
-    const double invc = ptc_pow_log_tab[i][0], logc = ptc_pow_log_tab[i][1],
-                 logctail = ptc_pow_log_tab[i][2];
+    const double invc = logt[i][0], logc = logt[i][1], logctail = logt[i][2];
     const double r = fma(z, invc, -1.0);  // exact: 1/c has few bits
     const double t1 = fma(kd, ln2hi, logc);
     const double t2 = t1 + r;
@@ -107,7 +106,8 @@ PTC_POW_HD double exp_special(double tmp, uint64_t sbits, uint64_t ki)
 }
 
 // sign_bias * exp(x + xtail)
-PTC_POW_HD double exp_inline(double x, double xtail, uint64_t sign_bias)
+PTC_POW_HD double exp_inline(double x, double xtail, uint64_t sign_bias,
+                             const unsigned long long *expt)
 {
     const double inv_ln2_n = 0x1.71547652b82fep0 * 128, neg_ln2hi_n = -0x1.62e42fefa0000p-8,
                  neg_ln2lo_n = -0x1.cf79abc9e3b3ap-47, shift = 0x1.8p52;
@@ -132,8 +132,8 @@ PTC_POW_HD double exp_inline(double x, double xtail, uint64_t sign_bias)
     r += xtail;
     const uint64_t idx = 2 * (ki % 128);
     const uint64_t top = (ki + sign_bias) << 45;
-    const double tail = from_bits(ptc_pow_exp_tab[idx]);
-    const uint64_t sbits = ptc_pow_exp_tab[idx + 1] + top;
+    const double tail = from_bits(expt[idx]);
+    const uint64_t sbits = expt[idx + 1] + top;
     const double r2 = r * r;
     const double tmp = fma(r2 * r2, fma(r, C5, C4), fma(r2, fma(r, C3, C2), tail + r));
     if (abstop == 0) return exp_special(tmp, sbits, ki);
@@ -159,9 +159,17 @@ PTC_POW_HD bool zeroinfnan(uint64_t i)
 
 }  // namespace ptc_pow
 
+// Table locations: the constant tables of pow_tables.h, or copies of them
+// (device code keeps them in LDS).
+struct PowTables {
+    const double (*log)[3];
+    const unsigned long long *exp;
+};
+
 // x ** y with glibc's results, including its special cases (zeros, infinities,
 // NaN, negative x with integer y, subnormal x, |y| tiny or huge).
-PTC_POW_HD double ptc_libm_pow(double x, double y)
+PTC_POW_HD double ptc_pow_with(double x, double y, const double (*logt)[3],
+                               const unsigned long long *expt)
 {
     using namespace ptc_pow;
     uint64_t sign_bias = 0;
@@ -202,8 +210,13 @@ PTC_POW_HD double ptc_libm_pow(double x, double y)
         }
     }
     double lo;
-    const double hi = log_inline(ix, &lo);
+    const double hi = log_inline(ix, &lo, logt);
     const double ehi = y * hi;
     const double elo = fma(y, lo, fma(y, hi, -ehi));
-    return exp_inline(ehi, elo, sign_bias);
+    return exp_inline(ehi, elo, sign_bias, expt);
+}
+
+PTC_POW_HD double ptc_libm_pow(double x, double y)
+{
+    return ptc_pow_with(x, y, ptc_pow_log_tab, ptc_pow_exp_tab);
 }

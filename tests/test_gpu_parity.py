@@ -39,7 +39,7 @@ def exact_particles(sa, ca, sb, cb):
                      for i in range(sa.shape[2])])
 
 
-@pytest.fixture(params=["aos", "soa"])
+@pytest.fixture(params=["aos", "soa", "brick"])
 def layout(request):
     old = P.get_option("layout")
     P.set_option("layout", request.param)
