@@ -131,9 +131,9 @@ int ptc_probe(const ptc_fields *f, int op, int64_t n, const double *params, int 
  *                 memory of "soa")
  *   "max_steps"   attempted-step cap per particle (0 = none)
  *   "cell_cache"  1 (default) keeps, per particle, the corner values of the
- *                 last cell it sampled (aos layout), so stage points in the
- *                 same cell skip the reload; 0 reloads every time.  Results
- *                 are identical either way.
+ *                 last cell it sampled (aos layout) in registers, so stage
+ *                 points in the same cell skip the reload; 0 reloads every
+ *                 time.  Results are identical either way.
  * Takes effect for the next create/advance. */
 int ptc_set_option(const char *key, const char *value);
 /* Reads an option back as text into buf (size len). */

@@ -116,7 +116,7 @@ int ptc_set_option(const char *key, const char *value)
     } else if (!strcmp(key, "cell_cache")) {
         if (strcmp(value, "0") && strcmp(value, "1"))
             return fail(PTC_EARG, "cell_cache must be 0 or 1 (got %s)", value);
-        g_opt.cell_cache = value[0] == '1';
+        g_opt.cell_cache = value[0] - '0';
     } else {
         return fail(PTC_EARG, "unknown option %s", key);
     }
@@ -355,7 +355,7 @@ int ptc_particles_advance(ptc_particles *p, double t_stop)
     const Options o = options();
     p->sp.max_steps = o.max_steps;
     LaunchConfig lc{o.block, o.grid_blocks,
-                    o.cell_cache != 0 && p->fields->d.layout == LAYOUT_AOS};
+                    p->fields->d.layout == LAYOUT_AOS ? o.cell_cache : 0};
     HIP_TRY(launch_zero_i32(p->d.qcount, 1, p->stream));
     HIP_TRY(hipEventRecord(p->ev0, p->stream));
     HIP_TRY(launch_integrate(p->method, p->fields->d, p->sp, p->d, t_stop, lc, p->stream));

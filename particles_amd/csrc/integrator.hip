@@ -190,24 +190,25 @@ __device__ __forceinline__ void rhs(const DeviceFields &F, const StepParams &P, 
     force_from_fields<MODE>(P, s, f, a);
 }
 
-// The RHS as the integrator calls it.  With CACHE, each lane keeps the 48
-// corner values of the last cell it gathered from (the node rows are
+// The RHS as the integrator calls it.  With a cell cache, each lane keeps the
+// 48 corner values of the last cell it gathered from (the node rows are
 // identified by the offsets of corners 0 and 7, which fix both the left and
 // the right index triples); a stage point in the same cell re-weights those
 // values instead of loading them again.  The values and the summation order
-// are the same either way, so the result is bit-identical.
-template <int MODE, int LAYOUT, bool CACHE>
+// are the same either way, so the result is bit-identical.  CACHE 0 reloads
+// every time (SOA and brick layouts).
+template <int MODE, int LAYOUT, int CACHE>
 struct Force {
     const DeviceFields &F;
     const StepParams &P;
     uint32_t key0 = 0xffffffffu, key7 = 0xffffffffu;
-    double v[CACHE ? 48 : 1];
+    double v[CACHE == 1 ? 48 : 1];
 
     __device__ __forceinline__ Force(const DeviceFields &F_, const StepParams &P_) : F(F_), P(P_) {}
 
     __device__ __forceinline__ void operator()(const double *s, double *a)
     {
-        if constexpr (!CACHE) {
+        if constexpr (CACHE == 0) {
             rhs<MODE, LAYOUT>(F, P, s, a);
         } else {
             static_assert(LAYOUT == LAYOUT_AOS, "the cell cache reads AOS node rows");
@@ -548,7 +549,7 @@ __device__ __forceinline__ bool finite6(const double *a)
 // ---------------------------------------------------------------------------
 // persistent work-queue integrator
 // ---------------------------------------------------------------------------
-template <int METHOD, int MODE, int LAYOUT, bool CACHE>
+template <int METHOD, int MODE, int LAYOUT, int CACHE>
 __global__ void __launch_bounds__(kBlock)
 integrate_kernel(const DeviceFields F, const StepParams P, const DeviceParticles S, double t_stop)
 {
@@ -890,9 +891,9 @@ __global__ void to_bricks_kernel(const double *__restrict__ aos, int dm0, int dm
     }
 }
 
-template <int METHOD, int MODE, int LAYOUT, bool CACHE>
+template <int METHOD, int MODE, int LAYOUT, int CACHE>
 hipError_t launch_t(const DeviceFields &f, const StepParams &sp, const DeviceParticles &p,
-                    double t_stop, const LaunchConfig &lc, hipStream_t stream)
+                    double t_stop, LaunchConfig lc, hipStream_t stream)
 {
     auto kern = integrate_kernel<METHOD, MODE, LAYOUT, CACHE>;
     int grid = lc.grid_blocks;
@@ -923,22 +924,22 @@ hipError_t launch_integrate(int method, const DeviceFields &f, const StepParams 
     if (method == M && f.mode == MO && f.layout == L && lc.cell_cache == C)    \
         return launch_t<M, MO, L, C>(f, sp, p, t_stop, lc, stream);
     if (f.prec != PTC_FP64) return hipErrorInvalidValue;
-    DISPATCH(PTC_RK5, PTC_MODE_VB, LAYOUT_AOS, true)
-    DISPATCH(PTC_RK5, PTC_MODE_EB, LAYOUT_AOS, true)
-    DISPATCH(PTC_DOP853, PTC_MODE_VB, LAYOUT_AOS, true)
-    DISPATCH(PTC_DOP853, PTC_MODE_EB, LAYOUT_AOS, true)
-    DISPATCH(PTC_RK5, PTC_MODE_VB, LAYOUT_SOA, false)
-    DISPATCH(PTC_RK5, PTC_MODE_VB, LAYOUT_AOS, false)
-    DISPATCH(PTC_RK5, PTC_MODE_EB, LAYOUT_SOA, false)
-    DISPATCH(PTC_RK5, PTC_MODE_EB, LAYOUT_AOS, false)
-    DISPATCH(PTC_DOP853, PTC_MODE_VB, LAYOUT_SOA, false)
-    DISPATCH(PTC_DOP853, PTC_MODE_VB, LAYOUT_AOS, false)
-    DISPATCH(PTC_DOP853, PTC_MODE_EB, LAYOUT_SOA, false)
-    DISPATCH(PTC_DOP853, PTC_MODE_EB, LAYOUT_AOS, false)
-    DISPATCH(PTC_RK5, PTC_MODE_VB, LAYOUT_BRICK, false)
-    DISPATCH(PTC_RK5, PTC_MODE_EB, LAYOUT_BRICK, false)
-    DISPATCH(PTC_DOP853, PTC_MODE_VB, LAYOUT_BRICK, false)
-    DISPATCH(PTC_DOP853, PTC_MODE_EB, LAYOUT_BRICK, false)
+    DISPATCH(PTC_RK5, PTC_MODE_VB, LAYOUT_AOS, 1)
+    DISPATCH(PTC_RK5, PTC_MODE_EB, LAYOUT_AOS, 1)
+    DISPATCH(PTC_DOP853, PTC_MODE_VB, LAYOUT_AOS, 1)
+    DISPATCH(PTC_DOP853, PTC_MODE_EB, LAYOUT_AOS, 1)
+    DISPATCH(PTC_RK5, PTC_MODE_VB, LAYOUT_SOA, 0)
+    DISPATCH(PTC_RK5, PTC_MODE_VB, LAYOUT_AOS, 0)
+    DISPATCH(PTC_RK5, PTC_MODE_EB, LAYOUT_SOA, 0)
+    DISPATCH(PTC_RK5, PTC_MODE_EB, LAYOUT_AOS, 0)
+    DISPATCH(PTC_DOP853, PTC_MODE_VB, LAYOUT_SOA, 0)
+    DISPATCH(PTC_DOP853, PTC_MODE_VB, LAYOUT_AOS, 0)
+    DISPATCH(PTC_DOP853, PTC_MODE_EB, LAYOUT_SOA, 0)
+    DISPATCH(PTC_DOP853, PTC_MODE_EB, LAYOUT_AOS, 0)
+    DISPATCH(PTC_RK5, PTC_MODE_VB, LAYOUT_BRICK, 0)
+    DISPATCH(PTC_RK5, PTC_MODE_EB, LAYOUT_BRICK, 0)
+    DISPATCH(PTC_DOP853, PTC_MODE_VB, LAYOUT_BRICK, 0)
+    DISPATCH(PTC_DOP853, PTC_MODE_EB, LAYOUT_BRICK, 0)
 #undef DISPATCH
     return hipErrorInvalidValue;
 }

@@ -62,7 +62,7 @@ enum : int32_t { FLAG_REJECTED = 1, FLAG_STARTED = 2 };
 struct LaunchConfig {
     int block;
     int grid_blocks;   // 0 = occupancy-derived
-    bool cell_cache;   // keep the last cell's corner values per lane (AOS only)
+    int cell_cache;    // keep the last cell's corner values per lane (AOS only): 0 off, 1 on
 };
 
 // Launches the integrator on `stream`; returns a hipError_t.

@@ -170,6 +170,27 @@ def test_segments_equal_one_call(method):
     assert tot["all"] == c1[0].sum()
 
 
+@pytest.mark.parametrize("method", ["rk5", "dop853"])
+def test_cell_cache_modes_identical(oracle, method):
+    """The per-particle cell cache changes where the corner values come from,
+    never their values or summation order."""
+    uu, bb = W.turbulent_vb(24, seed=12, nmodes=64)
+    p = W.params_vector(tmax=2 * W.T0, dtmax=1.0, atol=1e-8, rtol=1e-8)
+    tg = W.time_grid(4, p[2])
+    st0 = W.initial_state(500, len(tg), seed=13, gamma_range=(1.0, 1000.0))
+    so, co, _ = oracle.integrate(method, uu, bb, p, tg, st0)
+    try:
+        for mode in (0, 1):
+            P.set_option("cell_cache", mode)
+            say(f"[cache] {method} cell_cache={mode}")
+            with P.Fields(uu, bb) as f:
+                st, cnt, status = P.integrate(f, method, p, tg, st0)
+            assert (status == 0).all()
+            assert np.array_equal(st, so) and np.array_equal(cnt, co), mode
+    finally:
+        P.set_option("cell_cache", 1)
+
+
 def test_launch_shape_does_not_change_results():
     """The work queue hands particles to lanes in a launch-dependent order;
     results must not depend on it."""
