@@ -114,6 +114,7 @@ def main():
     ap.add_argument("--layout", default="aos", choices=["aos", "soa", "brick"])
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--cell-cache", type=int, default=1, choices=[0, 1])
+    ap.add_argument("--grid-blocks", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -130,6 +131,7 @@ def main():
     uu, bb, p, tg, st0, method, cfg = build_workload(args)
     P.set_option("layout", args.layout)
     P.set_option("cell_cache", args.cell_cache)
+    P.set_option("grid_blocks", args.grid_blocks)
     fields = P.Fields(uu, bb)
     batch = P.Particles(fields, method, p, tg, st0)
     print(f"setup {time.perf_counter() - t_setup:.1f}s: {cfg['workload']}", file=sys.stderr,
