@@ -63,7 +63,7 @@ def build_workload(args):
                        f"step = advance all particles by 1 gyro-period of B0",
            "grid": dm, "particles": npart, "method": method, "tol": tol,
            "snapshots_every_T0": tmax / W.T0 / nsnap, "layout": args.layout,
-           "cell_cache": args.cell_cache}
+           "cell_cache": args.cell_cache, "field_precision": args.prec}
     return uu, bb, p, tg, st0, method, cfg
 
 
@@ -115,6 +115,8 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--cell-cache", type=int, default=1, choices=[0, 1])
     ap.add_argument("--grid-blocks", type=int, default=0)
+    ap.add_argument("--prec", default="fp64", choices=["fp64", "fp32"],
+                    help="fp32 = mixed precision (FP32 fields, weights and RHS; FP64 state)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -132,7 +134,7 @@ def main():
     P.set_option("layout", args.layout)
     P.set_option("cell_cache", args.cell_cache)
     P.set_option("grid_blocks", args.grid_blocks)
-    fields = P.Fields(uu, bb)
+    fields = P.Fields(uu, bb, prec=args.prec)
     batch = P.Particles(fields, method, p, tg, st0)
     print(f"setup {time.perf_counter() - t_setup:.1f}s: {cfg['workload']}", file=sys.stderr,
           flush=True)

@@ -149,12 +149,14 @@ int ptc_fields_create(const int dm[3], const double *f1, const double *bb, int m
         return fail(PTC_EARG, "ptc_fields_create: dimensions must be >= 2 (got %d %d %d)", dm[0],
                     dm[1], dm[2]);
     if (mode != PTC_MODE_VB && mode != PTC_MODE_EB) return fail(PTC_EARG, "bad field mode %d", mode);
-    if (prec != PTC_FP64) return fail(PTC_EARG, "field precision %d not supported", prec);
+    if (prec != PTC_FP64 && prec != PTC_FP32)
+        return fail(PTC_EARG, "field precision %d not supported", prec);
     const int64_t nodes = (int64_t)dm[0] * dm[1] * dm[2];
     if (nodes > (int64_t)0xFFFFFFFFll)
         return fail(PTC_EARG, "ptc_fields_create: %lld nodes exceed 32-bit node offsets",
                     (long long)nodes);
-    const Options o = options();
+    Options o = options();
+    if (prec == PTC_FP32) o.layout = LAYOUT_AOS;  // FP32 fields are always node rows
 
     double *soa = nullptr;
     HIP_TRY(hipMalloc(&soa, sizeof(double) * 6 * nodes));
@@ -167,9 +169,9 @@ int ptc_fields_create(const int dm[3], const double *f1, const double *bb, int m
     }
     void *data = soa, *bricks = nullptr;
     if (o.layout == LAYOUT_AOS || o.layout == LAYOUT_BRICK) {
-        double *aos = nullptr;
-        e = hipMalloc(&aos, sizeof(double) * 6 * nodes);
-        if (e == hipSuccess) e = launch_to_aos(soa, nodes, aos, PTC_FP64, nullptr);
+        void *aos = nullptr;
+        e = hipMalloc(&aos, prec == PTC_FP32 ? sizeof(float) * 8 * nodes : sizeof(double) * 6 * nodes);
+        if (e == hipSuccess) e = launch_to_aos(soa, nodes, aos, prec, nullptr);
         if (e == hipSuccess) e = hipDeviceSynchronize();
         (void)hipFree(soa);
         if (e != hipSuccess) {

@@ -29,6 +29,8 @@
 
 #include "particles_amd.h"
 
+#include <type_traits>
+
 namespace ptc {
 namespace {
 
@@ -47,17 +49,22 @@ __device__ __forceinline__ int wrap_idx(int i, int n)
     return (unsigned)i < (unsigned)n ? i : 0;
 }
 
-struct Corners {
+template <class W>
+struct CornersT {
     uint32_t o[8];  // node offsets of the 8 corners, reference order
-    double w[8];    // trilinear weights, reference order
+    W w[8];         // trilinear weights, reference order
     bool adjacent;  // right index == left + 1 (periodic) in every direction
 };
+using Corners = CornersT<double>;
 
 // Index and weight math of interpolations.F90:140-149 and :207-210.  The
 // right index is its own floor of x + 1/dm, as in the reference, so at a
 // cell face it can differ from left+1; an index equal to dm (x just below a
 // periodic image, where the reference reads one past the array) wraps to 0.
-__device__ __forceinline__ void corners(const StepParams &P, const double *pos, Corners &c)
+// W = float (mixed precision): the in-cell fractions are formed in FP64 as
+// above and rounded once; the weight products are FP32.
+template <class W>
+__device__ __forceinline__ void corners(const StepParams &P, const double *pos, CornersT<W> &c)
 {
     int l[3], r[3];
     double xd[3], xu[3];
@@ -85,18 +92,21 @@ __device__ __forceinline__ void corners(const StepParams &P, const double *pos, 
     const uint32_t xl = (uint32_t)l[0], xr = (uint32_t)r[0];
     c.o[0] = xl + yl + zl; c.o[1] = xl + yl + zr; c.o[2] = xl + yr + zl; c.o[3] = xl + yr + zr;
     c.o[4] = xr + yl + zl; c.o[5] = xr + yl + zr; c.o[6] = xr + yr + zl; c.o[7] = xr + yr + zr;
-    c.w[0] = xu[0] * xu[1] * xu[2];
-    c.w[1] = xu[0] * xu[1] * xd[2];
-    c.w[2] = xu[0] * xd[1] * xu[2];
-    c.w[3] = xu[0] * xd[1] * xd[2];
-    c.w[4] = xd[0] * xu[1] * xu[2];
-    c.w[5] = xd[0] * xu[1] * xd[2];
-    c.w[6] = xd[0] * xd[1] * xu[2];
-    c.w[7] = xd[0] * xd[1] * xd[2];
+    const W u0 = W(xu[0]), u1 = W(xu[1]), u2 = W(xu[2]);
+    const W d0 = W(xd[0]), d1 = W(xd[1]), d2 = W(xd[2]);
+    c.w[0] = u0 * u1 * u2;
+    c.w[1] = u0 * u1 * d2;
+    c.w[2] = u0 * d1 * u2;
+    c.w[3] = u0 * d1 * d2;
+    c.w[4] = d0 * u1 * u2;
+    c.w[5] = d0 * u1 * d2;
+    c.w[6] = d0 * d1 * u2;
+    c.w[7] = d0 * d1 * d2;
 }
 
 // f = w0 v0 + w1 v1 + ... + w7 v7, summed left to right per component
-__device__ __forceinline__ void accumulate_row(int j, double w, const double *v, double *f)
+template <class T>
+__device__ __forceinline__ void accumulate_row(int j, T w, const T *v, T *f)
 {
     if (j == 0) {
 #pragma unroll
@@ -181,6 +191,56 @@ __device__ __forceinline__ void force_from_fields(const StepParams &P, const dou
     }
 }
 
+// Mixed precision: the same force in FP32 from the FP64 stage point.
+template <int MODE>
+__device__ __forceinline__ void force_from_fields_f32(const StepParams &P, const double *s,
+                                                      const float *f, float *a)
+{
+    const float ux = float(s[3]), uy = float(s[4]), uz = float(s[5]);
+    const float g = sqrtf(ux * ux + uy * uy + uz * uz + 1.0f);
+    const float v0 = ux / g, v1 = uy / g, v2 = uz / g;
+    const float vun = float(P.vun), qom = float(P.qom);
+    a[0] = v0 / vun;
+    a[1] = v1 / vun;
+    a[2] = v2 / vun;
+    if constexpr (MODE == PTC_MODE_EB) {
+        a[3] = qom * (f[0] + (v1 * f[5] - v2 * f[4]));
+        a[4] = qom * (f[1] + (v2 * f[3] - v0 * f[5]));
+        a[5] = qom * (f[2] + (v0 * f[4] - v1 * f[3]));
+    } else {
+        const float w0 = v0 - f[0], w1 = v1 - f[1], w2 = v2 - f[2];
+        a[3] = qom * (w1 * f[5] - w2 * f[4]);
+        a[4] = qom * (w2 * f[3] - w0 * f[5]);
+        a[5] = qom * (w0 * f[4] - w1 * f[3]);
+    }
+}
+
+// FP32 nodes: eight floats (U or E x,y,z, B x,y,z, |B|, 0), 32 bytes, so a
+// corner is one 16-byte and one 8-byte load
+__device__ __forceinline__ void load_node_f32(const float *__restrict__ d, uint32_t o, float *v)
+{
+    const float4 a = *(const float4 *)(d + (size_t)o * 8);
+    const float2 b = *(const float2 *)(d + (size_t)o * 8 + 4);
+    v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w; v[4] = b.x; v[5] = b.y;
+}
+
+template <int MODE>
+__device__ __forceinline__ void rhs_f32(const DeviceFields &F, const StepParams &P, const double *s,
+                                        float *a)
+{
+    CornersT<float> c;
+    corners(P, s, c);
+    const float *__restrict__ d = (const float *)F.data;
+    float f[6];
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+        float v[6];
+        load_node_f32(d, c.o[j], v);
+        accumulate_row(j, c.w[j], v, f);
+    }
+    force_from_fields_f32<MODE>(P, s, f, a);
+}
+
 template <int MODE, int LAYOUT>
 __device__ __forceinline__ void rhs(const DeviceFields &F, const StepParams &P, const double *s,
                                     double *a)
@@ -197,18 +257,40 @@ __device__ __forceinline__ void rhs(const DeviceFields &F, const StepParams &P, 
 // values instead of loading them again.  The values and the summation order
 // are the same either way, so the result is bit-identical.  CACHE 0 reloads
 // every time (SOA and brick layouts).
-template <int MODE, int LAYOUT, int CACHE>
+// PREC = PTC_FP32 is the mixed-precision force: FP32 nodes, weights and
+// arithmetic, slopes returned as float (K = float).
+template <int MODE, int LAYOUT, int CACHE, int PREC>
 struct Force {
+    using K = std::conditional_t<PREC == PTC_FP32, float, double>;
     const DeviceFields &F;
     const StepParams &P;
     uint32_t key0 = 0xffffffffu, key7 = 0xffffffffu;
-    double v[CACHE == 1 ? 48 : 1];
+    K v[CACHE == 1 ? 48 : 1];
 
     __device__ __forceinline__ Force(const DeviceFields &F_, const StepParams &P_) : F(F_), P(P_) {}
 
-    __device__ __forceinline__ void operator()(const double *s, double *a)
+    __device__ __forceinline__ void operator()(const double *s, K *a)
     {
-        if constexpr (CACHE == 0) {
+        if constexpr (PREC == PTC_FP32) {
+            static_assert(LAYOUT == LAYOUT_AOS, "FP32 fields are stored as AOS nodes");
+            if constexpr (CACHE == 0) {
+                rhs_f32<MODE>(F, P, s, a);
+            } else {
+                CornersT<float> c;
+                corners(P, s, c);
+                if (c.o[0] != key0 || c.o[7] != key7) {
+                    const float *__restrict__ d = (const float *)F.data;
+#pragma unroll
+                    for (int j = 0; j < 8; j++) load_node_f32(d, c.o[j], &v[6 * j]);
+                    key0 = c.o[0];
+                    key7 = c.o[7];
+                }
+                float f[6];
+#pragma unroll
+                for (int j = 0; j < 8; j++) accumulate_row(j, c.w[j], &v[6 * j], f);
+                force_from_fields_f32<MODE>(P, s, f, a);
+            }
+        } else if constexpr (CACHE == 0) {
             rhs<MODE, LAYOUT>(F, P, s, a);
         } else {
             static_assert(LAYOUT == LAYOUT_AOS, "the cell cache reads AOS node rows");
@@ -237,8 +319,9 @@ struct Force {
 // ---------------------------------------------------------------------------
 // initial step, sources/integrations.F90:524-589 (ki = f(si) is given)
 // ---------------------------------------------------------------------------
+template <class K>
 __device__ __forceinline__ double dti_guess(const StepParams &P, int iord, const double *si,
-                                            const double *ki, const PowTables &T)
+                                            const K *ki, const PowTables &T)
 {
     double sr[6], s, q, d0, d1, d2, dti, dtf;
 #pragma unroll
@@ -249,13 +332,13 @@ __device__ __forceinline__ double dti_guess(const StepParams &P, int iord, const
     d0 = sqrt(s / 6.0);
     s = 0.0;
 #pragma unroll
-    for (int i = 0; i < 6; i++) { q = ki[i] / sr[i]; s = s + q * q; }
+    for (int i = 0; i < 6; i++) { q = double(ki[i]) / sr[i]; s = s + q * q; }
     d1 = sqrt(s / 6.0);
     dti = (fmin(d0, d1) < 1.0e-10) ? 1.0e-06 : 1.0e-01 * (d0 / d1);
     s = 0.0;
 #pragma unroll
     for (int i = 0; i < 6; i++) {
-        const double sf = si[i] + dti * ki[i];
+        const double sf = si[i] + dti * double(ki[i]);
         q = (sf - si[i]) / sr[i];
         s = s + q * q;
     }
@@ -304,42 +387,49 @@ __device__ __forceinline__ double dti_guess(const StepParams &P, int iord, const
 #define CK_D6 (277.0 / 7084.0)
 
 // k1 = dt f(si) from the cached derivative f1; returns del (error / atol),
-// sf, and the stage-6 slope k6 used by the dense output.
+// sf, and the stage-6 slope k6 used by the dense output.  Stage slopes are
+// kept in the RHS's type K (double, or float in mixed precision); stage
+// points, sums and the error norm are always FP64.
 template <class Rhs>
 __device__ __forceinline__ double attempt_rk5(Rhs &rhsf, const StepParams &P, const double *si,
-                                              const double *f1, double dt, double *sf, double *k6)
+                                              const typename Rhs::K *f1, double dt, double *sf,
+                                              double *k6d)
 {
-    double k1[6], k2[6], k3[6], k4[6], k5[6], s[6], a[6];
+    using K = typename Rhs::K;
+    K k1[6], k2[6], k3[6], k4[6], k5[6], k6[6], a[6];
+    double s[6];
 #pragma unroll
-    for (int c = 0; c < 6; c++) k1[c] = dt * f1[c];
+    for (int c = 0; c < 6; c++) k1[c] = K(dt * double(f1[c]));
 #pragma unroll
-    for (int c = 0; c < 6; c++) s[c] = si[c] + CK_A21 * k1[c];
+    for (int c = 0; c < 6; c++) s[c] = si[c] + CK_A21 * double(k1[c]);
     rhsf(s, a);
 #pragma unroll
-    for (int c = 0; c < 6; c++) k2[c] = dt * a[c];
+    for (int c = 0; c < 6; c++) k2[c] = K(dt * double(a[c]));
 #pragma unroll
-    for (int c = 0; c < 6; c++) s[c] = si[c] + CK_A31 * k1[c] + CK_A32 * k2[c];
+    for (int c = 0; c < 6; c++) s[c] = si[c] + CK_A31 * double(k1[c]) + CK_A32 * double(k2[c]);
     rhsf(s, a);
 #pragma unroll
-    for (int c = 0; c < 6; c++) k3[c] = dt * a[c];
-#pragma unroll
-    for (int c = 0; c < 6; c++) s[c] = si[c] + CK_A41 * k1[c] + CK_A42 * k2[c] + CK_A43 * k3[c];
-    rhsf(s, a);
-#pragma unroll
-    for (int c = 0; c < 6; c++) k4[c] = dt * a[c];
+    for (int c = 0; c < 6; c++) k3[c] = K(dt * double(a[c]));
 #pragma unroll
     for (int c = 0; c < 6; c++)
-        s[c] = si[c] + CK_A51 * k1[c] + CK_A52 * k2[c] + CK_A53 * k3[c] + CK_A54 * k4[c];
+        s[c] = si[c] + CK_A41 * double(k1[c]) + CK_A42 * double(k2[c]) + CK_A43 * double(k3[c]);
     rhsf(s, a);
 #pragma unroll
-    for (int c = 0; c < 6; c++) k5[c] = dt * a[c];
+    for (int c = 0; c < 6; c++) k4[c] = K(dt * double(a[c]));
 #pragma unroll
     for (int c = 0; c < 6; c++)
-        s[c] = si[c] + CK_A61 * k1[c] + CK_A62 * k2[c] + CK_A63 * k3[c] + CK_A64 * k4[c] +
-               CK_A65 * k5[c];
+        s[c] = si[c] + CK_A51 * double(k1[c]) + CK_A52 * double(k2[c]) + CK_A53 * double(k3[c]) +
+               CK_A54 * double(k4[c]);
     rhsf(s, a);
 #pragma unroll
-    for (int c = 0; c < 6; c++) k6[c] = dt * a[c];
+    for (int c = 0; c < 6; c++) k5[c] = K(dt * double(a[c]));
+#pragma unroll
+    for (int c = 0; c < 6; c++)
+        s[c] = si[c] + CK_A61 * double(k1[c]) + CK_A62 * double(k2[c]) + CK_A63 * double(k3[c]) +
+               CK_A64 * double(k4[c]) + CK_A65 * double(k5[c]);
+    rhsf(s, a);
+#pragma unroll
+    for (int c = 0; c < 6; c++) k6[c] = K(dt * double(a[c]));
 
     const double eps = 2.2250738585072014e-308;  // tiny(1d0), integrations.F90:99
     double del = 0.0;
@@ -347,22 +437,23 @@ __device__ __forceinline__ double attempt_rk5(Rhs &rhsf, const StepParams &P, co
     for (int c = 0; c < 6; c++) {
         // sum(c(1:6) k(1:6,i)) and sum(d(1:6) k(1:6,i)), zero weights kept
         double sc = 0.0;
-        sc = sc + CK_C1 * k1[c];
-        sc = sc + 0.0 * k2[c];
-        sc = sc + CK_C3 * k3[c];
-        sc = sc + CK_C4 * k4[c];
-        sc = sc + 0.0 * k5[c];
-        sc = sc + CK_C6 * k6[c];
+        sc = sc + CK_C1 * double(k1[c]);
+        sc = sc + 0.0 * double(k2[c]);
+        sc = sc + CK_C3 * double(k3[c]);
+        sc = sc + CK_C4 * double(k4[c]);
+        sc = sc + 0.0 * double(k5[c]);
+        sc = sc + CK_C6 * double(k6[c]);
         sf[c] = si[c] + sc;
         double se = 0.0;
-        se = se + CK_D1 * k1[c];
-        se = se + 0.0 * k2[c];
-        se = se + CK_D3 * k3[c];
-        se = se + CK_D4 * k4[c];
-        se = se + CK_D5 * k5[c];
-        se = se + CK_D6 * k6[c];
-        const double q = fabs(se / (fabs(si[c]) + fabs(k1[c]) + eps));
+        se = se + CK_D1 * double(k1[c]);
+        se = se + 0.0 * double(k2[c]);
+        se = se + CK_D3 * double(k3[c]);
+        se = se + CK_D4 * double(k4[c]);
+        se = se + CK_D5 * double(k5[c]);
+        se = se + CK_D6 * double(k6[c]);
+        const double q = fabs(se / (fabs(si[c]) + fabs(double(k1[c])) + eps));
         if (c == 0 || q > del) del = q;
+        k6d[c] = double(k6[c]);
     }
     return del / P.atol;
 }
@@ -442,77 +533,80 @@ __device__ __forceinline__ double attempt_rk5(Rhs &rhsf, const StepParams &P, co
 #define ER12 -0.2235530786388629525884427845e-01
 
 // k1 = f(si) is given (raw derivatives, no dt inside, as in the reference);
-// returns err and sf.
+// returns err and sf.  Slopes are stored in the RHS's type K; stage points,
+// sums and the error norm are FP64.
 template <class Rhs>
 __device__ __forceinline__ double attempt_dp853(Rhs &rhsf, const StepParams &P, const double *si,
-                                                const double *k1, double dt, double *sf)
+                                                const typename Rhs::K *k1, double dt, double *sf)
 {
-    double k2[6], k3[6], k4[6], k5[6], k6[6], k7[6], k8[6], k9[6], k10[6], k11[6], k12[6], ss[6];
+    using K = typename Rhs::K;
+    K k2[6], k3[6], k4[6], k5[6], k6[6], k7[6], k8[6], k9[6], k10[6], k11[6], k12[6];
+    double ss[6];
 #define RHS(out) rhsf(ss, out)
 #pragma unroll
-    for (int i = 0; i < 6; i++) ss[i] = si[i] + dt * A0201 * k1[i];
+    for (int i = 0; i < 6; i++) ss[i] = si[i] + dt * A0201 * double(k1[i]);
     RHS(k2);
 #pragma unroll
-    for (int i = 0; i < 6; i++) ss[i] = si[i] + dt * (A0301 * k1[i] + A0302 * k2[i]);
+    for (int i = 0; i < 6; i++) ss[i] = si[i] + dt * (A0301 * double(k1[i]) + A0302 * double(k2[i]));
     RHS(k3);
 #pragma unroll
-    for (int i = 0; i < 6; i++) ss[i] = si[i] + dt * (A0401 * k1[i] + A0403 * k3[i]);
+    for (int i = 0; i < 6; i++) ss[i] = si[i] + dt * (A0401 * double(k1[i]) + A0403 * double(k3[i]));
     RHS(k4);
 #pragma unroll
-    for (int i = 0; i < 6; i++) ss[i] = si[i] + dt * (A0501 * k1[i] + A0503 * k3[i] + A0504 * k4[i]);
+    for (int i = 0; i < 6; i++) ss[i] = si[i] + dt * (A0501 * double(k1[i]) + A0503 * double(k3[i]) + A0504 * double(k4[i]));
     RHS(k5);
 #pragma unroll
-    for (int i = 0; i < 6; i++) ss[i] = si[i] + dt * (A0601 * k1[i] + A0604 * k4[i] + A0605 * k5[i]);
+    for (int i = 0; i < 6; i++) ss[i] = si[i] + dt * (A0601 * double(k1[i]) + A0604 * double(k4[i]) + A0605 * double(k5[i]));
     RHS(k6);
 #pragma unroll
     for (int i = 0; i < 6; i++)
-        ss[i] = si[i] + dt * (A0701 * k1[i] + A0704 * k4[i] + A0705 * k5[i] + A0706 * k6[i]);
+        ss[i] = si[i] + dt * (A0701 * double(k1[i]) + A0704 * double(k4[i]) + A0705 * double(k5[i]) + A0706 * double(k6[i]));
     RHS(k7);
 #pragma unroll
     for (int i = 0; i < 6; i++)
-        ss[i] = si[i] + dt * (A0801 * k1[i] + A0804 * k4[i] + A0805 * k5[i] + A0806 * k6[i] +
-                              A0807 * k7[i]);
+        ss[i] = si[i] + dt * (A0801 * double(k1[i]) + A0804 * double(k4[i]) + A0805 * double(k5[i]) + A0806 * double(k6[i]) +
+                              A0807 * double(k7[i]));
     RHS(k8);
 #pragma unroll
     for (int i = 0; i < 6; i++)
-        ss[i] = si[i] + dt * (A0901 * k1[i] + A0904 * k4[i] + A0905 * k5[i] + A0906 * k6[i] +
-                              A0907 * k7[i] + A0908 * k8[i]);
+        ss[i] = si[i] + dt * (A0901 * double(k1[i]) + A0904 * double(k4[i]) + A0905 * double(k5[i]) + A0906 * double(k6[i]) +
+                              A0907 * double(k7[i]) + A0908 * double(k8[i]));
     RHS(k9);
 #pragma unroll
     for (int i = 0; i < 6; i++)
-        ss[i] = si[i] + dt * (A1001 * k1[i] + A1004 * k4[i] + A1005 * k5[i] + A1006 * k6[i] +
-                              A1007 * k7[i] + A1008 * k8[i] + A1009 * k9[i]);
+        ss[i] = si[i] + dt * (A1001 * double(k1[i]) + A1004 * double(k4[i]) + A1005 * double(k5[i]) + A1006 * double(k6[i]) +
+                              A1007 * double(k7[i]) + A1008 * double(k8[i]) + A1009 * double(k9[i]));
     RHS(k10);
 #pragma unroll
     for (int i = 0; i < 6; i++)
-        ss[i] = si[i] + dt * (A1101 * k1[i] + A1104 * k4[i] + A1105 * k5[i] + A1106 * k6[i] +
-                              A1107 * k7[i] + A1108 * k8[i] + A1109 * k9[i] + A1110 * k10[i]);
+        ss[i] = si[i] + dt * (A1101 * double(k1[i]) + A1104 * double(k4[i]) + A1105 * double(k5[i]) + A1106 * double(k6[i]) +
+                              A1107 * double(k7[i]) + A1108 * double(k8[i]) + A1109 * double(k9[i]) + A1110 * double(k10[i]));
     RHS(k11);
 #pragma unroll
     for (int i = 0; i < 6; i++)
-        ss[i] = si[i] + dt * (A1201 * k1[i] + A1204 * k4[i] + A1205 * k5[i] + A1206 * k6[i] +
-                              A1207 * k7[i] + A1208 * k8[i] + A1209 * k9[i] + A1210 * k10[i] +
-                              A1211 * k11[i]);
+        ss[i] = si[i] + dt * (A1201 * double(k1[i]) + A1204 * double(k4[i]) + A1205 * double(k5[i]) + A1206 * double(k6[i]) +
+                              A1207 * double(k7[i]) + A1208 * double(k8[i]) + A1209 * double(k9[i]) + A1210 * double(k10[i]) +
+                              A1211 * double(k11[i]));
     RHS(k12);
 #undef RHS
 
     double err3 = 0.0, err5 = 0.0;
 #pragma unroll
     for (int i = 0; i < 6; i++) {
-        const double kb = DB01 * k1[i] + DB06 * k6[i] + DB07 * k7[i] + DB08 * k8[i] + DB09 * k9[i] +
-                          DB10 * k10[i] + DB11 * k11[i] + DB12 * k12[i];
+        const double kb = DB01 * double(k1[i]) + DB06 * double(k6[i]) + DB07 * double(k7[i]) + DB08 * double(k8[i]) + DB09 * double(k9[i]) +
+                          DB10 * double(k10[i]) + DB11 * double(k11[i]) + DB12 * double(k12[i]);
         sf[i] = si[i] + dt * kb;
         const double a = fabs(si[i]), b = fabs(sf[i]);
         const double sr = P.atol + P.rtol * (a > b ? a : b);
-        const double e3 = (kb - BH1 * k1[i] - BH2 * k9[i] - BH3 * k12[i]) / sr;
+        const double e3 = (kb - BH1 * double(k1[i]) - BH2 * double(k9[i]) - BH3 * double(k12[i])) / sr;
         err3 = err3 + e3 * e3;
     }
 #pragma unroll
     for (int i = 0; i < 6; i++) {
         const double a = fabs(si[i]), b = fabs(sf[i]);
         const double sr = P.atol + P.rtol * (a > b ? a : b);
-        const double e5 = (ER01 * k1[i] + ER06 * k6[i] + ER07 * k7[i] + ER08 * k8[i] + ER09 * k9[i] +
-                           ER10 * k10[i] + ER11 * k11[i] + ER12 * k12[i]) / sr;
+        const double e5 = (ER01 * double(k1[i]) + ER06 * double(k6[i]) + ER07 * double(k7[i]) + ER08 * double(k8[i]) + ER09 * double(k9[i]) +
+                           ER10 * double(k10[i]) + ER11 * double(k11[i]) + ER12 * double(k12[i])) / sr;
         err5 = err5 + e5 * e5;
     }
     const double deno = err5 + 1.0e-02 * err3;
@@ -549,7 +643,7 @@ __device__ __forceinline__ bool finite6(const double *a)
 // ---------------------------------------------------------------------------
 // persistent work-queue integrator
 // ---------------------------------------------------------------------------
-template <int METHOD, int MODE, int LAYOUT, int CACHE>
+template <int METHOD, int MODE, int LAYOUT, int CACHE, int PREC>
 __global__ void __launch_bounds__(kBlock)
 integrate_kernel(const DeviceFields F, const StepParams P, const DeviceParticles S, double t_stop)
 {
@@ -563,14 +657,16 @@ integrate_kernel(const DeviceFields F, const StepParams P, const DeviceParticles
     __syncthreads();
     const PowTables T{s_logt, s_expt};
 
-    Force<MODE, LAYOUT, CACHE> force(F, P);
+    Force<MODE, LAYOUT, CACHE, PREC> force(F, P);
+    using K = typename decltype(force)::K;
     const int lane = (int)__lane_id();
     const double t_end = t_stop < P.tmx ? t_stop : P.tmx;
     const int64_t N = S.np;
 
     bool active = false, exhausted = false, need_rhs = false, fresh = false;
     int64_t idx = 0;
-    double si[6], f1[6];
+    double si[6];
+    K f1[6];
     double t = 0.0, dt = 0.0, errold = 1.0e-04, tnext = 0.0;
     int32_t n = 2, flags = 0, status = PTC_STATUS_RUNNING;
     int64_t c_all = 0, c_acc = 0, c_rej = 0, c_rhs = 0;
@@ -676,7 +772,7 @@ integrate_kernel(const DeviceFields F, const StepParams P, const DeviceParticles
                         if (n <= P.m && t >= tnext) {
                             double k1[6];
 #pragma unroll
-                            for (int c = 0; c < 6; c++) k1[c] = dt * f1[c];
+                            for (int c = 0; c < 6; c++) k1[c] = dt * double(f1[c]);
                             do {
                                 hermite_out(S.out + (size_t)8 * ((n - 1) + (size_t)P.m * idx), t,
                                             tnext, dt, si, sf, k1, k6);
@@ -708,11 +804,15 @@ integrate_kernel(const DeviceFields F, const StepParams P, const DeviceParticles
                         if (n <= P.m && t >= tnext) {
                             // the end slope f(sf) is needed now; it is also
                             // the next step's stage 1
-                            double fe[6], ka[6], kb[6];
+                            K fe[6];
+                            double ka[6], kb[6];
                             force(sf, fe);
                             c_rhs++;
 #pragma unroll
-                            for (int c = 0; c < 6; c++) { ka[c] = dt * f1[c]; kb[c] = dt * fe[c]; }
+                            for (int c = 0; c < 6; c++) {
+                                ka[c] = dt * double(f1[c]);
+                                kb[c] = dt * double(fe[c]);
+                            }
                             do {
                                 hermite_out(S.out + (size_t)8 * ((n - 1) + (size_t)P.m * idx), t,
                                             tnext, dt, si, sf, ka, kb);
@@ -778,42 +878,58 @@ integrate_kernel(const DeviceFields F, const StepParams P, const DeviceParticles
 //   op 1: RHS at vec(6)             -> acc(6)
 //   op 2: dti_guess, order 5, vec(6) -> dt(1) (op 3: order 8)
 //   op 4: step-size power x^y, (x, y) -> (1)
-template <int MODE, int LAYOUT>
+template <int MODE, int LAYOUT, int PREC>
 __global__ void probe_kernel(const DeviceFields F, const StepParams P, int op, int64_t n,
                              const double *__restrict__ in, double *__restrict__ out)
 {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= n) return;
-    double v[6], r[6];
+    double v[6];
+    Force<MODE, LAYOUT, 0, PREC> force(F, P);
+    typename decltype(force)::K r[6];
     if (op == 0) {
 #pragma unroll
         for (int k = 0; k < 3; k++) v[k] = in[3 * i + k];
-        gather<LAYOUT>(F, P, v, r);
+        if constexpr (PREC == PTC_FP32) {
+            CornersT<float> c;
+            corners(P, v, c);
+            float f[6];
 #pragma unroll
-        for (int k = 0; k < 6; k++) out[6 * i + k] = r[k];
+            for (int j = 0; j < 8; j++) {
+                float w[6];
+                load_node_f32((const float *)F.data, c.o[j], w);
+                accumulate_row(j, c.w[j], w, f);
+            }
+#pragma unroll
+            for (int k = 0; k < 6; k++) r[k] = f[k];
+        } else {
+            gather<LAYOUT>(F, P, v, r);
+        }
+#pragma unroll
+        for (int k = 0; k < 6; k++) out[6 * i + k] = double(r[k]);
     } else if (op == 1) {
 #pragma unroll
         for (int k = 0; k < 6; k++) v[k] = in[6 * i + k];
-        rhs<MODE, LAYOUT>(F, P, v, r);
+        force(v, r);
 #pragma unroll
-        for (int k = 0; k < 6; k++) out[6 * i + k] = r[k];
+        for (int k = 0; k < 6; k++) out[6 * i + k] = double(r[k]);
     } else if (op == 2 || op == 3) {
 #pragma unroll
         for (int k = 0; k < 6; k++) v[k] = in[6 * i + k];
-        rhs<MODE, LAYOUT>(F, P, v, r);
+        force(v, r);
         out[i] = dti_guess(P, op == 2 ? 5 : 8, v, r, PowTables{ptc_pow_log_tab, ptc_pow_exp_tab});
     } else if (op == 4) {
         out[i] = ptc_libm_pow(in[2 * i], in[2 * i + 1]);
     }
 }
 
-template <int MODE, int LAYOUT>
+template <int MODE, int LAYOUT, int PREC = PTC_FP64>
 hipError_t launch_probe_t(const DeviceFields &f, const StepParams &sp, int op, int64_t n,
                           const double *in, double *out, hipStream_t stream)
 {
     const int64_t blocks = (n + 255) / 256;
-    hipLaunchKernelGGL((probe_kernel<MODE, LAYOUT>), dim3((unsigned)blocks), dim3(256), 0, stream, f,
-                       sp, op, n, in, out);
+    hipLaunchKernelGGL((probe_kernel<MODE, LAYOUT, PREC>), dim3((unsigned)blocks), dim3(256), 0,
+                       stream, f, sp, op, n, in, out);
     return hipGetLastError();
 }
 
@@ -891,11 +1007,32 @@ __global__ void to_bricks_kernel(const double *__restrict__ aos, int dm0, int dm
     }
 }
 
-template <int METHOD, int MODE, int LAYOUT, int CACHE>
+// SOA planes (FP64) -> FP32 nodes (U or E, B, |B| from the FP64 values, 0)
+__global__ void to_nodes_f32_kernel(const double *__restrict__ soa, int64_t nodes,
+                                    float *__restrict__ out)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nodes;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const double bx = soa[3 * nodes + i], by = soa[4 * nodes + i], bz = soa[5 * nodes + i];
+        float4 a, b;
+        a.x = float(soa[i]);
+        a.y = float(soa[nodes + i]);
+        a.z = float(soa[2 * nodes + i]);
+        a.w = float(bx);
+        b.x = float(by);
+        b.y = float(bz);
+        b.z = float(sqrt(bx * bx + by * by + bz * bz));
+        b.w = 0.0f;
+        ((float4 *)out)[2 * i] = a;
+        ((float4 *)out)[2 * i + 1] = b;
+    }
+}
+
+template <int METHOD, int MODE, int LAYOUT, int CACHE, int PREC>
 hipError_t launch_t(const DeviceFields &f, const StepParams &sp, const DeviceParticles &p,
                     double t_stop, LaunchConfig lc, hipStream_t stream)
 {
-    auto kern = integrate_kernel<METHOD, MODE, LAYOUT, CACHE>;
+    auto kern = integrate_kernel<METHOD, MODE, LAYOUT, CACHE, PREC>;
     int grid = lc.grid_blocks;
     if (grid <= 0) {
         int dev = 0, cus = 0, per_cu = 0;
@@ -920,26 +1057,22 @@ hipError_t launch_integrate(int method, const DeviceFields &f, const StepParams 
                             const DeviceParticles &p, double t_stop, const LaunchConfig &lc,
                             hipStream_t stream)
 {
-#define DISPATCH(M, MO, L, C)                                                  \
-    if (method == M && f.mode == MO && f.layout == L && lc.cell_cache == C)    \
-        return launch_t<M, MO, L, C>(f, sp, p, t_stop, lc, stream);
-    if (f.prec != PTC_FP64) return hipErrorInvalidValue;
-    DISPATCH(PTC_RK5, PTC_MODE_VB, LAYOUT_AOS, 1)
-    DISPATCH(PTC_RK5, PTC_MODE_EB, LAYOUT_AOS, 1)
-    DISPATCH(PTC_DOP853, PTC_MODE_VB, LAYOUT_AOS, 1)
-    DISPATCH(PTC_DOP853, PTC_MODE_EB, LAYOUT_AOS, 1)
-    DISPATCH(PTC_RK5, PTC_MODE_VB, LAYOUT_SOA, 0)
-    DISPATCH(PTC_RK5, PTC_MODE_VB, LAYOUT_AOS, 0)
-    DISPATCH(PTC_RK5, PTC_MODE_EB, LAYOUT_SOA, 0)
-    DISPATCH(PTC_RK5, PTC_MODE_EB, LAYOUT_AOS, 0)
-    DISPATCH(PTC_DOP853, PTC_MODE_VB, LAYOUT_SOA, 0)
-    DISPATCH(PTC_DOP853, PTC_MODE_VB, LAYOUT_AOS, 0)
-    DISPATCH(PTC_DOP853, PTC_MODE_EB, LAYOUT_SOA, 0)
-    DISPATCH(PTC_DOP853, PTC_MODE_EB, LAYOUT_AOS, 0)
-    DISPATCH(PTC_RK5, PTC_MODE_VB, LAYOUT_BRICK, 0)
-    DISPATCH(PTC_RK5, PTC_MODE_EB, LAYOUT_BRICK, 0)
-    DISPATCH(PTC_DOP853, PTC_MODE_VB, LAYOUT_BRICK, 0)
-    DISPATCH(PTC_DOP853, PTC_MODE_EB, LAYOUT_BRICK, 0)
+#define DISPATCH(M, MO, L, C, PR)                                              \
+    if (method == M && f.mode == MO && f.layout == L && lc.cell_cache == C &&  \
+        f.prec == PR)                                                          \
+        return launch_t<M, MO, L, C, PR>(f, sp, p, t_stop, lc, stream);
+#define DISPATCH_MODES(L, C, PR)                                               \
+    DISPATCH(PTC_RK5, PTC_MODE_VB, L, C, PR)                                   \
+    DISPATCH(PTC_RK5, PTC_MODE_EB, L, C, PR)                                   \
+    DISPATCH(PTC_DOP853, PTC_MODE_VB, L, C, PR)                                \
+    DISPATCH(PTC_DOP853, PTC_MODE_EB, L, C, PR)
+    DISPATCH_MODES(LAYOUT_AOS, 1, PTC_FP64)
+    DISPATCH_MODES(LAYOUT_AOS, 0, PTC_FP64)
+    DISPATCH_MODES(LAYOUT_SOA, 0, PTC_FP64)
+    DISPATCH_MODES(LAYOUT_BRICK, 0, PTC_FP64)
+    DISPATCH_MODES(LAYOUT_AOS, 1, PTC_FP32)
+    DISPATCH_MODES(LAYOUT_AOS, 0, PTC_FP32)
+#undef DISPATCH_MODES
 #undef DISPATCH
     return hipErrorInvalidValue;
 }
@@ -948,7 +1081,12 @@ hipError_t launch_probe(const DeviceFields &f, const StepParams &sp, int op, int
                         const double *in, double *out, hipStream_t stream)
 {
     if (n <= 0) return hipSuccess;
-    if (f.prec != PTC_FP64 || op < 0 || op > 4) return hipErrorInvalidValue;
+    if (op < 0 || op > 4) return hipErrorInvalidValue;
+    if (f.prec == PTC_FP32) {
+        if (f.mode == PTC_MODE_VB)
+            return launch_probe_t<PTC_MODE_VB, LAYOUT_AOS, PTC_FP32>(f, sp, op, n, in, out, stream);
+        return launch_probe_t<PTC_MODE_EB, LAYOUT_AOS, PTC_FP32>(f, sp, op, n, in, out, stream);
+    }
     if (f.mode == PTC_MODE_VB && f.layout == LAYOUT_SOA)
         return launch_probe_t<PTC_MODE_VB, LAYOUT_SOA>(f, sp, op, n, in, out, stream);
     if (f.mode == PTC_MODE_VB && f.layout == LAYOUT_AOS)
@@ -1002,9 +1140,13 @@ hipError_t launch_zero_i64(int64_t *ptr, int64_t n, hipStream_t stream)
 
 hipError_t launch_to_aos(const double *soa, int64_t nodes, void *aos, int prec, hipStream_t stream)
 {
-    if (prec != PTC_FP64) return hipErrorInvalidValue;
     int64_t blocks = (nodes + 255) / 256;
     if (blocks > 65536) blocks = 65536;
+    if (prec == PTC_FP32) {
+        hipLaunchKernelGGL(to_nodes_f32_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, soa,
+                           nodes, (float *)aos);
+        return hipGetLastError();
+    }
     hipLaunchKernelGGL(to_aos_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, soa, nodes,
                        (double *)aos);
     return hipGetLastError();

@@ -274,3 +274,57 @@ def test_energy_conserved_without_electric_field(oracle):
     rel_o = np.abs(so[6, 1:, :] - so[6, 0, :]) / so[6, 0, :]
     say(f"[energy] max relative gamma drift {rel.max():.3e} (oracle {rel_o.max():.3e})")
     assert (status == 0).all() and np.array_equal(rel, rel_o) and rel.max() < 1e-6
+
+
+# ---------------------------------------------------------------------------
+# mixed precision (FP32 fields, weights and RHS; FP64 state, sums, error norm)
+# ---------------------------------------------------------------------------
+
+def test_mixed_probes_close_to_reference():
+    """FP32 node storage and FP32 weights: the gather and the RHS agree with
+    the reference's FP64 values to FP32 accuracy (tolerance 1e-6 relative to
+    the field scale)."""
+    g = load_golden("probe_dm9x12x8")
+    with P.Fields(g["uu"], g["bb"], prec="fp32") as f:
+        fb = P.probe(f, "gather", g["pos"], g["params"])
+        acc = P.probe(f, "rhs", g["vec"], g["params"])
+    scale_u = np.abs(g["uu"]).max()
+    scale_b = np.abs(g["bb"]).max()
+    eu = np.abs(fb[:, :3] - g["u"]).max() / scale_u
+    eb = np.abs(fb[:, 3:] - g["b"]).max() / scale_b
+    ea = np.abs(acc - g["acc"]).max(0) / np.maximum(np.abs(g["acc"]).max(0), 1e-300)
+    say(f"[mixed probe] gather rel err U {eu:.2e} B {eb:.2e}; rhs rel err {ea.max():.2e}")
+    assert eu < 1e-6 and eb < 1e-6 and ea.max() < 1e-6
+
+
+@pytest.mark.parametrize("method", ["rk5", "dop853"])
+def test_mixed_precision_error_against_oracle(oracle, method):
+    """Mixed precision differs from the FP64 oracle by FP32 rounding in the
+    RHS; over 0.1 gyro-periods the median relative deviation in position
+    and momentum stays below 1e-6 and every particle below 1e-3 (the bound
+    is set by chaotic growth of FP32-level differences, which the FP64 path
+    does not have)."""
+    uu, bb = W.turbulent_vb(32, seed=1)
+    p = W.params_vector(tmax=0.1 * W.T0, dtmax=1.0, atol=1e-8, rtol=1e-8)
+    tg = W.time_grid(2, p[2])
+    st0 = W.initial_state(512, len(tg), seed=2, gamma_range=(1.0, 1.0e3))
+    so, co, _ = oracle.integrate(method, uu, bb, p, tg, st0)
+    with P.Fields(uu, bb, prec="fp32") as f:
+        st, cnt, status = P.integrate(f, method, p, tg, st0)
+    dev = deviation(st, so)
+    say(f"[mixed] {method}: deviation median {np.median(dev):.2e} max {dev.max():.2e}")
+    assert (status == 0).all()
+    assert np.median(dev) < 1e-6 and dev.max() < 1e-3
+
+
+def test_mixed_precision_energy_without_electric_field():
+    _, bb = W.turbulent_vb(24, seed=2)
+    uu = np.zeros_like(bb, order="F")
+    p = W.params_vector(tmax=5 * W.T0, dtmax=0.5, atol=1e-8, rtol=1e-8)
+    tg = W.time_grid(5, p[2])
+    st0 = W.initial_state(256, len(tg), seed=5, gamma_range=(2.0, 100.0))
+    with P.Fields(uu, bb, prec="fp32") as f:
+        st, _, status = P.integrate(f, "dop853", p, tg, st0)
+    rel = np.abs(st[6, 1:, :] - st[6, 0, :]) / st[6, 0, :]
+    say(f"[mixed energy] max relative gamma drift {rel.max():.3e}")
+    assert (status == 0).all() and rel.max() < 1e-6
