@@ -1,0 +1,110 @@
+!! ptc_binding -- Fortran 2003 interfaces (iso_c_binding) to the C-ABI of
+!! libparticles_amd.so (include/particles_amd.h).
+!!
+!! With this module the reference driver's particle loop
+!! (sources/particles.F90:337-356) becomes one call:
+!!
+!!   use ptc_binding
+!!   type(c_ptr) :: fh
+!!   iret = ptc_fields_create(dm, uu, bb, PTC_MODE_VB, PTC_FP64, fh)
+!!   iret = ptc_integrate(fh, PTC_DOP853, int(nparticles, c_int64_t), nsteps, &
+!!                        params, size(params), time, state, counter, c_null_ptr)
+!!   iret = ptc_fields_destroy(fh)
+!!
+!! Arrays are passed as the reference declares them: uu/bb(dm1,dm2,dm3,3),
+!! params(32), time(nsteps), state(8,nsteps,nparticles), counter(3,nparticles).
+module ptc_binding
+
+  use iso_c_binding
+  implicit none
+
+  integer(c_int), parameter :: PTC_OK = 0, PTC_EARG = 1, PTC_EHIP = 2, PTC_ENOMEM = 3, PTC_ESTATE = 4
+  integer(c_int), parameter :: PTC_RK5 = 0, PTC_DOP853 = 1
+  integer(c_int), parameter :: PTC_MODE_VB = 0, PTC_MODE_EB = 1
+  integer(c_int), parameter :: PTC_FP64 = 0, PTC_FP32 = 1
+
+  interface
+
+    integer(c_int) function ptc_fields_create(dm, f1, bb, mode, prec, out) bind(C, name="ptc_fields_create")
+      import :: c_int, c_double, c_ptr
+      integer(c_int), intent(in) :: dm(3)
+      real(c_double), intent(in) :: f1(*), bb(*)
+      integer(c_int), value      :: mode, prec
+      type(c_ptr), intent(out)   :: out
+    end function
+
+    integer(c_int) function ptc_fields_destroy(f) bind(C, name="ptc_fields_destroy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: f
+    end function
+
+    integer(c_int) function ptc_integrate(f, method, np, m, params, nparams, time, state, counter, &
+                                          status) bind(C, name="ptc_integrate")
+      import :: c_int, c_int64_t, c_double, c_ptr
+      type(c_ptr), value             :: f
+      integer(c_int), value          :: method, m, nparams
+      integer(c_int64_t), value      :: np
+      real(c_double), intent(in)     :: params(*), time(*)
+      real(c_double), intent(inout)  :: state(*)
+      integer(c_int64_t), intent(out):: counter(*)
+      type(c_ptr), value             :: status
+    end function
+
+    integer(c_int) function ptc_particles_create(f, method, np, m, params, nparams, time, state, out) &
+                                                 bind(C, name="ptc_particles_create")
+      import :: c_int, c_int64_t, c_double, c_ptr
+      type(c_ptr), value         :: f
+      integer(c_int), value      :: method, m, nparams
+      integer(c_int64_t), value  :: np
+      real(c_double), intent(in) :: params(*), time(*), state(*)
+      type(c_ptr), intent(out)   :: out
+    end function
+
+    integer(c_int) function ptc_particles_advance(p, t_stop) bind(C, name="ptc_particles_advance")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value    :: p
+      real(c_double), value :: t_stop
+    end function
+
+    integer(c_int) function ptc_particles_download(p, state, counter, status) &
+                                                   bind(C, name="ptc_particles_download")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: p, state, counter, status
+    end function
+
+    integer(c_int) function ptc_particles_destroy(p) bind(C, name="ptc_particles_destroy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: p
+    end function
+
+    integer(c_int) function ptc_set_option(key, value) bind(C, name="ptc_set_option")
+      import :: c_int, c_char
+      character(kind=c_char), intent(in) :: key(*), value(*)
+    end function
+
+    type(c_ptr) function ptc_last_error_c() bind(C, name="ptc_last_error")
+      import :: c_ptr
+    end function
+
+  end interface
+
+contains
+
+  ! ptc_last_error() as a Fortran string
+  function ptc_last_error() result(msg)
+    character(len=:), allocatable :: msg
+    character(kind=c_char), pointer :: p(:)
+    type(c_ptr) :: cp
+    integer :: n
+    cp = ptc_last_error_c()
+    call c_f_pointer(cp, p, [4096])
+    n = 0
+    do while (n < 4096)
+      if (p(n + 1) == c_null_char) exit
+      n = n + 1
+    end do
+    allocate(character(len=n) :: msg)
+    msg = transfer(p(1:n), msg)
+  end function
+
+end module ptc_binding

@@ -328,3 +328,19 @@ def test_mixed_precision_energy_without_electric_field():
     rel = np.abs(st[6, 1:, :] - st[6, 0, :]) / st[6, 0, :]
     say(f"[mixed energy] max relative gamma drift {rel.max():.3e}")
     assert (status == 0).all() and rel.max() < 1e-6
+
+
+def test_fortran_caller_integrates():
+    """The Fortran demo (ptc_binding) runs the device integrator: ten
+    particles gyrating in uniform B, one gyro-period."""
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                       "particles_amd", "fortran", "ptc_demo")
+    if not os.path.exists(exe):
+        pytest.skip("Fortran demo not built")
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    say(r.stdout[-400:])
+    assert r.returncode == 0
+    rows = [l.split() for l in r.stdout.splitlines() if l.strip().startswith("particle")]
+    assert len(rows) == 10 and all(int(row[4]) > 0 for row in rows)
