@@ -118,8 +118,82 @@ int ptc_particles_destroy(ptc_particles *p);
 #define PTC_PROBE_DTI5   2
 #define PTC_PROBE_DTI8   3
 #define PTC_PROBE_POW    4
+/*   PTC_PROBE_FEATURES : in (vec(6), h, err, pi_ratio, rejected)(10,n)
+ *                        -> out (16,n) = the step-size network's features
+ *                        at vec after an accepted step h with error ratio
+ *                        err (no counterpart in the reference; aos layout) */
+#define PTC_PROBE_FEATURES 5
 int ptc_probe(const ptc_fields *f, int op, int64_t n, const double *params, int nparams,
               const double *in, double *out);
+
+/* ---- learned step-size proposal (no counterpart in the reference) ----
+ *
+ * The reference sizes steps with a PI controller (sources/integrations.F90:
+ * 177,186 for RK5, :466-490 for DOP853).  A small network can propose the
+ * step after each accepted step instead, from features of the local field
+ * and orbit (PTC_PROBE_FEATURES lists them); the embedded error estimate
+ * still accepts or rejects every step and the PI rule still sizes the step
+ * after a rejection.  Results then differ from the reference's (other step
+ * sequence, same tolerance).
+ *
+ * Network: 16 -> 64 -> 64 -> 1, GELU, BF16 weights and activations with FP32
+ * accumulation on the matrix cores; output = log(h_next / h).  Parameters are
+ * PTC_MLP_PARAMS floats, packed as w1[16][64], b1[64], w2[64][64], b2[64],
+ * w3[64], b3, mu[16], sig[16] (weights [in][out]; inputs are normalised as
+ * (f - mu) / sig; mu and sig are not trained).
+ *
+ * Training records (PTC_MLP_RECORD floats each): 16 features taken after an
+ * accepted step of size h, log(h_next / h) of the next accepted step, log of
+ * that step's error ratio, the rejections before it, and the order q of the
+ * error estimate (err ~ h^q: 5 for RK5, 1 / (0.125 - 0.2 beta) for DOP853).
+ * The training target is log(h_next / h) + alpha log(1 / err) / q, clamped
+ * to [-4, 4]: the accepted step itself (alpha = 0) or extrapolated to an
+ * error ratio of one (alpha = 1). */
+#define PTC_MLP_PARAMS 5345
+#define PTC_MLP_TRAINABLE 5313
+#define PTC_MLP_RECORD 20
+typedef struct ptc_mlp ptc_mlp;
+/* params = NULL: random initialisation from seed (mu = 0, sig = 1). */
+int ptc_mlp_create(const float *params, uint64_t seed, ptc_mlp **out);
+int ptc_mlp_destroy(ptc_mlp *net);
+int ptc_mlp_get_params(const ptc_mlp *net, float *params);
+int ptc_mlp_set_params(ptc_mlp *net, const float *params);
+/* out[i] = network(features[i][0:16]), host arrays */
+int ptc_mlp_forward(const ptc_mlp *net, int64_t n, const float *features, float *out);
+/* gradient (PTC_MLP_TRAINABLE floats, packed order) and value of the loss
+ * mean((out - target)^2) over n host records */
+int ptc_mlp_gradient(const ptc_mlp *net, int64_t n, const float *records, float alpha,
+                     float *grad, double *loss);
+/* sets mu, sig to the mean and standard deviation (floored at 1e-3) of the
+ * records' features; records in host (on_device = 0) or device memory */
+int ptc_mlp_normalize(ptc_mlp *net, int64_t n, const float *records, int on_device);
+/* `steps` Adam steps (lr, beta 0.9 / 0.999) on minibatches of `batch`
+ * records drawn with replacement from n (stream `seed`); losses (steps
+ * floats, may be NULL) receives each minibatch's loss before its update */
+int ptc_mlp_train(ptc_mlp *net, int64_t n, const float *records, int on_device, int steps,
+                  int64_t batch, float lr, float alpha, uint64_t seed, float *losses);
+/* c[64][64] = a[64][64] bt[64][64]^T through the device's BF16 wave product
+ * (inputs rounded to BF16, FP32 accumulation), host arrays */
+int ptc_mlp_probe_gemm(const float *a, const float *bt, float *c);
+
+/* Record mode: the batch keeps the PI controller and, for a fraction `rate`
+ * of its accepted steps (hash-sampled per particle and step), writes a
+ * training record into a device buffer of `capacity` records (later records
+ * are dropped).  capacity = 0 leaves record mode.  Needs the aos layout with
+ * cell_cache 1. */
+int ptc_particles_record(ptc_particles *p, double rate, int64_t capacity);
+/* Number of records written (at most capacity); host_out (count x
+ * PTC_MLP_RECORD) and device_records (the device buffer, valid until the
+ * batch is destroyed or leaves record mode) may be NULL. */
+int ptc_particles_records(const ptc_particles *p, int64_t *count, float *host_out,
+                          const float **device_records);
+/* Learned mode: after each accepted step h the next step is h exp(network +
+ * shift), kept within [facmin h_PI, facmax h] (h_PI: the PI controller's
+ * proposal, which stays the floor) and capped at dtmax; net = NULL returns to
+ * the PI controller.  The network must outlive the batch's use of it and
+ * must have been trained or given parameters (PTC_ESTATE otherwise).  Needs
+ * the aos layout with cell_cache 1. */
+int ptc_particles_use_mlp(ptc_particles *p, const ptc_mlp *net, double shift);
 
 /* Tuning and behaviour switches (process-wide), e.g.
  *   "block"       threads per block (64..1024, multiple of 64), default 256

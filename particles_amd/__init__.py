@@ -27,8 +27,8 @@ import os
 
 import numpy as np
 
-__all__ = ["Fields", "Particles", "integrate", "set_option", "get_option", "PtcError",
-           "METHODS", "MODES", "STATUS", "library_path"]
+__all__ = ["Fields", "Particles", "Mlp", "integrate", "set_option", "get_option", "PtcError",
+           "METHODS", "MODES", "STATUS", "FEATURES", "library_path"]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 library_path = os.path.join(_HERE, "libparticles_amd.so")
@@ -38,6 +38,11 @@ METHODS = {"rk5": 0, "dop853": 1, "dp853": 1, "rk8": 1}
 MODES = {"vb": 0, "eb": 1}
 PRECS = {"fp64": 0, "fp32": 1}
 STATUS = {-1: "running", 0: "done", 1: "max_steps", 2: "nonfinite", 3: "dt_small"}
+# inputs of the step-size network (PTC_PROBE_FEATURES), in order
+FEATURES = ("log|B|", "log(|E|/|B|)", "log(Bx-/B)", "log(Bx+/B)", "log(By-/B)", "log(By+/B)",
+            "log(Bz-/B)", "log(Bz+/B)", "log(gamma)", "cos(pitch)", "log(h*Omega)", "log(err)",
+            "log(t_face/h)", "log(h_PI/h)", "rejected", "log(r_L/dx)")
+MLP_PARAMS, MLP_TRAINABLE, MLP_RECORD = 5345, 5313, 20
 
 
 class PtcError(RuntimeError):
@@ -67,6 +72,18 @@ def _load():
         "ptc_probe": ([vp, i32, i64, vp, i32, vp, vp], i32),
         "ptc_set_option": ([C.c_char_p, C.c_char_p], i32),
         "ptc_get_option": ([C.c_char_p, C.c_char_p, i32], i32),
+        "ptc_particles_record": ([vp, dbl, i64], i32),
+        "ptc_particles_records": ([vp, C.POINTER(i64), vp, C.POINTER(vp)], i32),
+        "ptc_particles_use_mlp": ([vp, vp, dbl], i32),
+        "ptc_mlp_create": ([vp, C.c_uint64, C.POINTER(vp)], i32),
+        "ptc_mlp_destroy": ([vp], i32),
+        "ptc_mlp_get_params": ([vp, vp], i32),
+        "ptc_mlp_set_params": ([vp, vp], i32),
+        "ptc_mlp_forward": ([vp, i64, vp, vp], i32),
+        "ptc_mlp_gradient": ([vp, i64, vp, C.c_float, vp, C.POINTER(dbl)], i32),
+        "ptc_mlp_normalize": ([vp, i64, vp, i32], i32),
+        "ptc_mlp_train": ([vp, i64, vp, i32, i32, i64, C.c_float, C.c_float, C.c_uint64, vp], i32),
+        "ptc_mlp_probe_gemm": ([vp, vp, vp], i32),
         "ptc_last_error": ([], C.c_char_p),
         "ptc_version": ([], C.c_char_p),
     }
@@ -144,14 +161,16 @@ class Fields:
 
 
 _PROBES = {"gather": (0, 3, 6), "rhs": (1, 6, 6), "dti5": (2, 6, 1), "dti8": (3, 6, 1),
-           "pow": (4, 2, 1)}
+           "pow": (4, 2, 1), "features": (5, 10, 16)}
 
 
 def probe(fields: Fields, op: str, inputs, params) -> np.ndarray:
     """Evaluates one device building block per input row: "gather" (pos ->
     U or E, B; interpolate_lin), "rhs" (vec -> d vec/dt; acceleration),
     "dti5"/"dti8" (vec -> initial step; dti_guess), "pow" ((x, y) -> x**y as
-    the step controller computes it).  inputs is (n, k); returns (n, 6) or (n,)."""
+    the step controller computes it), "features" ((vec, h, err, pi_ratio,
+    rejected) -> the step-size network's 16 inputs, FEATURES).  inputs is
+    (n, k); returns (n, 6), (n, 16) or (n,)."""
     code, nin, nout = _PROBES[op]
     x = np.ascontiguousarray(inputs, dtype=np.float64).reshape(-1, nin)
     params = np.ascontiguousarray(params, dtype=np.float64)
@@ -217,6 +236,34 @@ class Particles:
         return dict(all=int(t[0]), accepted=int(t[1]), rejected=int(t[2]), rhs=int(t[3]),
                     running=int(t[4]))
 
+    def record(self, rate: float, capacity: int) -> None:
+        """Record mode: keep the PI controller and write a training record for
+        a fraction `rate` of accepted steps into a device buffer of
+        `capacity` records; capacity 0 leaves record mode."""
+        _check(_lib.ptc_particles_record(self._h, float(rate), int(capacity)))
+
+    def records(self, device: bool = False):
+        """The records written so far: a (count, 20) float32 array, or with
+        device=True a DeviceRecords view of the device buffer (no copy)."""
+        n = C.c_int64()
+        if device:
+            d = C.c_void_p()
+            _check(_lib.ptc_particles_records(self._h, C.byref(n), None, C.byref(d)))
+            return DeviceRecords(d.value, n.value, self)
+        _check(_lib.ptc_particles_records(self._h, C.byref(n), None, None))
+        out = np.zeros((n.value, MLP_RECORD), dtype=np.float32)
+        if n.value:
+            _check(_lib.ptc_particles_records(self._h, C.byref(n), _ptr(out), None))
+        return out
+
+    def use_mlp(self, net, shift: float = 0.0) -> None:
+        """Learned mode: the network proposes the step after each accepted
+        step (log factor + shift, clamped to [facmin, facmax]); None returns
+        to the PI controller."""
+        self._net = net  # keeps the network alive while in use
+        _check(_lib.ptc_particles_use_mlp(self._h, net.handle if net is not None else None,
+                                          float(shift)))
+
     def download(self):
         state = np.zeros((8, self.m, self.np), dtype=np.float64, order="F")
         counter = np.zeros((3, self.np), dtype=np.int64, order="F")
@@ -240,3 +287,113 @@ class Particles:
 
     def __exit__(self, *a):
         self.close()
+
+
+class DeviceRecords:
+    """Training records left in device memory by a Particles batch in
+    record mode (valid while the batch exists and stays in record mode)."""
+
+    def __init__(self, ptr: int, count: int, owner):
+        self.ptr, self.count, self.owner = ptr, count, owner
+
+    def __len__(self):
+        return self.count
+
+
+def _records_arg(records):
+    if isinstance(records, DeviceRecords):
+        return C.c_void_p(records.ptr), records.count, 1, None
+    r = np.ascontiguousarray(records, dtype=np.float32)
+    if r.ndim != 2 or r.shape[1] != MLP_RECORD:
+        raise ValueError(f"records must be (n, {MLP_RECORD}), got {r.shape}")
+    return _ptr(r), r.shape[0], 0, r
+
+
+class Mlp:
+    """The step-size network (16 -> 64 -> 64 -> 1, GELU, BF16 MFMA with FP32
+    accumulation) with its device training state.  params is the packed
+    PTC_MLP_PARAMS float vector (None: random initialisation from seed)."""
+
+    def __init__(self, params=None, seed: int = 0):
+        h = C.c_void_p()
+        if params is None:
+            _check(_lib.ptc_mlp_create(None, int(seed), C.byref(h)))
+        else:
+            p = np.ascontiguousarray(params, dtype=np.float32)
+            if p.size != MLP_PARAMS:
+                raise ValueError(f"need {MLP_PARAMS} parameters, got {p.size}")
+            _check(_lib.ptc_mlp_create(_ptr(p), 0, C.byref(h)))
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def params(self) -> np.ndarray:
+        p = np.zeros(MLP_PARAMS, dtype=np.float32)
+        _check(_lib.ptc_mlp_get_params(self._h, _ptr(p)))
+        return p
+
+    def set_params(self, params) -> None:
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        if p.size != MLP_PARAMS:
+            raise ValueError(f"need {MLP_PARAMS} parameters, got {p.size}")
+        _check(_lib.ptc_mlp_set_params(self._h, _ptr(p)))
+
+    def forward(self, features) -> np.ndarray:
+        f = np.ascontiguousarray(features, dtype=np.float32).reshape(-1, 16)
+        out = np.zeros(f.shape[0], dtype=np.float32)
+        _check(_lib.ptc_mlp_forward(self._h, f.shape[0], _ptr(f), _ptr(out)))
+        return out
+
+    def gradient(self, records, alpha: float = 0.0):
+        """(loss, grad) of mean((out - target)^2) over host records."""
+        ptr, n, on_dev, keep = _records_arg(records)
+        if on_dev:
+            raise ValueError("gradient takes host records")
+        g = np.zeros(MLP_TRAINABLE, dtype=np.float32)
+        loss = C.c_double()
+        _check(_lib.ptc_mlp_gradient(self._h, n, ptr, float(alpha), _ptr(g), C.byref(loss)))
+        return loss.value, g
+
+    def normalize(self, records) -> None:
+        """Input normalisation from the records' feature statistics."""
+        ptr, n, on_dev, keep = _records_arg(records)
+        _check(_lib.ptc_mlp_normalize(self._h, n, ptr, on_dev))
+
+    def train(self, records, steps: int, batch: int = 16384, lr: float = 1e-3,
+              alpha: float = 0.0, seed: int = 1) -> np.ndarray:
+        """`steps` Adam steps on minibatches drawn from the records (host
+        array or DeviceRecords); returns each minibatch's loss."""
+        ptr, n, on_dev, keep = _records_arg(records)
+        losses = np.zeros(max(int(steps), 1), dtype=np.float32)
+        _check(_lib.ptc_mlp_train(self._h, n, ptr, on_dev, int(steps), int(batch), float(lr),
+                                  float(alpha), int(seed), _ptr(losses)))
+        return losses[:int(steps)]
+
+    def close(self):
+        if self._h:
+            _check(_lib.ptc_mlp_destroy(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+def probe_gemm(a, bt) -> np.ndarray:
+    """c = a bt^T (64 x 64 each) through the device's BF16 wave product."""
+    a = np.ascontiguousarray(a, dtype=np.float32).reshape(64, 64)
+    bt = np.ascontiguousarray(bt, dtype=np.float32).reshape(64, 64)
+    c = np.zeros((64, 64), dtype=np.float32)
+    _check(_lib.ptc_mlp_probe_gemm(_ptr(a), _ptr(bt), _ptr(c)))
+    return c

@@ -3,6 +3,7 @@
 // integrator.hip.
 #include "internal.hpp"
 #include "libm_pow.hpp"
+#include "mlp_device.hpp"
 
 #include "particles_amd.h"
 
@@ -32,6 +33,20 @@ struct ptc_particles {
     hipStream_t stream;
     hipEvent_t ev0, ev1;
     double last_ms;
+    int ctrl;             // CTRL_*
+    void *rec_block;      // record mode: pending features, records, counter
+};
+
+struct ptc_mlp {
+    float *params;        // PTC_MLP_PARAMS FP32 master copy
+    float *m1, *m2;       // Adam moments (PTC_MLP_TRAINABLE each)
+    float *partials;      // kMlpMaxGradBlocks x (PTC_MLP_TRAINABLE + 1)
+    float *grad;          // PTC_MLP_TRAINABLE + 1
+    double *stats;        // feature statistics partials
+    MlpPacked *packed;    // BF16 layouts read by the kernels
+    int64_t t;            // Adam steps taken
+    bool trained;         // parameters given or trained (not the random start)
+    hipStream_t stream;
 };
 
 namespace {
@@ -241,9 +256,19 @@ static int build_step_params(const ptc_fields *f, int m, const double *params, i
     sp.safe = params[9];
     sp.beta = params[10];
     sp.expo = 2.0e-01 * sp.beta - 1.25e-01;  // integrations.F90:348
+    // a power of two scales exactly: division by it equals multiplication by
+    // its (exact) reciprocal
+    auto pow2 = [](double x) {
+        int e = 0;
+        return x != 0.0 && std::fabs(std::frexp(x, &e)) == 0.5;
+    };
+    sp.pow2 = pow2(params[1]);
+    sp.rvun = 1.0 / params[1];
     for (int i = 0; i < 3; i++) {
         sp.xl[i] = params[11 + i];
         sp.xs[i] = params[14 + i];
+        sp.rxs[i] = 1.0 / sp.xs[i];
+        sp.pow2 = sp.pow2 && pow2(sp.xs[i]);
         sp.dm[i] = f->d.dm[i];
         sp.dmd[i] = (double)f->d.dm[i];
         sp.inv_dm[i] = 1.0 / (double)f->d.dm[i];
@@ -357,7 +382,10 @@ int ptc_particles_advance(ptc_particles *p, double t_stop)
     const Options o = options();
     p->sp.max_steps = o.max_steps;
     LaunchConfig lc{o.block, o.grid_blocks,
-                    p->fields->d.layout == LAYOUT_AOS ? o.cell_cache : 0};
+                    p->fields->d.layout == LAYOUT_AOS ? o.cell_cache : 0, p->ctrl};
+    if (p->ctrl != CTRL_PI && lc.cell_cache != 1)
+        return fail(PTC_ESTATE, "record and learned step-size modes need the aos layout with "
+                                "cell_cache 1");
     HIP_TRY(launch_zero_i32(p->d.qcount, 1, p->stream));
     HIP_TRY(hipEventRecord(p->ev0, p->stream));
     HIP_TRY(launch_integrate(p->method, p->fields->d, p->sp, p->d, t_stop, lc, p->stream));
@@ -420,6 +448,7 @@ int ptc_particles_destroy(ptc_particles *p)
     (void)hipStreamDestroy(p->stream);
     hipError_t e1 = hipFree(p->d_time);
     hipError_t e2 = hipFree(p->block);
+    if (p->rec_block) (void)hipFree(p->rec_block);
     delete p;
     if (e1 != hipSuccess) return hip_fail(e1, "ptc_particles_destroy");
     if (e2 != hipSuccess) return hip_fail(e2, "ptc_particles_destroy");
@@ -431,15 +460,23 @@ int ptc_probe(const ptc_fields *f, int op, int64_t n, const double *params, int 
               const double *in, double *out)
 {
     if (!f) return fail(PTC_EARG, "ptc_probe: fields handle is null");
-    if (op < PTC_PROBE_GATHER || op > PTC_PROBE_POW) return fail(PTC_EARG, "ptc_probe: bad op %d", op);
+    if (op < PTC_PROBE_GATHER || op > PTC_PROBE_FEATURES)
+        return fail(PTC_EARG, "ptc_probe: bad op %d", op);
     if (n < 0) return fail(PTC_EARG, "ptc_probe: negative count");
     if (n == 0) { ok(); return PTC_OK; }
     if (!in || !out) return fail(PTC_EARG, "ptc_probe: null in/out");
     StepParams sp;
     int rc = build_step_params(f, 1, params, nparams, sp);
     if (rc) return rc;
-    const int nin = op == PTC_PROBE_GATHER ? 3 : op == PTC_PROBE_POW ? 2 : 6;
-    const int nout = (op == PTC_PROBE_GATHER || op == PTC_PROBE_RHS) ? 6 : 1;
+    if (op == PTC_PROBE_FEATURES && f->d.layout != LAYOUT_AOS)
+        return fail(PTC_EARG, "ptc_probe: features need the aos layout");
+    const int nin = op == PTC_PROBE_GATHER    ? 3
+                    : op == PTC_PROBE_POW      ? 2
+                    : op == PTC_PROBE_FEATURES ? 10
+                                               : 6;
+    const int nout = (op == PTC_PROBE_GATHER || op == PTC_PROBE_RHS) ? 6
+                     : op == PTC_PROBE_FEATURES                       ? 16
+                                                                      : 1;
     double *d = nullptr;
     HIP_TRY(hipMalloc(&d, sizeof(double) * (nin + nout) * n));
     hipError_t e = hipMemcpy(d, in, sizeof(double) * nin * n, hipMemcpyHostToDevice);
@@ -449,6 +486,359 @@ int ptc_probe(const ptc_fields *f, int op, int64_t n, const double *params, int 
         e = hipMemcpy(out, d + nin * n, sizeof(double) * nout * n, hipMemcpyDeviceToHost);
     (void)hipFree(d);
     if (e != hipSuccess) return hip_fail(e, "ptc_probe");
+    ok();
+    return PTC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// step-size controller modes
+// ---------------------------------------------------------------------------
+int ptc_particles_record(ptc_particles *p, double rate, int64_t capacity)
+{
+    if (!p) return fail(PTC_EARG, "ptc_particles_record: null handle");
+    if (!(rate > 0.0 && rate <= 1.0) && capacity > 0)
+        return fail(PTC_EARG, "ptc_particles_record: rate must be in (0, 1] (got %g)", rate);
+    if (capacity < 0 || capacity > ((int64_t)1 << 32))
+        return fail(PTC_EARG, "ptc_particles_record: bad capacity %lld", (long long)capacity);
+    HIP_TRY(hipStreamSynchronize(p->stream));
+    if (p->rec_block) {
+        HIP_TRY(hipFree(p->rec_block));
+        p->rec_block = nullptr;
+    }
+    p->d.pend = nullptr;
+    p->d.pend_h = nullptr;
+    p->d.pend_rej = nullptr;
+    p->d.rec = nullptr;
+    p->d.rec_count = nullptr;
+    p->d.rec_cap = 0;
+    if (p->ctrl == CTRL_RECORD) p->ctrl = CTRL_PI;
+    if (capacity == 0) { ok(); return PTC_OK; }
+    if (p->fields->d.layout != LAYOUT_AOS)
+        return fail(PTC_ESTATE, "ptc_particles_record: needs the aos field layout");
+    const int64_t n = p->d.np > 0 ? p->d.np : 1;
+    const size_t bytes = sizeof(double) * n + sizeof(int64_t) * n + 256 +
+                         sizeof(float) * kMlpIn * n + sizeof(float) * kRecord * (size_t)capacity;
+    HIP_TRY(hipMalloc(&p->rec_block, bytes));
+    char *c = (char *)p->rec_block;
+    p->d.pend_h = (double *)c;
+    c += sizeof(double) * n;
+    p->d.pend_rej = (int64_t *)c;
+    c += sizeof(int64_t) * n;
+    p->d.rec_count = (unsigned long long *)c;
+    c += 256;
+    p->d.pend = (float *)c;
+    c += sizeof(float) * kMlpIn * n;
+    p->d.rec = (float *)c;
+    p->d.rec_cap = capacity;
+    p->d.rec_thresh = (uint64_t)(rate * 4294967296.0);
+    p->ctrl = CTRL_RECORD;
+    HIP_TRY(launch_zero_i64((int64_t *)p->d.rec_count, 1, p->stream));
+    if (p->d.np > 0) HIP_TRY(launch_clear_flag(p->d.flags, p->d.np, FLAG_PENDING, p->stream));
+    HIP_TRY(hipStreamSynchronize(p->stream));
+    ok();
+    return PTC_OK;
+}
+
+int ptc_particles_records(const ptc_particles *p, int64_t *count, float *host_out,
+                          const float **device_records)
+{
+    if (!p || !count) return fail(PTC_EARG, "ptc_particles_records: null argument");
+    *count = 0;
+    if (device_records) *device_records = p->d.rec;
+    if (!p->rec_block) { ok(); return PTC_OK; }
+    unsigned long long claimed = 0;
+    HIP_TRY(hipStreamSynchronize(p->stream));
+    HIP_TRY(hipMemcpy(&claimed, p->d.rec_count, sizeof claimed, hipMemcpyDeviceToHost));
+    const int64_t k = claimed < (unsigned long long)p->d.rec_cap ? (int64_t)claimed : p->d.rec_cap;
+    *count = k;
+    if (host_out && k > 0)
+        HIP_TRY(hipMemcpy(host_out, p->d.rec, sizeof(float) * kRecord * (size_t)k,
+                          hipMemcpyDeviceToHost));
+    ok();
+    return PTC_OK;
+}
+
+int ptc_particles_use_mlp(ptc_particles *p, const ptc_mlp *net, double shift)
+{
+    if (!p) return fail(PTC_EARG, "ptc_particles_use_mlp: null handle");
+    if (!std::isfinite(shift)) return fail(PTC_EARG, "ptc_particles_use_mlp: shift not finite");
+    if (!net) {
+        if (p->ctrl == CTRL_LEARNED) p->ctrl = CTRL_PI;
+        p->d.net = nullptr;
+        ok();
+        return PTC_OK;
+    }
+    if (p->fields->d.layout != LAYOUT_AOS)
+        return fail(PTC_ESTATE, "ptc_particles_use_mlp: needs the aos field layout");
+    if (!net->trained)
+        return fail(PTC_ESTATE, "ptc_particles_use_mlp: the network still has its random "
+                                "initialisation (train it or set its parameters)");
+    p->d.net = net->packed;
+    p->d.ctrl_shift = (float)shift;
+    p->ctrl = CTRL_LEARNED;
+    ok();
+    return PTC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// step-size network
+// ---------------------------------------------------------------------------
+namespace {
+
+uint64_t splitmix(uint64_t &s)
+{
+    uint64_t z = (s += 0x9e3779b97f4a7c15ull);
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
+// He-uniform hidden layers, small output layer, zero biases, identity
+// normalisation
+void init_params(uint64_t seed, std::vector<float> &w)
+{
+    w.assign(kMlpParams, 0.0f);
+    uint64_t s = seed;
+    auto uni = [&](float a) {
+        const double u = (double)(splitmix(s) >> 11) * (1.0 / 9007199254740992.0);
+        return (float)((2.0 * u - 1.0) * a);
+    };
+    for (int i = 0; i < kMlpIn * kMlpHidden; i++) w[kMlpW1 + i] = uni(std::sqrt(6.0f / kMlpIn));
+    for (int i = 0; i < kMlpHidden * kMlpHidden; i++)
+        w[kMlpW2 + i] = uni(std::sqrt(6.0f / kMlpHidden));
+    for (int i = 0; i < kMlpHidden; i++) w[kMlpW3 + i] = uni(0.1f * std::sqrt(6.0f / kMlpHidden));
+    for (int i = 0; i < kMlpIn; i++) w[kMlpSig + i] = 1.0f;
+}
+
+int check_params(const float *params)
+{
+    for (int i = 0; i < kMlpParams; i++)
+        if (!std::isfinite(params[i])) return fail(PTC_EARG, "network parameter %d not finite", i);
+    for (int i = 0; i < kMlpIn; i++)
+        if (!(params[kMlpSig + i] > 0.0f))
+            return fail(PTC_EARG, "network input scale sig[%d] must be > 0", i);
+    return PTC_OK;
+}
+
+// device copy of host records (or the device pointer itself)
+struct RecordBuf {
+    const float *d = nullptr;
+    float *owned = nullptr;
+    ~RecordBuf() { if (owned) (void)hipFree(owned); }
+    hipError_t set(int64_t n, const float *records, int on_device)
+    {
+        if (on_device) { d = records; return hipSuccess; }
+        hipError_t e = hipMalloc(&owned, sizeof(float) * kRecord * (size_t)n);
+        if (e != hipSuccess) return e;
+        d = owned;
+        return hipMemcpy(owned, records, sizeof(float) * kRecord * (size_t)n, hipMemcpyHostToDevice);
+    }
+};
+
+}  // namespace
+
+int ptc_mlp_create(const float *params, uint64_t seed, ptc_mlp **out)
+{
+    if (!out) return fail(PTC_EARG, "ptc_mlp_create: out is null");
+    *out = nullptr;
+    std::vector<float> w;
+    if (params) {
+        int rc = check_params(params);
+        if (rc) return rc;
+        w.assign(params, params + kMlpParams);
+    } else {
+        init_params(seed, w);
+    }
+    ptc_mlp *net = new ptc_mlp();
+    net->trained = params != nullptr;
+    const size_t fl = (size_t)kMlpParams + 2 * (size_t)kMlpTrainable +
+                      (size_t)kMlpMaxGradBlocks * (kMlpTrainable + 1) + (kMlpTrainable + 1) + 64;
+    void *blk = nullptr;
+    hipError_t e = hipMalloc(&blk, sizeof(float) * fl);
+    if (e != hipSuccess) { delete net; return hip_fail(e, "ptc_mlp_create"); }
+    float *c = (float *)blk;
+    net->params = c;
+    net->m1 = c + kMlpParams;
+    net->m2 = net->m1 + kMlpTrainable;
+    net->partials = net->m2 + kMlpTrainable;
+    net->grad = net->partials + (size_t)kMlpMaxGradBlocks * (kMlpTrainable + 1);
+    e = hipMalloc(&net->packed, sizeof(MlpPacked));
+    if (e == hipSuccess) e = hipMalloc(&net->stats, sizeof(double) * 2 * kMlpIn * 1024);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&net->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipMemcpy(net->params, w.data(), sizeof(float) * kMlpParams, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemset(net->m1, 0, sizeof(float) * 2 * kMlpTrainable);
+    if (e == hipSuccess) e = hipMemset(net->packed, 0, sizeof(MlpPacked));
+    if (e == hipSuccess) e = mlp_pack(net->params, net->packed, net->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(net->stream);
+    if (e != hipSuccess) {
+        (void)hipFree(blk);
+        if (net->packed) (void)hipFree(net->packed);
+        if (net->stats) (void)hipFree(net->stats);
+        delete net;
+        return hip_fail(e, "ptc_mlp_create");
+    }
+    *out = net;
+    ok();
+    return PTC_OK;
+}
+
+int ptc_mlp_destroy(ptc_mlp *net)
+{
+    if (!net) return PTC_OK;
+    (void)hipStreamSynchronize(net->stream);
+    (void)hipStreamDestroy(net->stream);
+    hipError_t e = hipFree(net->params);
+    (void)hipFree(net->packed);
+    (void)hipFree(net->stats);
+    delete net;
+    if (e != hipSuccess) return hip_fail(e, "ptc_mlp_destroy");
+    ok();
+    return PTC_OK;
+}
+
+int ptc_mlp_get_params(const ptc_mlp *net, float *params)
+{
+    if (!net || !params) return fail(PTC_EARG, "ptc_mlp_get_params: null argument");
+    HIP_TRY(hipStreamSynchronize(net->stream));
+    HIP_TRY(hipMemcpy(params, net->params, sizeof(float) * kMlpParams, hipMemcpyDeviceToHost));
+    ok();
+    return PTC_OK;
+}
+
+int ptc_mlp_set_params(ptc_mlp *net, const float *params)
+{
+    if (!net || !params) return fail(PTC_EARG, "ptc_mlp_set_params: null argument");
+    int rc = check_params(params);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(net->stream));
+    HIP_TRY(hipMemcpy(net->params, params, sizeof(float) * kMlpParams, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemset(net->m1, 0, sizeof(float) * 2 * kMlpTrainable));
+    net->t = 0;
+    net->trained = true;
+    HIP_TRY(mlp_pack(net->params, net->packed, net->stream));
+    HIP_TRY(hipStreamSynchronize(net->stream));
+    ok();
+    return PTC_OK;
+}
+
+int ptc_mlp_forward(const ptc_mlp *net, int64_t n, const float *features, float *out)
+{
+    if (!net) return fail(PTC_EARG, "ptc_mlp_forward: null handle");
+    if (n < 0) return fail(PTC_EARG, "ptc_mlp_forward: negative count");
+    if (n == 0) { ok(); return PTC_OK; }
+    if (!features || !out) return fail(PTC_EARG, "ptc_mlp_forward: null array");
+    float *d = nullptr;
+    HIP_TRY(hipMalloc(&d, sizeof(float) * (kMlpIn + 1) * (size_t)n));
+    hipError_t e = hipMemcpy(d, features, sizeof(float) * kMlpIn * n, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = mlp_forward(net->packed, n, d, kMlpIn, d + kMlpIn * n, net->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(net->stream);
+    if (e == hipSuccess) e = hipMemcpy(out, d + kMlpIn * n, sizeof(float) * n, hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    if (e != hipSuccess) return hip_fail(e, "ptc_mlp_forward");
+    ok();
+    return PTC_OK;
+}
+
+int ptc_mlp_gradient(const ptc_mlp *net, int64_t n, const float *records, float alpha,
+                     float *grad, double *loss)
+{
+    if (!net || !records || !grad) return fail(PTC_EARG, "ptc_mlp_gradient: null argument");
+    if (n <= 0) return fail(PTC_EARG, "ptc_mlp_gradient: need at least one record");
+    RecordBuf rb;
+    HIP_TRY(rb.set(n, records, 0));
+    HIP_TRY(mlp_gradient(net->packed, rb.d, n, n, 0, 0, alpha, -4.0f, 4.0f, net->partials,
+                         net->stream));
+    HIP_TRY(mlp_reduce(net->partials, mlp_grad_blocks(n), net->grad, 0, 0.0f, 1.0f, 1.0f, nullptr,
+                       nullptr, nullptr, nullptr, nullptr, net->stream));
+    HIP_TRY(hipStreamSynchronize(net->stream));
+    std::vector<float> g(kMlpTrainable + 1);
+    HIP_TRY(hipMemcpy(g.data(), net->grad, sizeof(float) * (kMlpTrainable + 1), hipMemcpyDeviceToHost));
+    for (int i = 0; i < kMlpTrainable; i++) grad[i] = g[i];
+    if (loss) *loss = (double)g[kMlpTrainable] / (double)n;
+    ok();
+    return PTC_OK;
+}
+
+int ptc_mlp_normalize(ptc_mlp *net, int64_t n, const float *records, int on_device)
+{
+    if (!net || !records) return fail(PTC_EARG, "ptc_mlp_normalize: null argument");
+    if (n <= 1) return fail(PTC_EARG, "ptc_mlp_normalize: need at least two records");
+    RecordBuf rb;
+    HIP_TRY(rb.set(n, records, on_device));
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > 1024) blocks = 1024;
+    HIP_TRY(mlp_stats(rb.d, n, net->stats, (int)blocks, net->stream));
+    HIP_TRY(hipStreamSynchronize(net->stream));
+    std::vector<double> h(2 * kMlpIn * (size_t)blocks);
+    HIP_TRY(hipMemcpy(h.data(), net->stats, sizeof(double) * h.size(), hipMemcpyDeviceToHost));
+    std::vector<float> w(kMlpParams);
+    HIP_TRY(hipMemcpy(w.data(), net->params, sizeof(float) * kMlpParams, hipMemcpyDeviceToHost));
+    for (int i = 0; i < kMlpIn; i++) {
+        double s = 0.0, q = 0.0;
+        for (int64_t b = 0; b < blocks; b++) {
+            s += h[b * 2 * kMlpIn + i];
+            q += h[b * 2 * kMlpIn + kMlpIn + i];
+        }
+        const double mu = s / (double)n;
+        const double var = q / (double)n - mu * mu;
+        const double sd = var > 1e-6 ? std::sqrt(var) : 1e-3;
+        w[kMlpMu + i] = (float)mu;
+        w[kMlpSig + i] = (float)sd;
+    }
+    HIP_TRY(hipMemcpy(net->params, w.data(), sizeof(float) * kMlpParams, hipMemcpyHostToDevice));
+    HIP_TRY(mlp_pack(net->params, net->packed, net->stream));
+    HIP_TRY(hipStreamSynchronize(net->stream));
+    ok();
+    return PTC_OK;
+}
+
+int ptc_mlp_train(ptc_mlp *net, int64_t n, const float *records, int on_device, int steps,
+                  int64_t batch, float lr, float alpha, uint64_t seed, float *losses)
+{
+    if (!net || !records) return fail(PTC_EARG, "ptc_mlp_train: null argument");
+    if (n <= 0 || steps < 0 || batch <= 0 || !(lr > 0.0f) || !std::isfinite(alpha))
+        return fail(PTC_EARG, "ptc_mlp_train: bad n/steps/batch/lr/alpha");
+    if (seed == 0) seed = 1;
+    RecordBuf rb;
+    HIP_TRY(rb.set(n, records, on_device));
+    float *d_loss = nullptr;
+    if (steps > 0) HIP_TRY(hipMalloc(&d_loss, sizeof(float) * steps));
+    const int blocks = mlp_grad_blocks(batch);
+    hipError_t e = hipSuccess;
+    for (int s = 0; s < steps && e == hipSuccess; s++) {
+        net->t++;
+        const float bc1 = 1.0f - (float)std::pow(0.9, (double)net->t);
+        const float bc2 = 1.0f - (float)std::pow(0.999, (double)net->t);
+        e = mlp_gradient(net->packed, rb.d, n, batch, seed, (uint64_t)net->t, alpha, -4.0f, 4.0f,
+                         net->partials, net->stream);
+        if (e == hipSuccess)
+            e = mlp_reduce(net->partials, blocks, nullptr, 1, lr, bc1, bc2, net->params, net->m1,
+                           net->m2, net->packed, d_loss + s, net->stream);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(net->stream);
+    if (e == hipSuccess && losses && steps > 0) {
+        e = hipMemcpy(losses, d_loss, sizeof(float) * steps, hipMemcpyDeviceToHost);
+        for (int s = 0; s < steps; s++) losses[s] /= (float)batch;
+    }
+    if (d_loss) (void)hipFree(d_loss);
+    if (e != hipSuccess) return hip_fail(e, "ptc_mlp_train");
+    if (steps > 0) net->trained = true;
+    ok();
+    return PTC_OK;
+}
+
+int ptc_mlp_probe_gemm(const float *a, const float *bt, float *c)
+{
+    if (!a || !bt || !c) return fail(PTC_EARG, "ptc_mlp_probe_gemm: null array");
+    float *d = nullptr;
+    HIP_TRY(hipMalloc(&d, sizeof(float) * 3 * 4096));
+    hipError_t e = hipMemcpy(d, a, sizeof(float) * 4096, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d + 4096, bt, sizeof(float) * 4096, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = mlp_gemm_probe(d, d + 4096, d + 8192, nullptr);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = hipMemcpy(c, d + 8192, sizeof(float) * 4096, hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    if (e != hipSuccess) return hip_fail(e, "ptc_mlp_probe_gemm");
     ok();
     return PTC_OK;
 }

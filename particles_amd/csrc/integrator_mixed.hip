@@ -1,0 +1,53 @@
+// Mixed-precision kernels (FP32 fields): the integrator of integrator.hip
+// instantiated for FP32 node storage, FP32 trilinear weights and RHS, with
+// FP64 positions, time, stage accumulation and error norm.  This unit is
+// compiled with FMA contraction: the mixed path is not bitwise comparable to
+// the reference anyway, and fused multiply-adds in the FP64 stage sums and
+// the FP32 gathers both cut instructions and round less.
+#include "integrator_impl.hpp"
+
+namespace ptc {
+
+hipError_t launch_integrate_mixed(int method, const DeviceFields &f, const StepParams &sp,
+                                  const DeviceParticles &p, double t_stop, const LaunchConfig &lc,
+                                  hipStream_t stream)
+{
+    if (f.prec != PTC_FP32 || f.layout != LAYOUT_AOS) return hipErrorInvalidValue;
+    const int geom = sp.pow2 ? 1 : 0;
+#define DISPATCH(M, MO, C, CT, G)                                              \
+    if (method == M && f.mode == MO && lc.cell_cache == C && lc.ctrl == CT && geom == G) \
+        return launch_t<M, MO, LAYOUT_AOS, C, PTC_FP32, CT, G>(f, sp, p, t_stop, lc, stream);
+#define DISPATCH_MODES(C, CT, G)                                               \
+    DISPATCH(PTC_RK5, PTC_MODE_VB, C, CT, G)                                   \
+    DISPATCH(PTC_RK5, PTC_MODE_EB, C, CT, G)                                   \
+    DISPATCH(PTC_DOP853, PTC_MODE_VB, C, CT, G)                                \
+    DISPATCH(PTC_DOP853, PTC_MODE_EB, C, CT, G)
+    DISPATCH_MODES(1, CTRL_PI, 1)
+    DISPATCH_MODES(1, CTRL_RECORD, 1)
+    DISPATCH_MODES(1, CTRL_LEARNED, 1)
+    DISPATCH_MODES(1, CTRL_PI, 0)
+    DISPATCH_MODES(1, CTRL_RECORD, 0)
+    DISPATCH_MODES(1, CTRL_LEARNED, 0)
+    if (lc.ctrl == CTRL_PI && lc.cell_cache == 0) {
+        if (method == PTC_RK5 && f.mode == PTC_MODE_VB)
+            return launch_t<PTC_RK5, PTC_MODE_VB, LAYOUT_AOS, 0, PTC_FP32, CTRL_PI, 0>(f, sp, p, t_stop, lc, stream);
+        if (method == PTC_RK5)
+            return launch_t<PTC_RK5, PTC_MODE_EB, LAYOUT_AOS, 0, PTC_FP32, CTRL_PI, 0>(f, sp, p, t_stop, lc, stream);
+        if (f.mode == PTC_MODE_VB)
+            return launch_t<PTC_DOP853, PTC_MODE_VB, LAYOUT_AOS, 0, PTC_FP32, CTRL_PI, 0>(f, sp, p, t_stop, lc, stream);
+        return launch_t<PTC_DOP853, PTC_MODE_EB, LAYOUT_AOS, 0, PTC_FP32, CTRL_PI, 0>(f, sp, p, t_stop, lc, stream);
+    }
+#undef DISPATCH_MODES
+#undef DISPATCH
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_probe_mixed(const DeviceFields &f, const StepParams &sp, int op, int64_t n,
+                              const double *in, double *out, hipStream_t stream)
+{
+    if (f.mode == PTC_MODE_VB)
+        return launch_probe_t<PTC_MODE_VB, LAYOUT_AOS, PTC_FP32>(f, sp, op, n, in, out, stream);
+    return launch_probe_t<PTC_MODE_EB, LAYOUT_AOS, PTC_FP32>(f, sp, op, n, in, out, stream);
+}
+
+}  // namespace ptc

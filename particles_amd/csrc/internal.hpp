@@ -7,6 +7,8 @@
 
 namespace ptc {
 
+struct MlpPacked;
+
 enum Layout : int { LAYOUT_SOA = 0, LAYOUT_AOS = 1, LAYOUT_BRICK = 2 };
 
 // Device copy of one field snapshot.
@@ -39,6 +41,8 @@ struct StepParams {
     int dm[3];
     int m;             // number of snapshots
     int64_t max_steps; // 0 = no cap
+    double rxs[3], rvun;  // 1 / xs, 1 / vun
+    int pow2;             // every xs and vun a power of two (divisions are exact scalings)
 };
 
 // Structure-of-arrays particle state resident in HBM.
@@ -55,20 +59,40 @@ struct DeviceParticles {
     const double *time;  // time(m)
     const int32_t *order;// work-queue permutation (nullptr = identity)
     int32_t *qcount;     // work-queue head
+    // step-size controller: the reference's PI rule (CTRL_PI), the same with
+    // sampled training records (CTRL_RECORD), or network proposals after
+    // accepted steps with the PI rule after rejections (CTRL_LEARNED)
+    const struct MlpPacked *net;  // CTRL_LEARNED
+    float ctrl_shift;             // CTRL_LEARNED: added to the network's log factor
+    uint64_t rec_thresh;          // CTRL_RECORD: sample an accepted step when hash < thresh
+    float *pend;                  // CTRL_RECORD: np x 16 features awaiting their target
+    double *pend_h;               // CTRL_RECORD: np, step after which they were taken
+    int64_t *pend_rej;            // CTRL_RECORD: np, rejections at that point
+    float *rec;                   // CTRL_RECORD: rec_cap x kRecord
+    unsigned long long *rec_count;// CTRL_RECORD: records claimed (may exceed rec_cap)
+    int64_t rec_cap;
 };
 
-enum : int32_t { FLAG_REJECTED = 1, FLAG_STARTED = 2 };
+enum : int32_t { FLAG_REJECTED = 1, FLAG_STARTED = 2, FLAG_PENDING = 4 };
+enum : int { CTRL_PI = 0, CTRL_RECORD = 1, CTRL_LEARNED = 2 };
 
 struct LaunchConfig {
     int block;
     int grid_blocks;   // 0 = occupancy-derived
     int cell_cache;    // keep the last cell's corner values per lane (AOS only): 0 off, 1 on
+    int ctrl;          // CTRL_*; RECORD and LEARNED need the AOS layout with the cell cache
 };
 
 // Launches the integrator on `stream`; returns a hipError_t.
 hipError_t launch_integrate(int method, const DeviceFields &f, const StepParams &sp,
                             const DeviceParticles &p, double t_stop, const LaunchConfig &lc,
                             hipStream_t stream);
+// The same for FP32 fields (integrator_mixed.hip).
+hipError_t launch_integrate_mixed(int method, const DeviceFields &f, const StepParams &sp,
+                                  const DeviceParticles &p, double t_stop, const LaunchConfig &lc,
+                                  hipStream_t stream);
+hipError_t launch_probe_mixed(const DeviceFields &f, const StepParams &sp, int op, int64_t n,
+                              const double *in, double *out, hipStream_t stream);
 // Evaluates one building block per input on the device (ptc_probe).
 hipError_t launch_probe(const DeviceFields &f, const StepParams &sp, int op, int64_t n,
                         const double *in, double *out, hipStream_t stream);
@@ -77,9 +101,26 @@ hipError_t launch_totals(const DeviceParticles &p, int64_t *d_out, hipStream_t s
 // Zeroes n int32 values (graph-safe replacement for hipMemsetAsync).
 hipError_t launch_zero_i32(int32_t *ptr, int64_t n, hipStream_t stream);
 hipError_t launch_zero_i64(int64_t *ptr, int64_t n, hipStream_t stream);
+// Clears bit in n int32 flags.
+hipError_t launch_clear_flag(int32_t *flags, int64_t n, int32_t bit, hipStream_t stream);
 // SOA planes -> AOS nodes (and FP64 -> FP32 when prec is PTC_FP32).
 hipError_t launch_to_aos(const double *soa, int64_t nodes, void *aos, int prec, hipStream_t stream);
 // AOS nodes -> per-cell corner bricks.
 hipError_t launch_to_bricks(const DeviceFields &f, hipStream_t stream);
+
+// step-size network (mlp.hip; layouts in mlp_device.hpp)
+constexpr int kMlpMaxGradBlocks = 512;
+hipError_t mlp_pack(const float *params, MlpPacked *packed, hipStream_t stream);
+hipError_t mlp_forward(const MlpPacked *packed, int64_t n, const float *feat, int ld, float *out,
+                       hipStream_t stream);
+int mlp_grad_blocks(int64_t batch);
+hipError_t mlp_gradient(const MlpPacked *packed, const float *rec, int64_t n, int64_t batch,
+                        uint64_t seed, uint64_t step, float alpha, float lo, float hi,
+                        float *partials, hipStream_t stream);
+hipError_t mlp_reduce(const float *partials, int nblocks, float *grad_out, int apply, float lr,
+                      float bc1, float bc2, float *params, float *m1, float *m2, MlpPacked *packed,
+                      float *loss_out, hipStream_t stream);
+hipError_t mlp_stats(const float *rec, int64_t n, double *partials, int blocks, hipStream_t stream);
+hipError_t mlp_gemm_probe(const float *a, const float *bt, float *c, hipStream_t stream);
 
 }  // namespace ptc
