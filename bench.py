@@ -10,6 +10,14 @@ particle by one gyro-period of B0 (T0 = 2 pi); the integration continues
 across steps exactly as one long run would.  Inputs are synthetic (seeded)
 and resident in HBM before the timed region.
 
+After the timed plain run, the same workload runs (untimed for `value`) in
+mixed precision (FP32 fields, weights and RHS; FP64 state, sums and error
+norm) and with the learned step-size proposal in mixed precision (network
+trained on the GPU from records of a PI-controlled run of a separate particle
+set); the `variants` object reports their rates, rejected-step fractions and
+RHS counts, their physical-time throughput relative to the plain run, and
+their deviation from the plain FP64 result.
+
 Prints one JSON line.  `roofline` is the integrator kernel's algorithmic
 gather traffic (8 corners x 6 components x 8 B = 384 B per RHS evaluation,
 times the RHS evaluations the kernel counted) over its duration measured with
@@ -102,6 +110,118 @@ def cpu_baseline(uu, bb, p, tg, st0, method, target_s):
                       f"({acc} accepted steps in {el:.2f} s, {threads} OpenMP threads)"}
 
 
+def traffic_from_profile(cfg):
+    """HBM bytes per integrator launch of this workload from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (scaled by a copy-kernel
+    calibration), or None when no summary matches the configuration."""
+    path = os.path.join(ROOT, "profiles", "integrate_dop853_turbulent256.json")
+    try:
+        with open(path) as fh:
+            prof = json.load(fh)
+    except (OSError, ValueError):
+        return None
+    keys = ("grid", "particles", "method", "tol", "layout", "cell_cache", "field_precision")
+    if any(prof.get("config", {}).get(k) != cfg.get(k) for k in keys):
+        return None
+    return prof["hbm_bytes_per_launch"]
+
+
+def run_variant(fields, method, p, tg, st, steps, warmup, net=None, shift=0.0):
+    """Advances a fresh batch warmup + steps gyro-periods; returns the timed
+    part's counters and wall time, and the snapshots."""
+    from particles_amd import workloads as W
+    import particles_amd as P
+    with P.Particles(fields, method, p, tg, st) as b:
+        if net is not None:
+            b.use_mlp(net, shift)
+        t = 0.0
+        for _ in range(warmup):
+            t += W.T0
+            b.advance(t)
+        tot0 = b.totals()
+        kms = 0.0
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            t += W.T0
+            kms += b.advance(t)
+        wall = time.perf_counter() - t0
+        tot1 = b.totals()
+        state = b.download()[0]
+    acc = tot1["accepted"] - tot0["accepted"]
+    rej = tot1["rejected"] - tot0["rejected"]
+    rhs = tot1["rhs"] - tot0["rhs"]
+    return dict(acc=acc, rej=rej, rhs=rhs, wall=wall, kernel_ms=kms, t_end=t), state
+
+
+def variants(args, fields64, uu, bb, p, tg, st0, method, plain):
+    """Mixed precision and learned-h (mixed) on the same workload, then their
+    accuracy next to the plain run's on the first particles over one T0."""
+    import particles_amd as P
+    from particles_amd import workloads as W
+
+    out = {}
+    time_per_s_plain = st0.shape[2] * args.steps / plain["wall"]
+    f32 = P.Fields(uu, bb, prec="fp32")
+    r, _ = run_variant(f32, method, p, tg, st0, args.steps, args.warmup)
+    out["mixed"] = summary(r, st0.shape[2], args.steps, time_per_s_plain)
+    print(f"mixed: {out['mixed']['value']:.3e} accepted/s", file=sys.stderr, flush=True)
+    # network trained on records of a separate particle set under the PI
+    # controller, its log-step offset calibrated on that set against PI
+    from particles_amd.learned import train_controller
+    st_train = W.initial_state(args.train_particles, 2, seed=args.seed + 101,
+                               gamma_range=tuple(args.train_gamma))
+    ctl = train_controller(f32, method, p, tg, st_train, fields_fp64=fields64, horizon=W.T0,
+                           steps=args.train_steps, seed=args.seed)
+    if args.shift is not None:
+        ctl.shift = args.shift
+    net = ctl.net
+    r, _ = run_variant(f32, method, p, tg, st0, args.steps, args.warmup, net, ctl.shift)
+    out["learned_h"] = summary(r, st0.shape[2], args.steps, time_per_s_plain)
+    out["learned_h"]["controller"] = dict(
+        network="16-64-64-1 GELU, BF16 MFMA, FP32 accumulation", shift=ctl.shift,
+        record_run=f"{args.train_particles} particles (separate seed), 1 T0, PI controller",
+        **ctl.info)
+    print(f"learned-h: {out['learned_h']['value']:.3e} accepted/s (shift {ctl.shift})",
+          file=sys.stderr, flush=True)
+
+    # accuracy: one T0 from the initial state, snapshot at the end, against
+    # the FP64 integrator at atol = rtol = 1e-12
+    n = min(args.accuracy_particles, st0.shape[2])
+    q = p.copy()
+    q[2] = W.T0
+    tga = np.array([0.0, W.T0])
+    sa = np.asfortranarray(st0[:, :1, :n].repeat(2, axis=1))
+    tight = q.copy()
+    tight[5] = tight[6] = 1e-12
+    ref = P.integrate(fields64, method, tight, tga, sa)[0]
+
+    def err(state):
+        d = np.abs(state[:6, 1, :] - ref[:6, 1, :]).max(0)
+        return {"median": float(np.median(d)), "p90": float(np.percentile(d, 90))}
+
+    acc_plain = err(P.integrate(fields64, method, q, tga, sa)[0])
+    out["mixed"]["error_vs_fp64_tol1e-12"] = err(P.integrate(f32, method, q, tga, sa)[0])
+    with P.Particles(f32, method, q, tga, sa) as b:
+        b.use_mlp(net, ctl.shift)
+        b.advance()
+        out["learned_h"]["error_vs_fp64_tol1e-12"] = err(b.download()[0])
+    out["plain_error_vs_fp64_tol1e-12"] = acc_plain
+    out["accuracy_sample"] = (f"first {n} particles, t = 0 .. 1 T0, max over x, u of "
+                              "|state - state(FP64, atol = rtol = 1e-12)|")
+    net.close()
+    f32.close()
+    return out
+
+
+def summary(r, npart, steps, time_per_s_plain):
+    tps = npart * steps / r["wall"]
+    return {"value": r["acc"] / r["wall"], "unit": "accepted particle-steps/s",
+            "particle_T0_per_s": tps, "time_throughput_vs_plain": tps / time_per_s_plain,
+            "accepted": r["acc"], "rejected": r["rej"],
+            "rejected_per_accepted": r["rej"] / max(r["acc"], 1),
+            "rhs_per_accepted": r["rhs"] / max(r["acc"], 1), "kernel_ms": r["kernel_ms"]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -119,6 +239,14 @@ def main():
                     help="fp32 = mixed precision (FP32 fields, weights and RHS; FP64 state)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-variants", action="store_true",
+                    help="skip the mixed-precision and learned-h runs")
+    ap.add_argument("--train-particles", type=int, default=20000)
+    ap.add_argument("--accuracy-particles", type=int, default=20000)
+    ap.add_argument("--train-gamma", type=float, nargs=2, default=(1.0, 1.0e3))
+    ap.add_argument("--train-steps", type=int, default=3000)
+    ap.add_argument("--shift", type=float, default=None,
+                    help="added to the network's log step factor (default: calibrated)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}): the particle batch would "
@@ -159,6 +287,12 @@ def main():
     rej = tot1["rejected"] - tot0["rejected"]
     rhs = tot1["rhs"] - tot0["rhs"]
     achieved = BYTES_PER_RHS_FP64 * rhs / (kernel_ms * 1e-3) / 1e9
+    batch.close()
+
+    var = None
+    if not args.no_variants and args.prec == "fp64":
+        print("variants ...", file=sys.stderr, flush=True)
+        var = variants(args, fields, uu, bb, p, tg, st0, method, dict(acc=acc, wall=wall))
 
     cpu = None
     if not args.no_cpu:
@@ -178,12 +312,15 @@ def main():
         "accepted": acc, "rejected": rej, "rejected_per_accepted": rej / max(acc, 1),
         "rhs_per_accepted": rhs / max(acc, 1), "kernel_ms": kernel_ms,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profile(cfg),
                      "bytes_per_rhs": BYTES_PER_RHS_FP64, "rhs_per_launch": rhs / args.steps},
         "cpu_baseline": cpu,
+        "variants": var,
     }
+    if args.prec == "fp32":
+        out["dtype"] = "f64 state, f32 fields and RHS"
+        out["variant"] = "mixed precision"
     print(json.dumps(out), flush=True)
-    batch.close()
     fields.close()
 
 

@@ -1,0 +1,95 @@
+"""Training workflow of the learned step-size proposal.
+
+The reference sizes every step with its PI controller
+(sources/integrations.F90:177,186 for RK5, :466-490 for DOP853).  Here the
+PI controller stays the fallback and the judge:
+
+  1. ``record``: the PI-controlled integrator runs a training particle set
+     and writes (features, next accepted step, error ratio) records on the
+     device (``Particles.record``);
+  2. ``train``: the network (16-64-64-1, BF16 matrix cores) is normalised to
+     the records' feature statistics and trained on the GPU by the library's
+     own forward/backward kernels, reading the records where they lie;
+  3. ``calibrate``: the network's log-step offset (``shift``) is chosen on the
+     training particles: among the candidates, the fastest one (integrator
+     kernel time) whose error against a tight-tolerance FP64 run is no larger
+     than the PI controller's own error on the same particles.  The result
+     says whether that candidate also ran faster than PI (``beats_pi``); where
+     it does not, the PI controller alone is the better choice for the
+     workload.
+
+Everything runs through the C-ABI; nothing here is on the integration path.
+"""
+from __future__ import annotations
+
+import time as _time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import Fields, Mlp, Particles, integrate
+
+
+@dataclass
+class Controller:
+    net: Mlp
+    shift: float
+    beats_pi: bool = False  # the calibrated shift ran faster than PI at no larger error
+    info: dict = field(default_factory=dict)
+
+
+def _final_error(state, ref):
+    d = np.abs(state[:6, -1, :] - ref[:6, -1, :]).max(0)
+    return float(np.median(d))
+
+
+def train_controller(fields: Fields, method: str, params, time, state_train, *,
+                     fields_fp64: Fields | None = None, horizon: float | None = None,
+                     steps: int = 3000, batch: int = 16384, lr: float = 2e-3, alpha: float = 1.0,
+                     shifts=(-0.35, -0.2, -0.1, 0.0, 0.1), capacity: int = 60_000_000,
+                     seed: int = 1) -> Controller:
+    """Records, trains and calibrates a learned controller for `fields` (the
+    precision it will run with).  `state_train` is (8, m, n) like every
+    state; records are taken over [0, horizon] (default params[2]).  The
+    calibration runs the same particles over the same horizon, with
+    `fields_fp64` (default: `fields`) at atol = rtol = 1e-12 as reference."""
+    params = np.asarray(params, dtype=np.float64)
+    q = params.copy()
+    if horizon is not None:
+        q[2] = horizon
+    tg = np.array([0.0, q[2]])
+    st = np.asfortranarray(np.asarray(state_train, dtype=np.float64)[:, :1, :].repeat(2, axis=1))
+    info = {}
+    t0 = _time.perf_counter()
+    net = Mlp(seed=seed)
+    with Particles(fields, method, q, tg, st) as b:
+        b.record(1.0, capacity)
+        ms_pi = b.advance()
+        recs = b.records(device=True)
+        info["records"] = len(recs)
+        net.normalize(recs)
+        losses = net.train(recs, steps=steps, batch=batch, lr=lr, alpha=alpha, seed=seed + 6)
+    info.update(loss_first=float(losses[:50].mean()), loss_last=float(losses[-50:].mean()),
+                train_steps=steps, batch=batch, alpha=alpha,
+                train_seconds=_time.perf_counter() - t0)
+
+    tight = q.copy()
+    tight[5] = tight[6] = 1e-12
+    ref = integrate(fields_fp64 if fields_fp64 is not None else fields, method, tight, tg, st)[0]
+    with Particles(fields, method, q, tg, st) as b:
+        ms_pi = b.advance()
+        e_pi = _final_error(b.download()[0], ref)
+    cand = []
+    for s in shifts:
+        with Particles(fields, method, q, tg, st) as b:
+            b.use_mlp(net, s)
+            ms = b.advance()
+            e = _final_error(b.download()[0], ref)
+        cand.append({"shift": float(s), "kernel_ms": ms, "error": e})
+    ok = [c for c in cand if c["error"] <= e_pi]
+    best = min(ok or cand, key=lambda c: c["kernel_ms"] if ok else c["error"])
+    beats = bool(ok) and best["kernel_ms"] < ms_pi
+    info.update(calibration=cand, pi_kernel_ms=ms_pi, pi_error=e_pi,
+                calibrated=bool(ok), beats_pi=beats, particles=int(st.shape[2]),
+                horizon=float(q[2]))
+    return Controller(net=net, shift=best["shift"], beats_pi=beats, info=info)

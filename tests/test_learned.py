@@ -174,3 +174,55 @@ def test_learned_mode_needs_a_trained_network_and_the_cell_cache():
             b.advance()
             _, c, status = b.download()
     assert (status == 0).all()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("method", ["dop853", "rk5"])
+def test_trained_network_gains_over_pi(method):
+    """Record with the PI controller on one particle set, train the network
+    on the GPU, integrate a different set: the learned controller rejects
+    far fewer steps and needs fewer RHS evaluations, at an error against a
+    tight-tolerance FP64 run no larger than the PI controller's (mixed
+    precision, tol 1e-8, turbulent 32^3 field, gamma in [1, 1e3])."""
+    import particles_amd as P
+    uu, bb = W.turbulent_vb(32, seed=1, nmodes=128)
+    p = W.params_vector(tmax=0.5 * W.T0, dtmax=1.0, atol=1e-8, rtol=1e-8)
+    tg = W.time_grid(1, p[2])
+    train = W.initial_state(20000, len(tg), seed=2, gamma_range=(1.0, 1e3))
+    test = W.initial_state(20000, len(tg), seed=3, gamma_range=(1.0, 1e3))
+    tight = p.copy()
+    tight[5] = tight[6] = 1e-12
+    with P.Fields(uu, bb) as f64:
+        ref, _, _ = P.integrate(f64, "dop853", tight, tg, test)
+
+    def err(s):
+        return float(np.median(np.abs(s[:6, -1, :] - ref[:6, -1, :]).max(0)))
+
+    with P.Fields(uu, bb, prec="fp32") as f:
+        with P.Particles(f, method, p, tg, train) as b:
+            b.record(1.0, 20_000_000)
+            b.advance()
+            r = b.records()
+        with P.Particles(f, method, p, tg, test) as b:
+            b.advance()
+            t_pi = b.totals()
+            e_pi = err(b.download()[0])
+        with P.Mlp(seed=1) as net:
+            net.normalize(r)
+            losses = net.train(r, steps=3000, batch=16384, lr=2e-3, alpha=1.0, seed=7)
+            with P.Particles(f, method, p, tg, test) as b:
+                b.use_mlp(net, -0.35)
+                b.advance()
+                t_l = b.totals()
+                s_l, _, status = b.download()
+    e_l = err(s_l)
+    rho_pi = t_pi["rejected"] / t_pi["accepted"]
+    rho_l = t_l["rejected"] / t_l["accepted"]
+    print(f"{method}: {len(r)} records, loss {losses[:50].mean():.3f} -> {losses[-50:].mean():.3f}; "
+          f"rejected/accepted PI {rho_pi:.3f} learned {rho_l:.3f}; RHS learned/PI "
+          f"{t_l['rhs'] / t_pi['rhs']:.3f}; median error PI {e_pi:.2e} learned {e_l:.2e}")
+    assert (status == 0).all()
+    assert losses[-50:].mean() < 0.5 * losses[:50].mean()
+    assert rho_l < (0.6 if method == "dop853" else 0.7) * rho_pi
+    assert t_l["rhs"] < 0.85 * t_pi["rhs"]
+    assert e_l <= 1.2 * e_pi
