@@ -43,13 +43,23 @@ BYTES_PER_RHS_FP64 = 8 * 6 * 8
 
 
 def build_workload(args):
+    """(make_fields(prec) -> Fields, host fields (uu, bb) or None, params, time
+    grid, initial state, method, config)."""
     from particles_amd import workloads as W
+    gr = None
+    device = False
     if args.workload == "turbulent256":
         dm, npart, method, tol, gr = 256, 100_000, "dop853", 1e-8, (1.0, 1.0e3)
-        tmax, nsnap = 1000 * W.T0, 10
+        tmax, nsnap, recipe = 1000 * W.T0, 10, "turbulent"
     elif args.workload == "turbulent64":
-        dm, npart, method, tol, gr = 64, 1_000, "rk5", 1e-6, None
-        tmax, nsnap = 100 * W.T0, 10
+        dm, npart, method, tol = 64, 1_000, "rk5", 1e-6
+        tmax, nsnap, recipe = 100 * W.T0, 10, "turbulent"
+    elif args.workload == "reconnection512":
+        dm, npart, method, tol, gr = 512, 1_000_000, "rk5", 1e-8, (1.0, 1.0e4)
+        tmax, nsnap, recipe, device = 10 * W.T0, 10, "reconnection", True
+    elif args.workload == "reconnection1024":
+        dm, npart, method, tol, gr = 1024, 4_000_000, "dop853", 1e-8, (1.0, 1.0e4)
+        tmax, nsnap, recipe, device = 10 * W.T0, 10, "reconnection", True
     else:
         raise SystemExit(f"unknown workload {args.workload}")
     if args.particles:
@@ -58,21 +68,32 @@ def build_workload(args):
         dm = args.grid
     if args.method:
         method = args.method
-    uu, bb = W.turbulent_vb(dm, seed=args.seed)
+    uu = bb = None
+    if recipe == "turbulent" and not device:
+        uu, bb = W.turbulent_vb(dm, seed=args.seed)
+
+        def make_fields(prec):
+            import particles_amd as P
+            return P.Fields(uu, bb, prec=prec)
+        desc = (f"{dm}^3 VB turbulent field (B0=1, dB rms 0.5, V rms 0.1, 256 modes)")
+    else:
+        def make_fields(prec):
+            return W.reconnection_eb_device(dm, seed=args.seed, prec=prec)
+        desc = (f"{dm}^3 EB reconnection field built on the device (B_x = sheet(y), width 0.02, "
+                f"guide 0.1, 256-mode noise rms 0.05; E = -V x B + 1e-3 curl B, V_y = -0.1 sheet(y))")
     p = W.params_vector(qom=1.0, vun=1.0, tmax=tmax, dtmax=W.T0, atol=tol, rtol=tol)
     tg = W.time_grid(nsnap, tmax)
     if gr is None:
         st0 = W.initial_state(npart, len(tg), seed=args.seed + 1, gamma=10.0)
     else:
         st0 = W.initial_state(npart, len(tg), seed=args.seed + 1, gamma_range=gr)
-    cfg = {"workload": f"{args.workload}: {method} tol={tol:g}, {dm}^3 VB turbulent field "
-                       f"(B0=1, dB rms 0.5, V rms 0.1, 256 modes), {npart} protons, "
+    cfg = {"workload": f"{args.workload}: {method} tol={tol:g}, {desc}, {npart} protons, "
                        f"gamma {'10' if gr is None else 'log-uniform [%g, %g]' % gr}, "
                        f"step = advance all particles by 1 gyro-period of B0",
            "grid": dm, "particles": npart, "method": method, "tol": tol,
            "snapshots_every_T0": tmax / W.T0 / nsnap, "layout": args.layout,
            "cell_cache": args.cell_cache, "field_precision": args.prec}
-    return uu, bb, p, tg, st0, method, cfg
+    return make_fields, uu, bb, p, tg, st0, method, cfg
 
 
 def cpu_baseline(uu, bb, p, tg, st0, method, target_s):
@@ -153,7 +174,7 @@ def run_variant(fields, method, p, tg, st, steps, warmup, net=None, shift=0.0):
     return dict(acc=acc, rej=rej, rhs=rhs, wall=wall, kernel_ms=kms, t_end=t), state
 
 
-def variants(args, fields64, uu, bb, p, tg, st0, method, plain):
+def variants(args, fields64, make_fields, p, tg, st0, method, plain):
     """Mixed precision and learned-h (mixed) on the same workload, then their
     accuracy next to the plain run's on the first particles over one T0."""
     import particles_amd as P
@@ -161,7 +182,7 @@ def variants(args, fields64, uu, bb, p, tg, st0, method, plain):
 
     out = {}
     time_per_s_plain = st0.shape[2] * args.steps / plain["wall"]
-    f32 = P.Fields(uu, bb, prec="fp32")
+    f32 = make_fields("fp32")
     r, _ = run_variant(f32, method, p, tg, st0, args.steps, args.warmup)
     out["mixed"] = summary(r, st0.shape[2], args.steps, time_per_s_plain)
     print(f"mixed: {out['mixed']['value']:.3e} accepted/s", file=sys.stderr, flush=True)
@@ -199,13 +220,26 @@ def variants(args, fields64, uu, bb, p, tg, st0, method, plain):
         d = np.abs(state[:6, 1, :] - ref[:6, 1, :]).max(0)
         return {"median": float(np.median(d)), "p90": float(np.percentile(d, 90))}
 
-    acc_plain = err(P.integrate(fields64, method, q, tga, sa)[0])
-    out["mixed"]["error_vs_fp64_tol1e-12"] = err(P.integrate(f32, method, q, tga, sa)[0])
+    s_plain = P.integrate(fields64, method, q, tga, sa)[0]
+    s_mixed = P.integrate(f32, method, q, tga, sa)[0]
     with P.Particles(f32, method, q, tga, sa) as b:
         b.use_mlp(net, ctl.shift)
         b.advance()
-        out["learned_h"]["error_vs_fp64_tol1e-12"] = err(b.download()[0])
-    out["plain_error_vs_fp64_tol1e-12"] = acc_plain
+        s_learned = b.download()[0]
+
+    def dev_plain(state):
+        # the same-seed FP64 run at the same tolerance: position/momentum and
+        # relative energy (gamma) deviation
+        d = np.abs(state[:6, 1, :] - s_plain[:6, 1, :]).max(0)
+        dg = np.abs(state[6, 1, :] - s_plain[6, 1, :]) / s_plain[6, 1, :]
+        return {"median": float(np.median(d)), "p90": float(np.percentile(d, 90)),
+                "gamma_rel_median": float(np.median(dg)), "gamma_rel_max": float(dg.max())}
+
+    out["plain_error_vs_fp64_tol1e-12"] = err(s_plain)
+    out["mixed"]["error_vs_fp64_tol1e-12"] = err(s_mixed)
+    out["learned_h"]["error_vs_fp64_tol1e-12"] = err(s_learned)
+    out["mixed"]["deviation_vs_plain_fp64"] = dev_plain(s_mixed)
+    out["learned_h"]["deviation_vs_plain_fp64"] = dev_plain(s_learned)
     out["accuracy_sample"] = (f"first {n} particles, t = 0 .. 1 T0, max over x, u of "
                               "|state - state(FP64, atol = rtol = 1e-12)|")
     net.close()
@@ -258,11 +292,11 @@ def main():
     from particles_amd import workloads as W
 
     t_setup = time.perf_counter()
-    uu, bb, p, tg, st0, method, cfg = build_workload(args)
+    make_fields, uu, bb, p, tg, st0, method, cfg = build_workload(args)
     P.set_option("layout", args.layout)
     P.set_option("cell_cache", args.cell_cache)
     P.set_option("grid_blocks", args.grid_blocks)
-    fields = P.Fields(uu, bb, prec=args.prec)
+    fields = make_fields(args.prec)
     batch = P.Particles(fields, method, p, tg, st0)
     print(f"setup {time.perf_counter() - t_setup:.1f}s: {cfg['workload']}", file=sys.stderr,
           flush=True)
@@ -292,10 +326,10 @@ def main():
     var = None
     if not args.no_variants and args.prec == "fp64":
         print("variants ...", file=sys.stderr, flush=True)
-        var = variants(args, fields, uu, bb, p, tg, st0, method, dict(acc=acc, wall=wall))
+        var = variants(args, fields, make_fields, p, tg, st0, method, dict(acc=acc, wall=wall))
 
     cpu = None
-    if not args.no_cpu:
+    if not args.no_cpu and uu is not None:
         print("cpu baseline ...", file=sys.stderr, flush=True)
         cpu = cpu_baseline(uu, bb, p, tg, st0, method, args.cpu_seconds)
 
@@ -308,7 +342,9 @@ def main():
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64", "data": "synthetic (seeded random-mode fields and particles)",
         "config": cfg,
-        "variant": "plain FP64, bit-identical to the reference",
+        "variant": ("plain FP64, bit-identical to the reference" if uu is not None else
+                    "plain FP64 (EB fields: bit-identical to the oracle; the reference has "
+                    "no E-field form)"),
         "accepted": acc, "rejected": rej, "rejected_per_accepted": rej / max(acc, 1),
         "rhs_per_accepted": rhs / max(acc, 1), "kernel_ms": kernel_ms,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

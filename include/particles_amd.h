@@ -73,6 +73,26 @@ int ptc_fields_create(const int dm[3], const double *f1, const double *bb, int m
 /* Replaces finalize_fields (sources/fields.F90:86-101). */
 int ptc_fields_destroy(ptc_fields *f);
 
+/* Synthesises a benchmark field on the device instead of uploading one (no
+ * counterpart in the reference, which reads FITS files; for grids too large
+ * to build on the host).  Node i sits at i / dm on the unit periodic box.
+ * Mode tables: n (3, nb) integer wave vectors, amp (nb), pol (3, nb),
+ * phase (nb); a table adds sum_m amp_m pol_m cos(2 pi n_m . x + phase_m).
+ *   PTC_RECIPE_TURBULENT_VB   : VB mode; B = scalars[0] z + B table,
+ *                               U = V table
+ *   PTC_RECIPE_RECONNECTION_EB: EB mode; B = s(y) x + scalars[1] z + B table,
+ *                               s(y) = tanh((y-1/4)/d) - tanh((y-3/4)/d) - 1,
+ *                               d = scalars[0]; E = -V x B + scalars[3] curl B
+ *                               (periodic central differences) with
+ *                               V = (0, -scalars[2] s(y), 0); no V table.
+ * prec as for ptc_fields_create. */
+#define PTC_RECIPE_TURBULENT_VB    0
+#define PTC_RECIPE_RECONNECTION_EB 1
+int ptc_fields_synthesize(const int dm[3], int recipe, const double *scalars, int nb,
+                          const int32_t *b_n, const double *b_amp, const double *b_pol,
+                          const double *b_phase, int nv, const int32_t *v_n, const double *v_amp,
+                          const double *v_pol, const double *v_phase, int prec, ptc_fields **out);
+
 /* The whole particle loop of sources/particles.F90:337-356 in one call:
  * integrates every particle from t = 0 to tmax = params(3), writes the
  * snapshots into state and the step counts into counter, exactly like

@@ -62,6 +62,8 @@ def _load():
     sig = {
         "ptc_fields_create": ([vp, vp, vp, i32, i32, C.POINTER(vp)], i32),
         "ptc_fields_destroy": ([vp], i32),
+        "ptc_fields_synthesize": ([vp, i32, vp, i32, vp, vp, vp, vp, i32, vp, vp, vp, vp, i32,
+                                   C.POINTER(vp)], i32),
         "ptc_integrate": ([vp, i32, i64, i32, vp, i32, vp, vp, vp, vp], i32),
         "ptc_particles_create": ([vp, i32, i64, i32, vp, i32, vp, vp, C.POINTER(vp)], i32),
         "ptc_particles_advance": ([vp, dbl], i32),
@@ -137,6 +139,37 @@ class Fields:
         _check(_lib.ptc_fields_create(_ptr(self.dm), _ptr(f1), _ptr(bb), MODES[mode], PRECS[prec],
                                       C.byref(h)))
         self._h = h
+
+    @classmethod
+    def synthesize(cls, dm, recipe: str, scalars, bmodes=None, vmodes=None, prec: str = "fp64"):
+        """A field built on the device (ptc_fields_synthesize): recipe
+        "turbulent_vb" (scalars = (b0,)) or "reconnection_eb" (scalars =
+        (delta, guide, vin, eta)); bmodes / vmodes are workloads.Modes."""
+        self = cls.__new__(cls)
+        self.dm = np.ascontiguousarray(dm, dtype=np.int32)
+        self.mode = "vb" if recipe == "turbulent_vb" else "eb"
+        self.prec = prec
+        code = {"turbulent_vb": 0, "reconnection_eb": 1}[recipe]
+        sc = np.zeros(8, dtype=np.float64)
+        sc[:len(scalars)] = scalars
+        keep = []
+
+        def table(m):
+            if m is None:
+                return 0, None, None, None, None
+            t = (np.ascontiguousarray(m.n, dtype=np.int32), np.ascontiguousarray(m.amp, dtype=np.float64),
+                 np.ascontiguousarray(m.pol, dtype=np.float64),
+                 np.ascontiguousarray(m.phase, dtype=np.float64))
+            keep.append(t)
+            return len(m.amp), *(_ptr(a) for a in t)
+
+        nb, bn, ba, bp, bph = table(bmodes)
+        nv, vn, va, vpo, vph = table(vmodes)
+        h = C.c_void_p()
+        _check(_lib.ptc_fields_synthesize(_ptr(self.dm), code, _ptr(sc), nb, bn, ba, bp, bph, nv,
+                                          vn, va, vpo, vph, PRECS[prec], C.byref(h)))
+        self._h = h
+        return self
 
     @property
     def handle(self):

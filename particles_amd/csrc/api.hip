@@ -154,35 +154,18 @@ int ptc_get_option(const char *key, char *buf, int len)
     return PTC_OK;
 }
 
-int ptc_fields_create(const int dm[3], const double *f1, const double *bb, int mode, int prec,
-                      ptc_fields **out)
+}  // extern "C"
+
+// Builds the handle from six SOA planes already on the device (`soa` is
+// taken over: freed here, or kept as the field array for the SOA layout).
+static int fields_from_device_soa(const int dm[3], double *soa, int mode, int prec,
+                                  ptc_fields **out)
 {
-    if (!out) return fail(PTC_EARG, "ptc_fields_create: out is null");
-    *out = nullptr;
-    if (!dm || !f1 || !bb) return fail(PTC_EARG, "ptc_fields_create: null dm/field pointer");
-    if (dm[0] < 2 || dm[1] < 2 || dm[2] < 2)
-        return fail(PTC_EARG, "ptc_fields_create: dimensions must be >= 2 (got %d %d %d)", dm[0],
-                    dm[1], dm[2]);
-    if (mode != PTC_MODE_VB && mode != PTC_MODE_EB) return fail(PTC_EARG, "bad field mode %d", mode);
-    if (prec != PTC_FP64 && prec != PTC_FP32)
-        return fail(PTC_EARG, "field precision %d not supported", prec);
     const int64_t nodes = (int64_t)dm[0] * dm[1] * dm[2];
-    if (nodes > (int64_t)0xFFFFFFFFll)
-        return fail(PTC_EARG, "ptc_fields_create: %lld nodes exceed 32-bit node offsets",
-                    (long long)nodes);
     Options o = options();
     if (prec == PTC_FP32) o.layout = LAYOUT_AOS;  // FP32 fields are always node rows
-
-    double *soa = nullptr;
-    HIP_TRY(hipMalloc(&soa, sizeof(double) * 6 * nodes));
-    hipError_t e = hipMemcpy(soa, f1, sizeof(double) * 3 * nodes, hipMemcpyHostToDevice);
-    if (e == hipSuccess)
-        e = hipMemcpy(soa + 3 * nodes, bb, sizeof(double) * 3 * nodes, hipMemcpyHostToDevice);
-    if (e != hipSuccess) {
-        (void)hipFree(soa);
-        return hip_fail(e, "ptc_fields_create: upload");
-    }
     void *data = soa, *bricks = nullptr;
+    hipError_t e = hipSuccess;
     if (o.layout == LAYOUT_AOS || o.layout == LAYOUT_BRICK) {
         void *aos = nullptr;
         e = hipMalloc(&aos, prec == PTC_FP32 ? sizeof(float) * 8 * nodes : sizeof(double) * 6 * nodes);
@@ -191,7 +174,7 @@ int ptc_fields_create(const int dm[3], const double *f1, const double *bb, int m
         (void)hipFree(soa);
         if (e != hipSuccess) {
             if (aos) (void)hipFree(aos);
-            return hip_fail(e, "ptc_fields_create: interleave");
+            return hip_fail(e, "ptc_fields: interleave");
         }
         data = aos;
     }
@@ -207,7 +190,7 @@ int ptc_fields_create(const int dm[3], const double *f1, const double *bb, int m
         if (e != hipSuccess) {
             if (bricks) (void)hipFree(bricks);
             (void)hipFree(data);
-            return hip_fail(e, "ptc_fields_create: bricks");
+            return hip_fail(e, "ptc_fields: bricks");
         }
     }
     ptc_fields *f = new ptc_fields;
@@ -223,6 +206,119 @@ int ptc_fields_create(const int dm[3], const double *f1, const double *bb, int m
     *out = f;
     ok();
     return PTC_OK;
+}
+
+static int check_field_args(const int dm[3], int mode, int prec)
+{
+    if (dm[0] < 2 || dm[1] < 2 || dm[2] < 2)
+        return fail(PTC_EARG, "ptc_fields: dimensions must be >= 2 (got %d %d %d)", dm[0],
+                    dm[1], dm[2]);
+    if (mode != PTC_MODE_VB && mode != PTC_MODE_EB) return fail(PTC_EARG, "bad field mode %d", mode);
+    if (prec != PTC_FP64 && prec != PTC_FP32)
+        return fail(PTC_EARG, "field precision %d not supported", prec);
+    const int64_t nodes = (int64_t)dm[0] * dm[1] * dm[2];
+    if (nodes > (int64_t)0xFFFFFFFFll)
+        return fail(PTC_EARG, "ptc_fields: %lld nodes exceed 32-bit node offsets", (long long)nodes);
+    return PTC_OK;
+}
+
+extern "C" {
+
+int ptc_fields_create(const int dm[3], const double *f1, const double *bb, int mode, int prec,
+                      ptc_fields **out)
+{
+    if (!out) return fail(PTC_EARG, "ptc_fields_create: out is null");
+    *out = nullptr;
+    if (!dm || !f1 || !bb) return fail(PTC_EARG, "ptc_fields_create: null dm/field pointer");
+    int rc = check_field_args(dm, mode, prec);
+    if (rc) return rc;
+    const int64_t nodes = (int64_t)dm[0] * dm[1] * dm[2];
+    double *soa = nullptr;
+    HIP_TRY(hipMalloc(&soa, sizeof(double) * 6 * nodes));
+    hipError_t e = hipMemcpy(soa, f1, sizeof(double) * 3 * nodes, hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(soa + 3 * nodes, bb, sizeof(double) * 3 * nodes, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)hipFree(soa);
+        return hip_fail(e, "ptc_fields_create: upload");
+    }
+    return fields_from_device_soa(dm, soa, mode, prec, out);
+}
+
+int ptc_fields_synthesize(const int dm[3], int recipe, const double *scalars, int nb,
+                          const int32_t *b_n, const double *b_amp, const double *b_pol,
+                          const double *b_phase, int nv, const int32_t *v_n, const double *v_amp,
+                          const double *v_pol, const double *v_phase, int prec, ptc_fields **out)
+{
+    if (!out) return fail(PTC_EARG, "ptc_fields_synthesize: out is null");
+    *out = nullptr;
+    if (!dm || !scalars) return fail(PTC_EARG, "ptc_fields_synthesize: null dm/scalars");
+    if (recipe != PTC_RECIPE_TURBULENT_VB && recipe != PTC_RECIPE_RECONNECTION_EB)
+        return fail(PTC_EARG, "ptc_fields_synthesize: unknown recipe %d", recipe);
+    const int mode = recipe == PTC_RECIPE_TURBULENT_VB ? PTC_MODE_VB : PTC_MODE_EB;
+    int rc = check_field_args(dm, mode, prec);
+    if (rc) return rc;
+    if (nb < 0 || nv < 0 || (nb > 0 && (!b_n || !b_amp || !b_pol || !b_phase)) ||
+        (nv > 0 && (!v_n || !v_amp || !v_pol || !v_phase)))
+        return fail(PTC_EARG, "ptc_fields_synthesize: bad mode tables");
+    if (recipe == PTC_RECIPE_RECONNECTION_EB && !(scalars[0] > 0.0))
+        return fail(PTC_EARG, "ptc_fields_synthesize: sheet width must be > 0");
+    const int64_t nodes = (int64_t)dm[0] * dm[1] * dm[2];
+    double *soa = nullptr;
+    HIP_TRY(hipMalloc(&soa, sizeof(double) * 6 * nodes));
+    // mode tables: n (int32), then amp, pol, phase (double), for B then V
+    const int nt = nb + nv;
+    void *tab = nullptr;
+    hipError_t e = hipMalloc(&tab, (sizeof(int32_t) * 4 + sizeof(double) * 5) * (nt > 0 ? nt : 1));
+    auto fin = [&](hipError_t err, const char *what) {
+        (void)hipFree(soa);
+        if (tab) (void)hipFree(tab);
+        return hip_fail(err, what);
+    };
+    if (e != hipSuccess) return fin(e, "ptc_fields_synthesize: tables");
+    double *dd = (double *)tab;          // amp (nt), pol (3 nt), phase (nt)
+    int32_t *dn = (int32_t *)(dd + 5 * (size_t)(nt > 0 ? nt : 1));
+    std::vector<double> h(5 * (size_t)(nt > 0 ? nt : 1), 0.0);
+    std::vector<int32_t> hn(3 * (size_t)(nt > 0 ? nt : 1), 0);
+    for (int m = 0; m < nt; m++) {
+        const bool isb = m < nb;
+        const int q = isb ? m : m - nb;
+        h[m] = isb ? b_amp[q] : v_amp[q];
+        for (int c = 0; c < 3; c++) {
+            h[nt + 3 * m + c] = isb ? b_pol[3 * q + c] : v_pol[3 * q + c];
+            hn[3 * m + c] = isb ? b_n[3 * q + c] : v_n[3 * q + c];
+        }
+        h[4 * nt + m] = isb ? b_phase[q] : v_phase[q];
+    }
+    if (nt > 0) {
+        e = hipMemcpy(dd, h.data(), sizeof(double) * 5 * nt, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(dn, hn.data(), sizeof(int32_t) * 3 * nt, hipMemcpyHostToDevice);
+        if (e != hipSuccess) return fin(e, "ptc_fields_synthesize: tables");
+    }
+    double *f1 = soa, *b = soa + 3 * nodes;
+    e = synth_fill(soa, 6 * nodes, 0.0, nullptr);
+    if (e == hipSuccess)
+        e = synth_modes(dm, nb, dn, dd, dd + nt, dd + 4 * nt, b, nullptr);
+    if (recipe == PTC_RECIPE_TURBULENT_VB) {
+        // U = V modes, B = b0 z + B modes
+        if (e == hipSuccess)
+            e = synth_modes(dm, nv, dn + 3 * nb, dd + nb, dd + nt + 3 * nb, dd + 4 * nt + nb, f1,
+                            nullptr);
+        if (e == hipSuccess) e = synth_add(b + 2 * nodes, nodes, scalars[0], nullptr);
+    } else {
+        // B += sheet(y) x + guide z; E = -V x B + eta curl B
+        if (e == hipSuccess) e = synth_sheet(dm, scalars[0], scalars[1], b, nullptr);
+        if (e == hipSuccess)
+            e = synth_reconnection_e(dm, scalars[0], scalars[2], scalars[3], b, f1, nullptr);
+    }
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    (void)hipFree(tab);
+    tab = nullptr;
+    if (e != hipSuccess) {
+        (void)hipFree(soa);
+        return hip_fail(e, "ptc_fields_synthesize");
+    }
+    return fields_from_device_soa(dm, soa, mode, prec, out);
 }
 
 int ptc_fields_destroy(ptc_fields *f)

@@ -108,6 +108,17 @@ hipError_t launch_to_aos(const double *soa, int64_t nodes, void *aos, int prec, 
 // AOS nodes -> per-cell corner bricks.
 hipError_t launch_to_bricks(const DeviceFields &f, hipStream_t stream);
 
+// field synthesis (fields.hip): fill / add a constant, add a mode sum to three
+// planes, add the reconnection sheet and guide field to B, and E of the
+// reconnection recipe from B
+hipError_t synth_fill(double *p, int64_t n, double v, hipStream_t s);
+hipError_t synth_add(double *p, int64_t n, double v, hipStream_t s);
+hipError_t synth_modes(const int dm[3], int nm, const int32_t *n, const double *amp,
+                       const double *pol, const double *phase, double *out, hipStream_t s);
+hipError_t synth_sheet(const int dm[3], double delta, double guide, double *b, hipStream_t s);
+hipError_t synth_reconnection_e(const int dm[3], double delta, double vin, double eta,
+                                const double *b, double *e, hipStream_t s);
+
 // step-size network (mlp.hip; layouts in mlp_device.hpp)
 constexpr int kMlpMaxGradBlocks = 512;
 hipError_t mlp_pack(const float *params, MlpPacked *packed, hipStream_t stream);

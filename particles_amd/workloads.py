@@ -185,6 +185,25 @@ def reconnection_eb(dm: int, seed: int, delta: float = 0.02, guide: float = 0.1,
     return ee, bb
 
 
+def turbulent_vb_device(dm: int, seed: int, prec: str = "fp64", b0: float = 1.0,
+                        brms: float = 0.5, vrms: float = 0.1, nmodes: int = 256):
+    """turbulent_vb built on the device (particles_amd.Fields), same modes."""
+    from . import Fields
+    mb = random_modes(seed, nmodes=nmodes, rms=brms, stream=1)
+    mv = random_modes(seed, nmodes=nmodes, rms=vrms, stream=2)
+    return Fields.synthesize((dm, dm, dm), "turbulent_vb", (b0,), mb, mv, prec=prec)
+
+
+def reconnection_eb_device(dm: int, seed: int, prec: str = "fp64", delta: float = 0.02,
+                           guide: float = 0.1, noise_rms: float = 0.05, vin: float = 0.1,
+                           eta: float = 1.0e-3, nmodes: int = 256):
+    """reconnection_eb built on the device (particles_amd.Fields), same modes."""
+    from . import Fields
+    mb = random_modes(seed, nmodes=nmodes, rms=noise_rms, stream=1)
+    return Fields.synthesize((dm, dm, dm), "reconnection_eb", (delta, guide, vin, eta), mb,
+                             None, prec=prec)
+
+
 # --------------------------------------------------------------------------
 # initial states
 # --------------------------------------------------------------------------

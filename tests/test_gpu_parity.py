@@ -344,3 +344,32 @@ def test_fortran_caller_integrates():
     assert r.returncode == 0
     rows = [l.split() for l in r.stdout.splitlines() if l.strip().startswith("particle")]
     assert len(rows) == 10 and all(int(row[4]) > 0 for row in rows)
+
+
+# ---------------------------------------------------------------------------
+# device field synthesis (large-grid inputs)
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("recipe", ["turbulent_vb", "reconnection_eb"])
+def test_device_synthesis_matches_host_generator(recipe):
+    """ptc_fields_synthesize builds the same fields as the host generator
+    (inverse FFT of the same mode tables; sheet, guide field and E = -V x B
+    + eta curl B): gathering both at random points agrees to 1e-12."""
+    dm = 24
+    if recipe == "turbulent_vb":
+        f1, bb = W.turbulent_vb(dm, seed=3, nmodes=64)
+        dev = W.turbulent_vb_device(dm, seed=3, nmodes=64)
+        mode = "vb"
+    else:
+        f1, bb = W.reconnection_eb(dm, seed=3, nmodes=64)
+        dev = W.reconnection_eb_device(dm, seed=3, nmodes=64)
+        mode = "eb"
+    pos = np.random.default_rng(1).uniform(0, 1, (2000, 3))
+    p = W.params_vector()
+    with P.Fields(f1, bb, mode=mode) as host, dev:
+        a = P.probe(host, "gather", pos, p)
+        b = P.probe(dev, "gather", pos, p)
+    scale = max(np.abs(f1).max(), np.abs(bb).max())
+    err = np.abs(a - b).max() / scale
+    say(f"[synthesis {recipe}] max relative difference {err:.2e}")
+    assert err < 1e-12
