@@ -86,6 +86,43 @@ module ptc_binding
       import :: c_ptr
     end function
 
+    ! FITS primary-image I/O (include/particles_amd_fits.h); paths are
+    ! NUL-terminated
+    integer(c_int) function ptc_fits_dims(path, naxis, dims, bitpix) bind(C, name="ptc_fits_dims")
+      import :: c_int, c_int64_t, c_char
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int), intent(out)        :: naxis, bitpix
+      integer(c_int64_t), intent(out)    :: dims(8)
+    end function
+
+    integer(c_int) function ptc_fits_read(path, n, out) bind(C, name="ptc_fits_read")
+      import :: c_int, c_int64_t, c_char, c_double
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int64_t), value          :: n
+      real(c_double), intent(out)        :: out(*)
+    end function
+
+    integer(c_int) function ptc_fits_write_f64(path, naxis, dims, data, nkeys, keys, values, &
+                                               comments) bind(C, name="ptc_fits_write_f64")
+      import :: c_int, c_int64_t, c_char, c_double, c_ptr
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int), value              :: naxis, nkeys
+      integer(c_int64_t), intent(in)     :: dims(*)
+      real(c_double), intent(in)         :: data(*), values(*)
+      type(c_ptr), intent(in)            :: keys(*), comments(*)
+    end function
+
+    integer(c_int) function ptc_fits_write_i64(path, naxis, dims, data) bind(C, name="ptc_fits_write_i64")
+      import :: c_int, c_int64_t, c_char
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int), value              :: naxis
+      integer(c_int64_t), intent(in)     :: dims(*), data(*)
+    end function
+
+    type(c_ptr) function ptc_fits_last_error_c() bind(C, name="ptc_fits_last_error")
+      import :: c_ptr
+    end function
+
   end interface
 
 contains
@@ -93,10 +130,21 @@ contains
   ! ptc_last_error() as a Fortran string
   function ptc_last_error() result(msg)
     character(len=:), allocatable :: msg
+    msg = c_string(ptc_last_error_c())
+  end function
+
+  ! ptc_fits_last_error() as a Fortran string
+  function ptc_fits_last_error() result(msg)
+    character(len=:), allocatable :: msg
+    msg = c_string(ptc_fits_last_error_c())
+  end function
+
+  ! a NUL-terminated C string as a Fortran string
+  function c_string(cp) result(msg)
+    type(c_ptr), intent(in) :: cp
+    character(len=:), allocatable :: msg
     character(kind=c_char), pointer :: p(:)
-    type(c_ptr) :: cp
     integer :: n
-    cp = ptc_last_error_c()
     call c_f_pointer(cp, p, [4096])
     n = 0
     do while (n < 4096)

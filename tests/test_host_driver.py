@@ -1,0 +1,142 @@
+"""The Fortran host driver (particles_amd/fortran/particles_host): the
+reference program's parameter file, unit algebra, time grid and output
+files around the device integrator.  CPU: a dry run (no GPU) checks the
+parsed parameters and derived quantities against the reference's formulas
+(sources/particles.F90:163-214,240-280,317-333).  GPU: a full run writes
+counter.fits, time.fits and state.fits, and its snapshots equal the device
+integrator's on the initial states it wrote."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from particles_amd import fits
+from particles_amd import workloads as W
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "particles_amd", "fortran", "particles_host")
+C_LIGHT, KB, MP, ME, QE = 2.99792458e8, 1.38064852e-23, 1.672621898e-27, 9.10938356e-31, 1.6021766208e-19
+
+
+def _setup(tmp, params_text, dm=12, gz=False):
+    uu, bb = W.turbulent_vb(dm, seed=5, nmodes=24)
+    data = os.path.join(tmp, "data")
+    os.makedirs(data, exist_ok=True)
+    names = ("velx", "vely", "velz", "magx", "magy", "magz")
+    for k, n in enumerate(names):
+        arr = uu[..., k] if k < 3 else bb[..., k - 3]
+        path = os.path.join(data, n + ".fits")
+        fits.write_f64(path, arr)
+        if gz:
+            import gzip
+            with open(path, "rb") as s, gzip.open(path + ".gz", "wb") as d:
+                d.write(s.read())
+            os.remove(path)
+    with open(os.path.join(tmp, "params.in"), "w") as fh:
+        fh.write(params_text)
+    return uu, bb
+
+
+def _run(tmp, *args, timeout=300):
+    if not os.path.exists(EXE):
+        pytest.skip("host driver not built")
+    return subprocess.run([EXE, "-i", "params.in", *args], cwd=tmp, capture_output=True,
+                          text=True, timeout=timeout)
+
+
+def _floats(text, label):
+    m = re.search(re.escape(label) + r"\s*=\s*\n((?:\s+[-+0-9.E]+)+)", text)
+    return np.array([float(x) for x in m.group(1).split()])
+
+
+PARAMS = """# unit test
+method      = 'dop853'      # DOP853
+particle_type = electron
+nparticles  = 9
+nsteps      = 3
+time_scale  = log
+tmin        = 1.0e-2
+tmax        = 10.0
+tunit       = 2.0e-6
+vunit       = 3.0e5
+temperature = 5.0e4
+dtmax       = 0.5
+atol        = 1.0e-9
+rtol        = 1.0e-9
+zmax        = 0.25
+xmin_domain = -0.5
+xmax_domain = 0.5
+datadir     = "data"
+"""
+
+
+def test_dry_run_parameters_and_units(tmp_path):
+    tmp = str(tmp_path)
+    _setup(tmp, PARAMS, gz=True)
+    r = _run(tmp, "--dry-run")
+    assert r.returncode == 0, r.stdout + r.stderr
+    out = r.stdout
+    vunit = 3.0e5 / C_LIGHT
+    qom = -1.0e-4 * QE * 2.0e-6 / ME
+    vth = np.sqrt(KB * 5.0e4 / MP) / C_LIGHT
+    nsteps = int(np.log10(10.0) - np.log10(1.0e-2)) * 3 + 2
+    assert re.search(r"nsteps\s*=\s*%d\b" % nsteps, out)
+    p = _floats(out, "params(1:17)")
+    want = W.params_vector(qom=qom, vun=vunit, tmax=10.0, dtmax=0.5, atol=1e-9, rtol=1e-9,
+                           xl=(-0.5, 0, 0), xs=(1.0, 1.0, 1.0))[:17]
+    assert np.allclose(p, want, rtol=1e-14, atol=0)
+    t = _floats(out, "time(1:nsteps)")
+    assert np.allclose(t, W.time_grid(3, 10.0, "log", 1.0e-2), rtol=1e-14, atol=0)
+    got_vth = float(re.search(r"thermal speed\s*=\s*([-+0-9.E]+)", out).group(1))
+    got_qom = float(re.search(r"q/m\s*=\s*([-+0-9.E]+)", out).group(1))
+    assert abs(got_vth - vth) < 1e-14 * vth and abs(got_qom - qom) < 1e-14 * abs(qom)
+    # seeding region: x in [max(0, -0.5), min(1, 0.5)], z in [0, 0.25]
+    mean = {m.group(1): float(m.group(2)) for m in
+            re.finditer(r"(pos[xyz]):\s+([-+0-9.E]+)", out)}
+    assert 0.0 < mean["posx"] < 0.5 and 0.0 < mean["posz"] < 0.25
+    # momenta ~ normal(0, vth): 9 particles, odd count handled
+    assert re.search(r"momz:\s+[-+0-9.E]+\s+[0-9.E+-]+", out)
+
+
+def test_dry_run_errors(tmp_path):
+    tmp = str(tmp_path)
+    _setup(tmp, PARAMS.replace("'dop853'", "'euler'"))
+    r = _run(tmp, "--dry-run")
+    assert r.returncode != 0 and "unknown integration method" in r.stdout
+    _setup(tmp, PARAMS.replace('"data"', '"nowhere"'))
+    r = _run(tmp, "--dry-run")
+    assert r.returncode != 0 and "missing field file" in r.stdout
+
+
+@pytest.mark.gpu
+def test_full_run_writes_reference_files(tmp_path):
+    import particles_amd as P
+    tmp = str(tmp_path)
+    text = PARAMS.replace("particle_type = electron", "particle_type = proton").replace(
+        "tunit       = 2.0e-6", "tunit       = 1.0e-4").replace("nparticles  = 9", "nparticles  = 64")
+    uu, bb = _setup(tmp, text)
+    r = _run(tmp)
+    print(r.stdout[-600:])
+    assert r.returncode == 0, r.stdout + r.stderr
+    state = fits.read(os.path.join(tmp, "state.fits"))
+    counter = fits.read(os.path.join(tmp, "counter.fits"))
+    time = fits.read(os.path.join(tmp, "time.fits"))
+    assert state.shape == (8, len(time), 64) and counter.shape == (3, 64)
+    assert float(fits.header(os.path.join(tmp, "time.fits"))["TUNIT"]) == 1.0e-4
+    assert float(fits.header(os.path.join(tmp, "state.fits"))["TEMP"]) == 5.0e4
+    assert fits.dims(os.path.join(tmp, "counter.fits"))[1] == 64
+    # the same integration through the library on the written initial state
+    vunit = 3.0e5 / C_LIGHT
+    qom = 1.0e-4 * QE * 1.0e-4 / MP
+    p = W.params_vector(qom=qom, vun=vunit, tmax=10.0, dtmax=0.5, atol=1e-9, rtol=1e-9,
+                        xl=(-0.5, 0, 0), xs=(1.0, 1.0, 1.0))
+    st0 = np.zeros_like(state, order="F")
+    st0[:, 0, :] = state[:, 0, :]
+    st0[6:, 0, :] = 0.0
+    with P.Fields(vunit * uu, 1.0 * bb) as f:
+        st, cnt, status = P.integrate(f, "dop853", p, time, st0)
+    assert (status == 0).all()
+    assert np.array_equal(st[:, 1:, :], state[:, 1:, :])
+    assert np.array_equal(cnt.astype(np.float64), counter)
