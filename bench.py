@@ -92,7 +92,8 @@ def build_workload(args):
                        f"step = advance all particles by 1 gyro-period of B0",
            "grid": dm, "particles": npart, "method": method, "tol": tol,
            "snapshots_every_T0": tmax / W.T0 / nsnap, "layout": args.layout,
-           "cell_cache": args.cell_cache, "field_precision": args.prec}
+           "cell_cache": args.cell_cache, "field_precision": args.prec, "sort": args.sort,
+           "order": args.order}
     return make_fields, uu, bb, p, tg, st0, method, cfg
 
 
@@ -141,7 +142,8 @@ def traffic_from_profile(cfg):
             prof = json.load(fh)
     except (OSError, ValueError):
         return None
-    keys = ("grid", "particles", "method", "tol", "layout", "cell_cache", "field_precision")
+    keys = ("grid", "particles", "method", "tol", "layout", "cell_cache", "field_precision",
+            "sort", "order")
     if any(prof.get("config", {}).get(k) != cfg.get(k) for k in keys):
         return None
     return prof["hbm_bytes_per_launch"]
@@ -269,6 +271,10 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--cell-cache", type=int, default=1, choices=[0, 1])
     ap.add_argument("--grid-blocks", type=int, default=0)
+    ap.add_argument("--sort", type=int, default=1,
+                    help="re-order the work queue every N steps (0 = never)")
+    ap.add_argument("--order", default="cost", choices=["spatial", "cost"],
+                    help="cost: most work in the previous step first; spatial: Morton order")
     ap.add_argument("--prec", default="fp64", choices=["fp64", "fp32"],
                     help="fp32 = mixed precision (FP32 fields, weights and RHS; FP64 state)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -296,6 +302,8 @@ def main():
     P.set_option("layout", args.layout)
     P.set_option("cell_cache", args.cell_cache)
     P.set_option("grid_blocks", args.grid_blocks)
+    P.set_option("sort", args.sort)
+    P.set_option("order", args.order)
     fields = make_fields(args.prec)
     batch = P.Particles(fields, method, p, tg, st0)
     print(f"setup {time.perf_counter() - t_setup:.1f}s: {cfg['workload']}", file=sys.stderr,
@@ -320,7 +328,10 @@ def main():
     acc = tot1["accepted"] - tot0["accepted"]
     rej = tot1["rejected"] - tot0["rejected"]
     rhs = tot1["rhs"] - tot0["rhs"]
-    achieved = BYTES_PER_RHS_FP64 * rhs / (kernel_ms * 1e-3) / 1e9
+    # algorithmic gather bytes per RHS: 8 corners x 6 components x 8 B (FP64)
+    # or x 4 B (FP32 fields)
+    bytes_per_rhs = BYTES_PER_RHS_FP64 if args.prec == "fp64" else BYTES_PER_RHS_FP64 // 2
+    achieved = bytes_per_rhs * rhs / (kernel_ms * 1e-3) / 1e9
     batch.close()
 
     var = None
@@ -349,7 +360,7 @@ def main():
         "rhs_per_accepted": rhs / max(acc, 1), "kernel_ms": kernel_ms,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profile(cfg),
-                     "bytes_per_rhs": BYTES_PER_RHS_FP64, "rhs_per_launch": rhs / args.steps},
+                     "bytes_per_rhs": bytes_per_rhs, "rhs_per_launch": rhs / args.steps},
         "cpu_baseline": cpu,
         "variants": var,
     }

@@ -228,6 +228,14 @@ int ptc_particles_use_mlp(ptc_particles *p, const ptc_mlp *net, double shift);
  *                 last cell it sampled (aos layout) in registers, so stage
  *                 points in the same cell skip the reload; 0 reloads every
  *                 time.  Results are identical either way.
+ *   "sort"        N > 0: before every N-th advance (default 1), re-order
+ *                 the work queue ("order"); 0: index order
+ *   "order"       "cost" (default): most RHS evaluations in the previous
+ *                 segment first, so the cheap particles fill the tail (the
+ *                 first advance has no history and keeps index order);
+ *                 "spatial": Morton order of the particles' cells
+ *                 (neighbouring lanes gather from neighbouring memory).
+ *                 Results are identical for every order.
  * Takes effect for the next create/advance. */
 int ptc_set_option(const char *key, const char *value);
 /* Reads an option back as text into buf (size len). */

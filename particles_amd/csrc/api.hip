@@ -35,6 +35,9 @@ struct ptc_particles {
     double last_ms;
     int ctrl;             // CTRL_*
     void *rec_block;      // record mode: pending features, records, counter
+    void *order_block;    // spatial order: order (np int32) + sort scratch
+    size_t order_bytes;
+    int64_t advances;     // advance calls so far (sort cadence)
 };
 
 struct ptc_mlp {
@@ -82,6 +85,8 @@ struct Options {
     int layout = LAYOUT_AOS;
     int64_t max_steps = 0;
     int cell_cache = 1;
+    int sort = 1;   // re-order the work queue every `sort` advances (0 = never)
+    int order = 2;  // the order then: 1 spatial (Morton), 2 cost (most work first)
 };
 
 std::mutex g_opt_mu;
@@ -132,6 +137,14 @@ int ptc_set_option(const char *key, const char *value)
         if (strcmp(value, "0") && strcmp(value, "1"))
             return fail(PTC_EARG, "cell_cache must be 0 or 1 (got %s)", value);
         g_opt.cell_cache = value[0] - '0';
+    } else if (!strcmp(key, "sort")) {
+        long v = strtol(value, &end, 10);
+        if (*end || v < 0 || v > 1000000) return fail(PTC_EARG, "bad sort cadence %s", value);
+        g_opt.sort = (int)v;
+    } else if (!strcmp(key, "order")) {
+        if (!strcmp(value, "spatial")) g_opt.order = 1;
+        else if (!strcmp(value, "cost")) g_opt.order = 2;
+        else return fail(PTC_EARG, "order must be spatial or cost (got %s)", value);
     } else {
         return fail(PTC_EARG, "unknown option %s", key);
     }
@@ -149,6 +162,8 @@ int ptc_get_option(const char *key, char *buf, int len)
         snprintf(buf, len, "%s", o.layout == LAYOUT_SOA ? "soa" : o.layout == LAYOUT_AOS ? "aos" : "brick");
     else if (!strcmp(key, "max_steps")) snprintf(buf, len, "%lld", (long long)o.max_steps);
     else if (!strcmp(key, "cell_cache")) snprintf(buf, len, "%d", o.cell_cache);
+    else if (!strcmp(key, "sort")) snprintf(buf, len, "%d", o.sort);
+    else if (!strcmp(key, "order")) snprintf(buf, len, "%s", o.order == 2 ? "cost" : "spatial");
     else return fail(PTC_EARG, "unknown option %s", key);
     ok();
     return PTC_OK;
@@ -482,6 +497,31 @@ int ptc_particles_advance(ptc_particles *p, double t_stop)
     if (p->ctrl != CTRL_PI && lc.cell_cache != 1)
         return fail(PTC_ESTATE, "record and learned step-size modes need the aos layout with "
                                 "cell_cache 1");
+    if (o.sort > 0) {
+        // order (np int32), previous RHS counts (np int64), sort scratch
+        const size_t head = sizeof(int32_t) * (size_t)p->d.np + sizeof(int64_t) * (size_t)p->d.np;
+        if (!p->order_block) {
+            p->order_bytes = head + 256 + order_scratch_bytes(p->d.np);
+            HIP_TRY(hipMalloc(&p->order_block, p->order_bytes));
+            HIP_TRY(launch_zero_i64((int64_t *)((int32_t *)p->order_block + p->d.np), p->d.np,
+                                    p->stream));
+        }
+        int32_t *order = (int32_t *)p->order_block;
+        int64_t *prev = (int64_t *)(order + p->d.np);
+        char *scratch = (char *)p->order_block + ((head + 255) & ~(size_t)255);
+        const size_t sbytes = p->order_bytes - (size_t)(scratch - (char *)p->order_block);
+        if (p->advances % o.sort == 0 || !p->d.order) {
+            p->d.order = nullptr;  // the sort reads particles by their own index
+            if (o.order == 2)
+                HIP_TRY(launch_cost_order(p->d, prev, order, scratch, sbytes, p->stream));
+            else
+                HIP_TRY(launch_spatial_order(p->d, p->sp, order, scratch, sbytes, p->stream));
+        }
+        p->d.order = order;
+    } else {
+        p->d.order = nullptr;
+    }
+    p->advances++;
     HIP_TRY(launch_zero_i32(p->d.qcount, 1, p->stream));
     HIP_TRY(hipEventRecord(p->ev0, p->stream));
     HIP_TRY(launch_integrate(p->method, p->fields->d, p->sp, p->d, t_stop, lc, p->stream));
@@ -545,6 +585,7 @@ int ptc_particles_destroy(ptc_particles *p)
     hipError_t e1 = hipFree(p->d_time);
     hipError_t e2 = hipFree(p->block);
     if (p->rec_block) (void)hipFree(p->rec_block);
+    if (p->order_block) (void)hipFree(p->order_block);
     delete p;
     if (e1 != hipSuccess) return hip_fail(e1, "ptc_particles_destroy");
     if (e2 != hipSuccess) return hip_fail(e2, "ptc_particles_destroy");

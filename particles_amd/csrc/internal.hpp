@@ -119,6 +119,16 @@ hipError_t synth_sheet(const int dm[3], double delta, double guide, double *b, h
 hipError_t synth_reconnection_e(const int dm[3], double delta, double vin, double eta,
                                 const double *b, double *e, hipStream_t s);
 
+// spatial work-queue order (order.hip): Morton order of the running
+// particles' cells into `order`; scratch of order_scratch_bytes(np)
+size_t order_scratch_bytes(int64_t np);
+hipError_t launch_spatial_order(const DeviceParticles &S, const StepParams &P, int32_t *order,
+                                void *scratch, size_t scratch_bytes, hipStream_t stream);
+// most RHS evaluations since the last call first; prev (np) holds the
+// counts of the last call and is updated
+hipError_t launch_cost_order(const DeviceParticles &S, int64_t *prev, int32_t *order, void *scratch,
+                             size_t scratch_bytes, hipStream_t stream);
+
 // step-size network (mlp.hip; layouts in mlp_device.hpp)
 constexpr int kMlpMaxGradBlocks = 512;
 hipError_t mlp_pack(const float *params, MlpPacked *packed, hipStream_t stream);

@@ -373,3 +373,28 @@ def test_device_synthesis_matches_host_generator(recipe):
     err = np.abs(a - b).max() / scale
     say(f"[synthesis {recipe}] max relative difference {err:.2e}")
     assert err < 1e-12
+
+
+def test_spatial_order_keeps_results():
+    """Handing particles to lanes in Morton order of their cells or most
+    work first (re-sorted before every segment or every other) changes no
+    snapshot and no counter."""
+    uu, bb = W.turbulent_vb(24, seed=3, nmodes=64)
+    p = W.params_vector(tmax=2.0 * W.T0, dtmax=1.0, atol=1e-8, rtol=1e-8)
+    tg = W.time_grid(4, p[2])
+    st0 = W.initial_state(3000, len(tg), seed=4, gamma_range=(1.0, 100.0))
+    out = []
+    for sort, order in ((0, "spatial"), (1, "spatial"), (2, "spatial"), (1, "cost")):
+        P.set_option("sort", sort)
+        P.set_option("order", order)
+        try:
+            with P.Fields(uu, bb) as f, P.Particles(f, "dop853", p, tg, st0) as b:
+                for k in range(1, 5):
+                    b.advance(k * 0.5 * W.T0)
+                out.append(b.download())
+        finally:
+            P.set_option("sort", 0)
+            P.set_option("order", "spatial")
+    for s, c, status in out[1:]:
+        assert np.array_equal(s, out[0][0]) and np.array_equal(c, out[0][1])
+        assert (status == 0).all()
