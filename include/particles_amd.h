@@ -97,7 +97,9 @@ int ptc_fields_synthesize(const int dm[3], int recipe, const double *scalars, in
  * integrates every particle from t = 0 to tmax = params(3), writes the
  * snapshots into state and the step counts into counter, exactly like
  * integrate_rk5 / integrate_dp853 per particle.  status (np entries) may be
- * NULL.  Synchronous; host buffers stay the caller's. */
+ * NULL.  Synchronous; host buffers stay the caller's.  Internally the run
+ * advances from snapshot time to snapshot time (at most 64 launches) so the
+ * work queue can be ordered by cost; results are those of one launch. */
 int ptc_integrate(const ptc_fields *f, int method, int64_t np, int m, const double *params,
                   int nparams, const double *time, double *state, int64_t *counter,
                   int32_t *status);

@@ -986,7 +986,15 @@ int ptc_integrate(const ptc_fields *f, int method, int64_t np, int m, const doub
     ptc_particles *p = nullptr;
     int rc = ptc_particles_create(f, method, np, m, params, nparams, time, state, &p);
     if (rc) return rc;
-    rc = ptc_particles_advance(p, INFINITY);
+    // advance snapshot interval by snapshot interval (at most 64 segments):
+    // the same result as one launch, and from the second segment on the
+    // work queue is ordered by each particle's cost in the previous one
+    const int nseg = m - 1 < 64 ? m - 1 : 64;
+    for (int k = 1; k < nseg && !rc; k++) {
+        const int n = 1 + (int)((int64_t)k * (m - 1) / nseg);
+        if (time[n - 1] < params[2]) rc = ptc_particles_advance(p, time[n - 1]);
+    }
+    if (!rc) rc = ptc_particles_advance(p, INFINITY);
     if (!rc) rc = ptc_particles_download(p, state, counter, status);
     std::string keep = g_err;
     int rc2 = ptc_particles_destroy(p);

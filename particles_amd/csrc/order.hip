@@ -51,6 +51,8 @@ __global__ void morton_keys_kernel(const DeviceParticles S, const StepParams P,
 }
 
 // key = ~(RHS evaluations since the last call), ascending = most work first
+// (with no history every key is equal and the stable sort keeps index
+// order; a lowest-gamma-first guess for that first launch measured slower)
 __global__ void cost_keys_kernel(const DeviceParticles S, int64_t *__restrict__ prev,
                                  uint64_t *__restrict__ keys, int32_t *__restrict__ idx)
 {
