@@ -108,7 +108,8 @@ int ptc_integrate(const ptc_fields *f, int method, int64_t np, int m, const doub
  * segments, so callers can flush snapshots per interval and benchmark with
  * inputs already in HBM.  create uploads state(8,m,np) (only state(1:6,1,:)
  * is read, the rest is kept where no snapshot is written, as the reference's
- * intent(inout) state); advance integrates every particle until
+ * intent(inout) state; state = NULL starts from zeros, for
+ * ptc_particles_generate); advance integrates every particle until
  * t >= min(t_stop, tmax) (continuing a trajectory across segments gives the
  * same result as one uninterrupted call); download copies state, counter and
  * status (either may be NULL) back to the host. */
@@ -117,6 +118,26 @@ int ptc_particles_create(const ptc_fields *f, int method, int64_t np, int m, con
                          ptc_particles **out);
 int ptc_particles_advance(ptc_particles *p, double t_stop);
 int ptc_particles_download(ptc_particles *p, double *state, int64_t *counter, int32_t *status);
+/* Initial states generated on the device (replaces the host generation of
+ * sources/particles.F90:291-301 for large batches; no bitwise counterpart:
+ * the reference draws from the compiler's random_number).  Philox4x32-10
+ * per particle (counter = particle index and draw, key = seed).  Positions
+ * uniform in [lo, hi]; momenta u = gamma v / c:
+ *   PTC_INIT_MAXWELL   : each component normal(0, a) (the reference's
+ *                        normal(0, vth) start)
+ *   PTC_INIT_GAMMA     : gamma = a, isotropic direction
+ *   PTC_INIT_GAMMA_LOG : gamma log-uniform in [a, b], isotropic direction
+ * Only before the batch's first advance (PTC_ESTATE otherwise); overwrites
+ * state(1:6,1,:) on the device (download returns it). */
+#define PTC_INIT_MAXWELL   0
+#define PTC_INIT_GAMMA     1
+#define PTC_INIT_GAMMA_LOG 2
+int ptc_particles_generate(ptc_particles *p, int kind, uint64_t seed, const double lo[3],
+                           const double hi[3], double a, double b);
+/* Mean, standard deviation, skewness and excess kurtosis of each of the six
+ * components of state(1:6,1,:) (the statistics sources/particles.F90:303-312
+ * prints, random.F90:231-353), reduced on the device: moments[4 c + k]. */
+int ptc_particles_moments(ptc_particles *p, double moments[24]);
 /* totals[0..4] = sum of all / accepted / rejected steps, number of RHS
  * evaluations, number of particles still running; reduced on the device. */
 int ptc_particles_totals(ptc_particles *p, int64_t totals[5]);

@@ -74,6 +74,8 @@ def _load():
         "ptc_probe": ([vp, i32, i64, vp, i32, vp, vp], i32),
         "ptc_set_option": ([C.c_char_p, C.c_char_p], i32),
         "ptc_get_option": ([C.c_char_p, C.c_char_p, i32], i32),
+        "ptc_particles_generate": ([vp, i32, C.c_uint64, vp, vp, dbl, dbl], i32),
+        "ptc_particles_moments": ([vp, vp], i32),
         "ptc_particles_record": ([vp, dbl, i64], i32),
         "ptc_particles_records": ([vp, C.POINTER(i64), vp, C.POINTER(vp)], i32),
         "ptc_particles_use_mlp": ([vp, vp, dbl], i32),
@@ -240,11 +242,27 @@ def integrate(fields: Fields, method: str, params, time, state):
     return state, counter, status
 
 
+INIT_KINDS = {"maxwell": 0, "gamma": 1, "gamma_log": 2}
+
+
 class Particles:
     """Device-resident particle batch advanced in segments; the result of
-    advancing to t1 < t2 < ... < tmax equals one uninterrupted integrate()."""
+    advancing to t1 < t2 < ... < tmax equals one uninterrupted integrate().
+    `state` may be an int (the particle count) to start from zeros and fill
+    the batch with generate()."""
 
     def __init__(self, fields: Fields, method: str, params, time, state):
+        if isinstance(state, (int, np.integer)):
+            params = np.ascontiguousarray(params, dtype=np.float64)
+            time = np.ascontiguousarray(time, dtype=np.float64)
+            self.fields = fields
+            self.m, self.np = time.shape[0], int(state)
+            h = C.c_void_p()
+            _check(_lib.ptc_particles_create(fields.handle, METHODS[method], self.np, self.m,
+                                             _ptr(params), params.size, _ptr(time), None,
+                                             C.byref(h)))
+            self._h = h
+            return
         params, time, state = _prep(params, time, state)
         self.fields = fields  # keeps the field handle alive
         self.m, self.np = state.shape[1], state.shape[2]
@@ -268,6 +286,23 @@ class Particles:
         _check(_lib.ptc_particles_totals(self._h, _ptr(t)))
         return dict(all=int(t[0]), accepted=int(t[1]), rejected=int(t[2]), rhs=int(t[3]),
                     running=int(t[4]))
+
+    def generate(self, kind: str, seed: int, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0),
+                 a: float = 1.0, b: float = 1.0) -> None:
+        """Initial states on the device (ptc_particles_generate): kind
+        "maxwell" (momenta normal(0, a)), "gamma" (gamma = a) or "gamma_log"
+        (gamma log-uniform in [a, b]); positions uniform in [lo, hi]."""
+        lo = np.ascontiguousarray(lo, dtype=np.float64)
+        hi = np.ascontiguousarray(hi, dtype=np.float64)
+        _check(_lib.ptc_particles_generate(self._h, INIT_KINDS[kind], int(seed), _ptr(lo),
+                                           _ptr(hi), float(a), float(b)))
+
+    def moments(self) -> np.ndarray:
+        """(6, 4): mean, standard deviation, skewness, excess kurtosis of each
+        component of state(1:6,1,:)."""
+        m = np.zeros(24, dtype=np.float64)
+        _check(_lib.ptc_particles_moments(self._h, _ptr(m)))
+        return m.reshape(6, 4)
 
     def record(self, rate: float, capacity: int) -> None:
         """Record mode: keep the PI controller and write a training record for

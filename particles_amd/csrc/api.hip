@@ -403,7 +403,7 @@ int ptc_particles_create(const ptc_fields *f, int method, int64_t np, int m, con
     if (method != PTC_RK5 && method != PTC_DOP853) return fail(PTC_EARG, "bad method %d", method);
     if (np < 0 || np > (int64_t)0x7F000000) return fail(PTC_EARG, "bad particle count %lld", (long long)np);
     if (m < 1) return fail(PTC_EARG, "need at least one snapshot (m = %d)", m);
-    if (np > 0 && (!time || !state)) return fail(PTC_EARG, "null time/state");
+    if (np > 0 && !time) return fail(PTC_EARG, "null time");
     StepParams sp;
     int rc = build_step_params(f, m, params, nparams, sp);
     if (rc) return rc;
@@ -455,17 +455,21 @@ int ptc_particles_create(const ptc_fields *f, int method, int64_t np, int m, con
 
     // initial state: SoA copy of state(1:6,1,:), full state(8,m,np) image
     if (np > 0) {
-        std::vector<double> h(9 * (size_t)np);
-        for (int64_t i = 0; i < np; i++) {
+        std::vector<double> h(9 * (size_t)np, 0.0);
+        for (int64_t i = 0; i < np && state; i++) {
             const double *s = state + (size_t)8 * m * i;
             for (int k = 0; k < 6; k++) h[k * np + i] = s[k];
+        }
+        for (int64_t i = 0; i < np; i++) {
             h[6 * np + i] = 0.0;        // t
             h[7 * np + i] = 0.0;        // dt
             h[8 * np + i] = 1.0e-04;    // errold (integrations.F90:349)
         }
         e = hipMemcpy(dx, h.data(), sizeof(double) * 9 * np, hipMemcpyHostToDevice);
-        if (e == hipSuccess)
+        if (e == hipSuccess && state)
             e = hipMemcpy(p->d.out, state, sizeof(double) * 8 * m * np, hipMemcpyHostToDevice);
+        if (e == hipSuccess && !state)
+            e = hipMemset(p->d.out, 0, sizeof(double) * 8 * m * np);
         std::vector<int32_t> hi(3 * (size_t)np);
         for (int64_t i = 0; i < np; i++) {
             hi[i] = 2;                            // next snapshot (1-based)
@@ -551,6 +555,71 @@ int ptc_particles_download(ptc_particles *p, double *state, int64_t *counter, in
     }
     if (status)
         HIP_TRY(hipMemcpy(status, p->d.status, sizeof(int32_t) * np, hipMemcpyDeviceToHost));
+    ok();
+    return PTC_OK;
+}
+
+int ptc_particles_generate(ptc_particles *p, int kind, uint64_t seed, const double lo[3],
+                           const double hi[3], double a, double b)
+{
+    if (!p || !lo || !hi) return fail(PTC_EARG, "ptc_particles_generate: null argument");
+    if (p->advances > 0)
+        return fail(PTC_ESTATE, "ptc_particles_generate: the batch has already been advanced");
+    if (kind != PTC_INIT_MAXWELL && kind != PTC_INIT_GAMMA && kind != PTC_INIT_GAMMA_LOG)
+        return fail(PTC_EARG, "ptc_particles_generate: unknown kind %d", kind);
+    for (int i = 0; i < 3; i++)
+        if (!std::isfinite(lo[i]) || !std::isfinite(hi[i]))
+            return fail(PTC_EARG, "ptc_particles_generate: region not finite");
+    if (!std::isfinite(a) || (kind == PTC_INIT_MAXWELL && a < 0.0) ||
+        (kind != PTC_INIT_MAXWELL && a < 1.0) ||
+        (kind == PTC_INIT_GAMMA_LOG && !(b >= a && std::isfinite(b))))
+        return fail(PTC_EARG, "ptc_particles_generate: bad distribution parameters %g %g", a, b);
+    if (p->d.np == 0) { ok(); return PTC_OK; }
+    HIP_TRY(launch_init(p->d, p->sp.m, kind, seed, lo, hi, a, b, p->stream));
+    HIP_TRY(hipStreamSynchronize(p->stream));
+    ok();
+    return PTC_OK;
+}
+
+int ptc_particles_moments(ptc_particles *p, double moments[24])
+{
+    if (!p || !moments) return fail(PTC_EARG, "ptc_particles_moments: null argument");
+    for (int j = 0; j < 24; j++) moments[j] = 0.0;
+    const int64_t n = p->d.np;
+    if (n == 0) { ok(); return PTC_OK; }
+    int blocks = (int)((n + 255) / 256 < 1024 ? (n + 255) / 256 : 1024);
+    double *d = nullptr;
+    HIP_TRY(hipMalloc(&d, sizeof(double) * (6 + 24 * (size_t)blocks)));
+    std::vector<double> h(24 * (size_t)blocks);
+    double shift[6] = {0, 0, 0, 0, 0, 0}, mean[6];
+    hipError_t e = hipSuccess;
+    // pass 1: means; pass 2: central moments about them
+    for (int pass = 0; pass < 2 && e == hipSuccess; pass++) {
+        e = hipMemcpy(d, shift, sizeof shift, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = launch_moments(p->d, p->sp.m, d, d + 6, blocks, p->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(p->stream);
+        if (e == hipSuccess)
+            e = hipMemcpy(h.data(), d + 6, sizeof(double) * h.size(), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) break;
+        double sum[24] = {0};
+        for (int b = 0; b < blocks; b++)
+            for (int j = 0; j < 24; j++) sum[j] += h[(size_t)b * 24 + j];
+        for (int c = 0; c < 6; c++) {
+            if (pass == 0) {
+                mean[c] = sum[4 * c] / (double)n;
+                shift[c] = mean[c];
+            } else {
+                const double m2 = sum[4 * c + 1] / (double)n;
+                const double sd = std::sqrt(m2);
+                moments[4 * c] = mean[c];
+                moments[4 * c + 1] = sd;
+                moments[4 * c + 2] = sd > 0.0 ? sum[4 * c + 2] / (double)n / (sd * sd * sd) : 0.0;
+                moments[4 * c + 3] = sd > 0.0 ? sum[4 * c + 3] / (double)n / (m2 * m2) - 3.0 : -3.0;
+            }
+        }
+    }
+    (void)hipFree(d);
+    if (e != hipSuccess) return hip_fail(e, "ptc_particles_moments");
     ok();
     return PTC_OK;
 }

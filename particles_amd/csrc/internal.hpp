@@ -129,6 +129,12 @@ hipError_t launch_spatial_order(const DeviceParticles &S, const StepParams &P, i
 hipError_t launch_cost_order(const DeviceParticles &S, int64_t *prev, int32_t *order, void *scratch,
                              size_t scratch_bytes, hipStream_t stream);
 
+// initial states and their moments (init.hip)
+hipError_t launch_init(const DeviceParticles &S, int m, int kind, uint64_t seed, const double lo[3],
+                       const double hi[3], double a, double b, hipStream_t stream);
+hipError_t launch_moments(const DeviceParticles &S, int m, const double *shift, double *partials,
+                          int blocks, hipStream_t stream);
+
 // step-size network (mlp.hip; layouts in mlp_device.hpp)
 constexpr int kMlpMaxGradBlocks = 512;
 hipError_t mlp_pack(const float *params, MlpPacked *packed, hipStream_t stream);
