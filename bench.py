@@ -273,6 +273,8 @@ def main():
     ap.add_argument("--grid-blocks", type=int, default=0)
     ap.add_argument("--sort", type=int, default=1,
                     help="re-order the work queue every N steps (0 = never)")
+    ap.add_argument("--autotune", action="store_true",
+                    help="tune block size and queue order during the warm-up steps")
     ap.add_argument("--order", default="cost", choices=["spatial", "cost"],
                     help="cost: most work in the previous step first; spatial: Morton order")
     ap.add_argument("--prec", default="fp64", choices=["fp64", "fp32"],
@@ -311,8 +313,13 @@ def main():
 
     dt_step = W.T0
     t = 0.0
+    tuned = None
     for i in range(args.warmup):
         t += dt_step
+        if args.autotune and i == args.warmup - 1:
+            tuned = batch.autotune(t)
+            print(f"warmup {i + 1}/{args.warmup}: autotuned {tuned}", file=sys.stderr, flush=True)
+            continue
         ms = batch.advance(t)
         print(f"warmup {i + 1}/{args.warmup}: kernel {ms:.1f} ms", file=sys.stderr, flush=True)
     tot0 = batch.totals()
@@ -363,6 +370,7 @@ def main():
                      "bytes_per_rhs": bytes_per_rhs, "rhs_per_launch": rhs / args.steps},
         "cpu_baseline": cpu,
         "variants": var,
+        "autotuned": tuned,
     }
     if args.prec == "fp32":
         out["dtype"] = "f64 state, f32 fields and RHS"

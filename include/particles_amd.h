@@ -138,6 +138,20 @@ int ptc_particles_generate(ptc_particles *p, int kind, uint64_t seed, const doub
  * components of state(1:6,1,:) (the statistics sources/particles.F90:303-312
  * prints, random.F90:231-353), reduced on the device: moments[4 c + k]. */
 int ptc_particles_moments(ptc_particles *p, double moments[24]);
+/* n advances in one HIP graph: the launches for the stop times t_stop[0..n-1]
+ * (typically the snapshot times) are captured once and submitted together;
+ * the same result as n calls of ptc_particles_advance.  The work-queue order
+ * is computed once, before the graph, and kept for all n intervals.
+ * ptc_particles_last_kernel_ms then reports the whole graph. */
+int ptc_particles_advance_graph(ptc_particles *p, int n, const double *t_stop);
+/* Autotuner: integrates the batch up to t_end in 1 + 6 equal segments, the
+ * first to warm up, then one per candidate launch setting (threads per
+ * block 256 / 128 / 64, work-queue order by cost / spatial / none), and
+ * keeps for this batch the one with the most RHS evaluations per kernel
+ * millisecond.  The segments are real progress: every setting gives the
+ * same results.  chosen (may be NULL) = block, sort cadence, order (1
+ * spatial, 2 cost). */
+int ptc_particles_autotune(ptc_particles *p, double t_end, int chosen[3]);
 /* totals[0..4] = sum of all / accepted / rejected steps, number of RHS
  * evaluations, number of particles still running; reduced on the device. */
 int ptc_particles_totals(ptc_particles *p, int64_t totals[5]);

@@ -76,6 +76,8 @@ def _load():
         "ptc_get_option": ([C.c_char_p, C.c_char_p, i32], i32),
         "ptc_particles_generate": ([vp, i32, C.c_uint64, vp, vp, dbl, dbl], i32),
         "ptc_particles_moments": ([vp, vp], i32),
+        "ptc_particles_autotune": ([vp, dbl, vp], i32),
+        "ptc_particles_advance_graph": ([vp, i32, vp], i32),
         "ptc_particles_record": ([vp, dbl, i64], i32),
         "ptc_particles_records": ([vp, C.POINTER(i64), vp, C.POINTER(vp)], i32),
         "ptc_particles_use_mlp": ([vp, vp, dbl], i32),
@@ -296,6 +298,23 @@ class Particles:
         hi = np.ascontiguousarray(hi, dtype=np.float64)
         _check(_lib.ptc_particles_generate(self._h, INIT_KINDS[kind], int(seed), _ptr(lo),
                                            _ptr(hi), float(a), float(b)))
+
+    def advance_graph(self, t_stops) -> float:
+        """Advances through every stop time in one captured HIP graph;
+        returns the graph's duration in ms."""
+        t = np.ascontiguousarray(t_stops, dtype=np.float64)
+        _check(_lib.ptc_particles_advance_graph(self._h, t.size, _ptr(t)))
+        ms = C.c_double()
+        _check(_lib.ptc_particles_last_kernel_ms(self._h, C.byref(ms)))
+        return ms.value
+
+    def autotune(self, t_end: float) -> dict:
+        """Integrates to t_end while timing candidate launch settings, keeps
+        the fastest for this batch (ptc_particles_autotune)."""
+        c = np.zeros(3, dtype=np.int32)
+        _check(_lib.ptc_particles_autotune(self._h, float(t_end), _ptr(c)))
+        return {"block": int(c[0]), "sort": int(c[1]),
+                "order": {1: "spatial", 2: "cost"}.get(int(c[2]), "index")}
 
     def moments(self) -> np.ndarray:
         """(6, 4): mean, standard deviation, skewness, excess kurtosis of each

@@ -38,6 +38,8 @@ struct ptc_particles {
     void *order_block;    // spatial order: order (np int32) + sort scratch
     size_t order_bytes;
     int64_t advances;     // advance calls so far (sort cadence)
+    int tuned;            // launch settings below chosen by ptc_particles_autotune
+    int tuned_block, tuned_sort, tuned_order;
 };
 
 struct ptc_mlp {
@@ -494,7 +496,12 @@ int ptc_particles_advance(ptc_particles *p, double t_stop)
     if (std::isnan(t_stop)) return fail(PTC_EARG, "ptc_particles_advance: t_stop is NaN");
     p->last_ms = 0.0;
     if (p->d.np == 0) { ok(); return PTC_OK; }
-    const Options o = options();
+    Options o = options();
+    if (p->tuned) {
+        o.block = p->tuned_block;
+        o.sort = p->tuned_sort;
+        o.order = p->tuned_order;
+    }
     p->sp.max_steps = o.max_steps;
     LaunchConfig lc{o.block, o.grid_blocks,
                     p->fields->d.layout == LAYOUT_AOS ? o.cell_cache : 0, p->ctrl};
@@ -620,6 +627,136 @@ int ptc_particles_moments(ptc_particles *p, double moments[24])
     }
     (void)hipFree(d);
     if (e != hipSuccess) return hip_fail(e, "ptc_particles_moments");
+    ok();
+    return PTC_OK;
+}
+
+int ptc_particles_advance_graph(ptc_particles *p, int n, const double *t_stop)
+{
+    if (!p || (n > 0 && !t_stop)) return fail(PTC_EARG, "ptc_particles_advance_graph: null argument");
+    if (n < 0 || n > 100000) return fail(PTC_EARG, "ptc_particles_advance_graph: bad count %d", n);
+    for (int k = 0; k < n; k++)
+        if (std::isnan(t_stop[k])) return fail(PTC_EARG, "ptc_particles_advance_graph: NaN t_stop");
+    p->last_ms = 0.0;
+    if (p->d.np == 0 || n == 0) { ok(); return PTC_OK; }
+    Options o = options();
+    if (p->tuned) {
+        o.block = p->tuned_block;
+        o.sort = p->tuned_sort;
+        o.order = p->tuned_order;
+    }
+    p->sp.max_steps = o.max_steps;
+    LaunchConfig lc{o.block, o.grid_blocks,
+                    p->fields->d.layout == LAYOUT_AOS ? o.cell_cache : 0, p->ctrl};
+    if (p->ctrl != CTRL_PI && lc.cell_cache != 1)
+        return fail(PTC_ESTATE, "record and learned step-size modes need the aos layout with "
+                                "cell_cache 1");
+    // the queue order is set once, before capture (the library sort stays
+    // out of the graph); every interval of the graph uses it
+    if (o.sort > 0) {
+        const size_t head = sizeof(int32_t) * (size_t)p->d.np + sizeof(int64_t) * (size_t)p->d.np;
+        if (!p->order_block) {
+            p->order_bytes = head + 256 + order_scratch_bytes(p->d.np);
+            HIP_TRY(hipMalloc(&p->order_block, p->order_bytes));
+            HIP_TRY(launch_zero_i64((int64_t *)((int32_t *)p->order_block + p->d.np), p->d.np,
+                                    p->stream));
+        }
+        int32_t *order = (int32_t *)p->order_block;
+        int64_t *prev = (int64_t *)(order + p->d.np);
+        char *scratch = (char *)p->order_block + ((head + 255) & ~(size_t)255);
+        const size_t sbytes = p->order_bytes - (size_t)(scratch - (char *)p->order_block);
+        p->d.order = nullptr;
+        if (o.order == 2)
+            HIP_TRY(launch_cost_order(p->d, prev, order, scratch, sbytes, p->stream));
+        else
+            HIP_TRY(launch_spatial_order(p->d, p->sp, order, scratch, sbytes, p->stream));
+        p->d.order = order;
+    } else {
+        p->d.order = nullptr;
+    }
+    HIP_TRY(hipStreamSynchronize(p->stream));
+    // capture: per interval, reset the queue head (a fill kernel) and run the
+    // integrator to that interval's stop time
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    HIP_TRY(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal));
+    hipError_t e = hipSuccess;
+    for (int k = 0; k < n && e == hipSuccess; k++) {
+        e = launch_zero_i32(p->d.qcount, 1, p->stream);
+        if (e == hipSuccess) e = launch_integrate(p->method, p->fields->d, p->sp, p->d, t_stop[k],
+                                                  lc, p->stream);
+    }
+    hipError_t e2 = hipStreamEndCapture(p->stream, &graph);
+    if (e == hipSuccess) e = e2;
+    if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (e == hipSuccess) e = hipEventRecord(p->ev0, p->stream);
+    if (e == hipSuccess) e = hipGraphLaunch(exec, p->stream);
+    if (e == hipSuccess) e = hipEventRecord(p->ev1, p->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(p->stream);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, p->ev0, p->ev1);
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) return hip_fail(e, "ptc_particles_advance_graph");
+    p->advances += n;
+    p->last_ms = ms;
+    ok();
+    return PTC_OK;
+}
+
+int ptc_particles_autotune(ptc_particles *p, double t_end, int chosen[3])
+{
+    if (!p) return fail(PTC_EARG, "ptc_particles_autotune: null handle");
+    if (!std::isfinite(t_end)) return fail(PTC_EARG, "ptc_particles_autotune: t_end not finite");
+    if (p->d.np == 0) { ok(); return PTC_OK; }
+    // the batch's current time: the earliest particle time (all start at 0
+    // unless the batch was advanced)
+    std::vector<double> ht((size_t)p->d.np);
+    HIP_TRY(hipStreamSynchronize(p->stream));
+    HIP_TRY(hipMemcpy(ht.data(), p->d.t, sizeof(double) * ht.size(), hipMemcpyDeviceToHost));
+    double t0 = ht[0];
+    for (double v : ht) t0 = v < t0 ? v : t0;
+    struct Cand { int block, sort, order; };
+    const Cand cand[] = {{256, 1, 2}, {128, 1, 2}, {64, 1, 2}, {256, 1, 1}, {256, 0, 2}, {128, 0, 2}};
+    const int nc = (int)(sizeof cand / sizeof cand[0]);
+    if (!(t_end > t0)) return fail(PTC_EARG, "ptc_particles_autotune: t_end must exceed the "
+                                             "batch's time %g", t0);
+    const int saved = p->tuned;
+    const int sb = p->tuned_block, ss = p->tuned_sort, so = p->tuned_order;
+    double best = -1.0;
+    int pick = 0;
+    int64_t tot[5];
+    for (int k = 0; k <= nc; k++) {
+        // segment 0 warms up (and gives the cost order its history)
+        const Cand c = cand[k == 0 ? 0 : k - 1];
+        p->tuned = 1;
+        p->tuned_block = c.block;
+        p->tuned_sort = c.sort;
+        p->tuned_order = c.order;
+        int rc = ptc_particles_totals(p, tot);
+        const int64_t rhs0 = tot[3];
+        if (!rc) rc = ptc_particles_advance(p, t0 + (t_end - t0) * (k + 1) / (nc + 1));
+        if (!rc) rc = ptc_particles_totals(p, tot);
+        if (rc) {
+            p->tuned = saved;
+            p->tuned_block = sb; p->tuned_sort = ss; p->tuned_order = so;
+            return rc;
+        }
+        const double rate = p->last_ms > 0.0 ? (double)(tot[3] - rhs0) / p->last_ms : 0.0;
+        if (k > 0 && rate > best) {
+            best = rate;
+            pick = k - 1;
+        }
+    }
+    p->tuned = 1;
+    p->tuned_block = cand[pick].block;
+    p->tuned_sort = cand[pick].sort;
+    p->tuned_order = cand[pick].order;
+    if (chosen) {
+        chosen[0] = p->tuned_block;
+        chosen[1] = p->tuned_sort;
+        chosen[2] = p->tuned_order;
+    }
     ok();
     return PTC_OK;
 }

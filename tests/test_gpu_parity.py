@@ -398,3 +398,44 @@ def test_spatial_order_keeps_results():
     for s, c, status in out[1:]:
         assert np.array_equal(s, out[0][0]) and np.array_equal(c, out[0][1])
         assert (status == 0).all()
+
+
+def test_autotune_keeps_results():
+    """The autotuner's probe segments are real integration: the batch ends
+    exactly where an untuned one does."""
+    uu, bb = W.turbulent_vb(16, seed=2, nmodes=32)
+    p = W.params_vector(tmax=1.0 * W.T0, dtmax=1.0, atol=1e-8, rtol=1e-8)
+    tg = W.time_grid(3, p[2])
+    st0 = W.initial_state(2000, len(tg), seed=6, gamma_range=(1.0, 100.0))
+    with P.Fields(uu, bb) as f:
+        s_ref, c_ref, _ = P.integrate(f, "rk5", p, tg, st0)
+        with P.Particles(f, "rk5", p, tg, st0) as b:
+            chosen = b.autotune(0.5 * W.T0)
+            b.advance()
+            s, c, status = b.download()
+    say(f"[autotune] chose {chosen}")
+    assert chosen["block"] in (64, 128, 256)
+    assert np.array_equal(s, s_ref) and np.array_equal(c, c_ref) and (status == 0).all()
+
+
+def test_graph_advances_equal_eager_advances():
+    """Snapshot intervals submitted as one captured HIP graph give the same
+    snapshots and counters as eager advances; two graphs in a row, with
+    eager work and synchronisation between them."""
+    uu, bb = W.turbulent_vb(16, seed=2, nmodes=32)
+    p = W.params_vector(tmax=2.0 * W.T0, dtmax=1.0, atol=1e-8, rtol=1e-8)
+    tg = W.time_grid(8, p[2])
+    st0 = W.initial_state(3000, len(tg), seed=7, gamma_range=(1.0, 100.0))
+    with P.Fields(uu, bb) as f:
+        with P.Particles(f, "dop853", p, tg, st0) as b:
+            for t in tg[1:]:
+                b.advance(t)
+            ref = b.download()
+        with P.Particles(f, "dop853", p, tg, st0) as b:
+            b.advance_graph(tg[1:4])
+            mid = b.totals()
+            assert mid["accepted"] > 0
+            b.advance_graph(tg[4:])
+            got = b.download()
+    assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1])
+    assert (got[2] == 0).all()
