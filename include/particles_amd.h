@@ -93,6 +93,25 @@ int ptc_fields_synthesize(const int dm[3], int recipe, const double *scalars, in
                           const double *b_phase, int nv, const int32_t *v_n, const double *v_amp,
                           const double *v_pol, const double *v_phase, int prec, ptc_fields **out);
 
+/* Time-dependent fields (no counterpart in the reference, whose field is
+ * static: sources/fields.F90:51-78 loads one snapshot and the RHS has no
+ * time argument, sources/accelerations.F90:59).  Builds a series from n
+ * snapshot handles of equal dimensions, mode and precision in the aos layout
+ * (ptc_fields_create or ptc_fields_synthesize), at strictly increasing
+ * finite times in the integrator's time unit (that of params(3)).  Their
+ * device arrays are copied into one block, so the snapshot handles may be
+ * destroyed afterwards; destroy the series with ptc_fields_destroy.
+ * A stage at time t sees f = f_k + a (f_{k+1} - f_k), the trilinear
+ * interpolants of snapshots k and k+1 blended with a = (t - times[k]) /
+ * (times[k+1] - times[k]), times[k] <= t < times[k+1]; at or before
+ * times[0] snapshot 0 alone, at or after times[n-1] snapshot n-1 alone.
+ * Stage i of a step from t of size dt is at t + c_i dt, c_i the method's
+ * published nodes (Cash-Karp 1990; Hairer-Norsett-Wanner dop853).  A
+ * series integrates with the PI step-size controller and without the cell
+ * cache; n = 1 is a static field taken through the series kernels. */
+int ptc_fields_series(const ptc_fields *const *snaps, const double *times, int n,
+                      ptc_fields **out);
+
 /* The whole particle loop of sources/particles.F90:337-356 in one call:
  * integrates every particle from t = 0 to tmax = params(3), writes the
  * snapshots into state and the step counts into counter, exactly like
@@ -180,6 +199,13 @@ int ptc_particles_destroy(ptc_particles *p);
  *                        at vec after an accepted step h with error ratio
  *                        err (no counterpart in the reference; aos layout) */
 #define PTC_PROBE_FEATURES 5
+/*   PTC_PROBE_GATHER_T : in (pos(3), t)(4,n) -> out (6,n) = (U or E, B) at
+ *                        time t in a snapshot series (ptc_fields_series)
+ *   PTC_PROBE_RHS_T    : in (vec(6), t)(7,n) -> out (6,n) = d(vec)/dt at t
+ *                        (aos layout; a static handle is its own snapshot 0;
+ *                        the other ops read snapshot 0 of a series) */
+#define PTC_PROBE_GATHER_T 6
+#define PTC_PROBE_RHS_T    7
 int ptc_probe(const ptc_fields *f, int op, int64_t n, const double *params, int nparams,
               const double *in, double *out);
 

@@ -55,7 +55,48 @@ class Oracle:
         lib.orc_integrate.argtypes = [C.c_int, C.c_int, C.c_int64, C.c_int, _ip, _dp, _dp, _dp, _dp,
                                       _dp, _lp, C.c_void_p, C.c_int64, C.c_int]
         lib.orc_max_threads.restype = C.c_int
+        lib.orc_interpolate_series.argtypes = [_ip, _dp, _dp, C.c_int, _dp, _dp, _dp, _dp,
+                                               C.c_double, _dp, _dp]
+        lib.orc_acceleration_series.argtypes = [C.c_int, C.c_double, C.c_double, _ip, _dp, _dp,
+                                                C.c_int, _dp, _dp, _dp, _dp, C.c_double, _dp]
+        lib.orc_integrate_series.argtypes = [C.c_int, C.c_int, C.c_int64, C.c_int, _ip, _dp, _dp,
+                                             C.c_int, _dp, _dp, _dp, _dp, _lp, C.c_void_p,
+                                             C.c_int64, C.c_int]
+        lib.orc_stage_nodes.argtypes = [C.c_int, _dp, _dp]
+        lib.orc_stage_nodes.restype = C.c_int
         self.lib = lib
+
+    # ---- time series: uu / bb are (dm1, dm2, dm3, 3, ns), times (ns,) ----
+    def interpolate_series(self, uu, bb, times, pos, t, xl=(0, 0, 0), xs=(1, 1, 1)):
+        u = np.zeros(3, order="F"); b = np.zeros(3, order="F")
+        self.lib.orc_interpolate_series(_dm(uu), _f64(xl), _f64(xs), uu.shape[4], _f64(times),
+                                        _f64(uu), _f64(bb), _f64(pos), t, u, b)
+        return u, b
+
+    def acceleration_series(self, uu, bb, times, vec, t, qom=1.0, vun=1.0, xl=(0, 0, 0),
+                            xs=(1, 1, 1), mode="vb"):
+        acc = np.zeros(6, order="F")
+        self.lib.orc_acceleration_series(MODES[mode], qom, vun, _dm(uu), _f64(xl), _f64(xs),
+                                         uu.shape[4], _f64(times), _f64(uu), _f64(bb), _f64(vec),
+                                         t, acc)
+        return acc
+
+    def integrate_series(self, method, uu, bb, times, params, time, state, mode="vb",
+                         max_steps=0, nthreads=0):
+        state = _f64(state).copy(order="F")
+        m, np_ = state.shape[1], state.shape[2]
+        cnt = np.zeros((3, np_), dtype=np.int64, order="F")
+        status = np.zeros(np_, dtype=np.int32)
+        self.lib.orc_integrate_series(METHODS[method], MODES[mode], np_, m, _dm(uu), _f64(params),
+                                      _f64(time), uu.shape[4], _f64(times), _f64(uu), _f64(bb),
+                                      state, cnt, status.ctypes.data, max_steps, nthreads)
+        return state, cnt, status
+
+    def stage_nodes(self, method):
+        """(c, row sums of the stage matrix) as the oracle holds them."""
+        c = np.zeros(12, order="F"); r = np.zeros(12, order="F")
+        k = self.lib.orc_stage_nodes(METHODS[method], c, r)
+        return c[:k].copy(), r[:k].copy()
 
     def interpolate_lin(self, uu, bb, pos, xl=(0, 0, 0), xs=(1, 1, 1)):
         u = np.zeros(3, order="F"); b = np.zeros(3, order="F")

@@ -89,15 +89,12 @@ void orc_interpolate_lin(const int dm[3], const double xl[3], const double xs[3]
     }
 }
 
-/* sources/accelerations.F90:59-96 (VB mode); EB mode replaces (v-U) x B by
- * E + v x B with E taken from the first field array. */
-void orc_acceleration(int mode, double qom, double vun, const int dm[3],
-                      const double xl[3], const double xs[3], const double *uu,
-                      const double *bb, const double vec[6], double acc[6])
+/* the force of sources/accelerations.F90:84-92 from interpolated fields u
+ * (U, or E in EB mode) and b at vec */
+static void lorentz(int mode, double qom, double vun, const double u[3], const double b[3],
+                    const double vec[6], double acc[6])
 {
-    double u[3], b[3], v[3], w[3], g;
-
-    orc_interpolate_lin(dm, xl, xs, uu, bb, vec, u, b);                    /* :82 */
+    double v[3], w[3], g;
 
     g = sqrt(vec[3] * vec[3] + vec[4] * vec[4] + vec[5] * vec[5] + 1.0);  /* :84 */
     v[0] = vec[3] / g;                                                     /* :85 */
@@ -121,17 +118,116 @@ void orc_acceleration(int mode, double qom, double vun, const int dm[3],
     }
 }
 
+/* sources/accelerations.F90:59-96 (VB mode); EB mode replaces (v-U) x B by
+ * E + v x B with E taken from the first field array. */
+void orc_acceleration(int mode, double qom, double vun, const int dm[3],
+                      const double xl[3], const double xs[3], const double *uu,
+                      const double *bb, const double vec[6], double acc[6])
+{
+    double u[3], b[3];
+
+    orc_interpolate_lin(dm, xl, xs, uu, bb, vec, u, b);                    /* :82 */
+    lorentz(mode, qom, vun, u, b, vec, acc);
+}
+
+/* Snapshot interval of time t in a series of ns snapshots at strictly
+ * increasing times st (no counterpart in the reference, whose field is
+ * static): *k with st[k] <= t < st[k+1] and *a = (t - st[k]) / (st[k+1] -
+ * st[k]), *two = 1; at or before st[0] (or t NaN) snapshot 0 alone, at or
+ * after st[ns-1] snapshot ns-1 alone (*two = 0).  Bisection. */
+static void series_interval(int ns, const double *st, double t, int *k, double *a, int *two)
+{
+    int lo = 0, hi = ns - 1;
+    *a = 0.0;
+    *two = 0;
+    if (ns <= 1 || !(t > st[0])) { *k = 0; return; }
+    if (t >= st[ns - 1]) { *k = ns - 1; return; }
+    while (hi - lo > 1) {           /* st[lo] <= t < st[hi] */
+        const int mid = lo + (hi - lo) / 2;
+        if (st[mid] <= t) lo = mid;
+        else hi = mid;
+    }
+    *k = lo;
+    *a = (t - st[lo]) / (st[lo + 1] - st[lo]);
+    *two = 1;
+}
+
+void orc_interpolate_series(const int dm[3], const double xl[3], const double xs[3], int ns,
+                            const double *st, const double *uu, const double *bb,
+                            const double pos[3], double t, double u[3], double b[3])
+{
+    const int64_t n3 = (int64_t)3 * dm[0] * dm[1] * dm[2];
+    int k, two;
+    double a, u2[3], b2[3];
+
+    series_interval(ns, st, t, &k, &a, &two);
+    orc_interpolate_lin(dm, xl, xs, uu + k * n3, bb + k * n3, pos, u, b);
+    if (two) {
+        orc_interpolate_lin(dm, xl, xs, uu + (k + 1) * n3, bb + (k + 1) * n3, pos, u2, b2);
+        for (int i = 0; i < 3; i++) {
+            u[i] = u[i] + a * (u2[i] - u[i]);
+            b[i] = b[i] + a * (b2[i] - b[i]);
+        }
+    }
+}
+
+void orc_acceleration_series(int mode, double qom, double vun, const int dm[3],
+                             const double xl[3], const double xs[3], int ns, const double *st,
+                             const double *uu, const double *bb, const double vec[6], double t,
+                             double acc[6])
+{
+    double u[3], b[3];
+
+    orc_interpolate_series(dm, xl, xs, ns, st, uu, bb, vec, t, u, b);
+    lorentz(mode, qom, vun, u, b, vec, acc);
+}
+
+/* The field one integration runs in: a static snapshot (ns = 0, the
+ * reference's case) or a series of ns snapshots at times st, uu / bb then
+ * holding ns consecutive (dm1, dm2, dm3, 3) arrays. */
+typedef struct {
+    int mode, ns;
+    const int *dm;
+    const double *xl, *xs, *st, *uu, *bb;
+    double qom, vun;
+} fsrc;
+
+static fsrc fsrc_make(int mode, const int dm[3], const double *params, int ns, const double *st,
+                      const double *uu, const double *bb)
+{
+    fsrc F;
+    F.mode = mode;
+    F.ns = ns;
+    F.dm = dm;
+    F.xl = params + 11;
+    F.xs = params + 14;
+    F.st = st;
+    F.uu = uu;
+    F.bb = bb;
+    F.qom = params[0];
+    F.vun = params[1];
+    return F;
+}
+
+/* the RHS at vec and time t (t is unused for a static field) */
+static void fsrc_rhs(const fsrc *F, double t, const double vec[6], double acc[6])
+{
+    if (F->ns == 0)
+        orc_acceleration(F->mode, F->qom, F->vun, F->dm, F->xl, F->xs, F->uu, F->bb, vec, acc);
+    else
+        orc_acceleration_series(F->mode, F->qom, F->vun, F->dm, F->xl, F->xs, F->ns, F->st,
+                                F->uu, F->bb, vec, t, acc);
+}
+
 /* sources/integrations.F90:524-589.  The reference also evaluates the RHS at
  * sf (:576) but never uses it; that evaluation is omitted (no side effects). */
-double orc_dti_guess(int iord, int mode, const int dm[3], const double *params,
-                     const double *uu, const double *bb, const double si[6])
+static double dti_guess_f(const fsrc *F, int iord, const double *params, const double si[6])
 {
-    const double qom = params[0], vun = params[1], dtm = params[4];
+    const double dtm = params[4];
     const double atol = params[5], rtol = params[6];
-    const double *xl = params + 11, *xs = params + 14;
     double ki[6], sr[6], sf[6], d0, d1, d2, dti, dtf, s, q;
 
-    orc_acceleration(mode, qom, vun, dm, xl, xs, uu, bb, si, ki);         /* :563 */
+    fsrc_rhs(F, 0.0, si, ki);                                              /* :563 */
     for (int i = 0; i < 6; i++) sr[i] = atol + rtol * fabs(si[i]);         /* :565 */
 
     s = 0.0;
@@ -160,6 +256,13 @@ double orc_dti_guess(int iord, int mode, const int dm[3], const double *params,
     return dti;
 }
 
+double orc_dti_guess(int iord, int mode, const int dm[3], const double *params,
+                     const double *uu, const double *bb, const double si[6])
+{
+    const fsrc F = fsrc_make(mode, dm, params, 0, NULL, uu, bb);
+    return dti_guess_f(&F, iord, params, si);
+}
+
 /* Cash-Karp 1990 tableau (Numerical Recipes 2nd ed. sec. 16.2): stage
  * matrix, 5th-order weights and 5th-minus-4th weights.  The reference holds
  * the same rationals over common denominators (integrations.F90:86-98);
@@ -175,6 +278,8 @@ static const double CK_C[6] = {37.0 / 378.0, 0.0, 250.0 / 621.0, 125.0 / 594.0, 
                                512.0 / 1771.0};
 static const double CK_D[6] = {-277.0 / 64512.0, 0.0, 6925.0 / 370944.0, -6925.0 / 202752.0,
                                -277.0 / 14336.0, 277.0 / 7084.0};
+/* stage nodes (row sums of CK_A): stage j at t + CK_N[j] dt, for a series */
+static const double CK_N[6] = {0.0, 1.0 / 5.0, 3.0 / 10.0, 3.0 / 5.0, 1.0, 7.0 / 8.0};
 
 static inline void hermite_out(double *st, double t, double tn, double dt, const double si[6],
                                const double sf[6], const double ka[6], const double kb[6])
@@ -198,21 +303,20 @@ static inline int all_finite6(const double *a)
     return 1;
 }
 
-/* sources/integrations.F90:57-200 */
-int orc_integrate_rk5(int mode, int m, const int dm[3], const double *params,
-                      const double *time, const double *uu, const double *bb,
-                      double *state, int64_t cnt[3], int64_t max_steps)
+/* sources/integrations.F90:57-200; in a series stage j is evaluated at
+ * t + CK_N[j] dt */
+static int rk5_run(const fsrc *F, int m, const double *params, const double *time,
+                   double *state, int64_t cnt[3], int64_t max_steps)
 {
-    const double qom = params[0], vun = params[1], tmx = params[2], dtm = params[4];
+    const double tmx = params[2], dtm = params[4];
     const double atol = params[5], fcmn = params[7], fcmx = params[8], safe = params[9];
-    const double *xl = params + 11, *xs = params + 14;
     const double eps = DBL_MIN;                                            /* tiny(), :99 */
     double t, dt, dtn, del, q, si[6], sf[6], er[6], ac[6], s[6][6], k[6][6], sum;
     int n, status = PTC_STATUS_DONE;
 
     cnt[0] = cnt[1] = cnt[2] = 0;
     t = 0.0;
-    dt = orc_dti_guess(5, mode, dm, params, uu, bb, state);                /* :122 */
+    dt = dti_guess_f(F, 5, params, state);                                 /* :122 */
     for (int i = 0; i < 6; i++) si[i] = state[i];
     state[6] = sqrt(si[3] * si[3] + si[4] * si[4] + si[5] * si[5] + 1.0);
     state[7] = dt;
@@ -227,7 +331,7 @@ int orc_integrate_rk5(int mode, int m, const int dm[3], const double *params,
             for (int c = 0; c < 6; c++) s[j][c] = si[c];
             for (int i = 0; i < j; i++)
                 for (int c = 0; c < 6; c++) s[j][c] = s[j][c] + CK_A[j][i] * k[i][c];
-            orc_acceleration(mode, qom, vun, dm, xl, xs, uu, bb, s[j], ac);
+            fsrc_rhs(F, t + CK_N[j] * dt, s[j], ac);
             for (int c = 0; c < 6; c++) k[j][c] = dt * ac[c];
         }
         for (int c = 0; c < 6; c++) {                                      /* :140-143 */
@@ -267,6 +371,14 @@ int orc_integrate_rk5(int mode, int m, const int dm[3], const double *params,
         dt = dtn < dtm ? dtn : dtm;                                        /* :195 */
     }
     return status;
+}
+
+int orc_integrate_rk5(int mode, int m, const int dm[3], const double *params,
+                      const double *time, const double *uu, const double *bb,
+                      double *state, int64_t cnt[3], int64_t max_steps)
+{
+    const fsrc F = fsrc_make(mode, dm, params, 0, NULL, uu, bb);
+    return rk5_run(&F, m, params, time, state, cnt, max_steps);
 }
 
 /* Hairer-Norsett-Wanner DOP853 coefficients (public dop853.f; the reference
@@ -340,28 +452,37 @@ int orc_integrate_rk5(int mode, int m, const int dm[3], const double *params,
 #define ER10  0.3341791187130174790297318841e+00
 #define ER11  0.8192320648511571246570742613e-01
 #define ER12 -0.2235530786388629525884427845e-01
+/* stage nodes c_i (public dop853.f; the row sums of the A rows), for a series */
+#define DN02 0.526001519587677318785587544488e-01
+#define DN03 0.789002279381515978178381316732e-01
+#define DN04 0.118350341907227396726757197510e+00
+#define DN05 0.281649658092772603273242802490e+00
+#define DN06 0.333333333333333333333333333333e+00
+#define DN07 0.25e+00
+#define DN08 0.307692307692307692307692307692e+00
+#define DN09 0.651282051282051282051282051282e+00
+#define DN10 0.6e+00
+#define DN11 0.857142857142857142857142857142e+00
 
 /* sources/integrations.F90:224-505.  Slot names follow the stage numbers:
  * k11 and k12 are what the reference stores in k(2) and k(3). */
-int orc_integrate_dp853(int mode, int m, const int dm[3], const double *params,
-                        const double *time, const double *uu, const double *bb,
-                        double *state, int64_t cnt[3], int64_t max_steps)
+static int dp853_run(const fsrc *F, int m, const double *params, const double *time,
+                     double *state, int64_t cnt[3], int64_t max_steps)
 {
-    const double qom = params[0], vun = params[1], tmx = params[2], dtm = params[4];
+    const double tmx = params[2], dtm = params[4];
     const double atol = params[5], rtol = params[6], fcmn = params[7], fcmx = params[8];
     const double safe = params[9], beta = params[10];
-    const double *xl = params + 11, *xs = params + 14;
     const double expo = 2.0e-01 * beta - 1.25e-01;                         /* :348 */
     double errold = 1.0e-04, t, dt, dtn, err3, err5, deno, err, q, a, b;
     double si[6], ss[6], sf[6], sr[6], er[6], kb[6], ka[6], ke[6];
     double k1[6], k2[6], k3[6], k4[6], k5[6], k6[6], k7[6], k8[6], k9[6], k10[6], k11[6], k12[6];
     int rejected = 0, n, status = PTC_STATUS_DONE;
 
-#define RHS(in, out) orc_acceleration(mode, qom, vun, dm, xl, xs, uu, bb, in, out)
+#define RHS(in, out, tt) fsrc_rhs(F, tt, in, out)
 
     cnt[0] = cnt[1] = cnt[2] = 0;
     t = 0.0;
-    dt = orc_dti_guess(8, mode, dm, params, uu, bb, state);                /* :354 */
+    dt = dti_guess_f(F, 8, params, state);                                 /* :354 */
     for (int i = 0; i < 6; i++) si[i] = state[i];
     state[6] = sqrt(si[3] * si[3] + si[4] * si[4] + si[5] * si[5] + 1.0);
     state[7] = dt;
@@ -372,43 +493,43 @@ int orc_integrate_dp853(int mode, int m, const int dm[3], const double *params,
         if (!isfinite(dt)) { status = PTC_STATUS_NONFINITE; break; }
         if (!(dt > 0.0) || t + dt == t) { status = PTC_STATUS_DTSMALL; break; }
 
-        RHS(si, k1);                                                       /* :365-416 */
+        RHS(si, k1, t);                                                       /* :365-416 */
         for (int i = 0; i < 6; i++) ss[i] = si[i] + dt * A0201 * k1[i];
-        RHS(ss, k2);
+        RHS(ss, k2, t + DN02 * dt);
         for (int i = 0; i < 6; i++) ss[i] = si[i] + dt * (A0301 * k1[i] + A0302 * k2[i]);
-        RHS(ss, k3);
+        RHS(ss, k3, t + DN03 * dt);
         for (int i = 0; i < 6; i++) ss[i] = si[i] + dt * (A0401 * k1[i] + A0403 * k3[i]);
-        RHS(ss, k4);
+        RHS(ss, k4, t + DN04 * dt);
         for (int i = 0; i < 6; i++)
             ss[i] = si[i] + dt * (A0501 * k1[i] + A0503 * k3[i] + A0504 * k4[i]);
-        RHS(ss, k5);
+        RHS(ss, k5, t + DN05 * dt);
         for (int i = 0; i < 6; i++)
             ss[i] = si[i] + dt * (A0601 * k1[i] + A0604 * k4[i] + A0605 * k5[i]);
-        RHS(ss, k6);
+        RHS(ss, k6, t + DN06 * dt);
         for (int i = 0; i < 6; i++)
             ss[i] = si[i] + dt * (A0701 * k1[i] + A0704 * k4[i] + A0705 * k5[i] + A0706 * k6[i]);
-        RHS(ss, k7);
+        RHS(ss, k7, t + DN07 * dt);
         for (int i = 0; i < 6; i++)
             ss[i] = si[i] + dt * (A0801 * k1[i] + A0804 * k4[i] + A0805 * k5[i] + A0806 * k6[i] +
                                   A0807 * k7[i]);
-        RHS(ss, k8);
+        RHS(ss, k8, t + DN08 * dt);
         for (int i = 0; i < 6; i++)
             ss[i] = si[i] + dt * (A0901 * k1[i] + A0904 * k4[i] + A0905 * k5[i] + A0906 * k6[i] +
                                   A0907 * k7[i] + A0908 * k8[i]);
-        RHS(ss, k9);
+        RHS(ss, k9, t + DN09 * dt);
         for (int i = 0; i < 6; i++)
             ss[i] = si[i] + dt * (A1001 * k1[i] + A1004 * k4[i] + A1005 * k5[i] + A1006 * k6[i] +
                                   A1007 * k7[i] + A1008 * k8[i] + A1009 * k9[i]);
-        RHS(ss, k10);
+        RHS(ss, k10, t + DN10 * dt);
         for (int i = 0; i < 6; i++)
             ss[i] = si[i] + dt * (A1101 * k1[i] + A1104 * k4[i] + A1105 * k5[i] + A1106 * k6[i] +
                                   A1107 * k7[i] + A1108 * k8[i] + A1109 * k9[i] + A1110 * k10[i]);
-        RHS(ss, k11);
+        RHS(ss, k11, t + DN11 * dt);
         for (int i = 0; i < 6; i++)
             ss[i] = si[i] + dt * (A1201 * k1[i] + A1204 * k4[i] + A1205 * k5[i] + A1206 * k6[i] +
                                   A1207 * k7[i] + A1208 * k8[i] + A1209 * k9[i] + A1210 * k10[i] +
                                   A1211 * k11[i]);
-        RHS(ss, k12);
+        RHS(ss, k12, t + dt);
 
         for (int i = 0; i < 6; i++) {                                      /* :418-421 */
             kb[i] = B01 * k1[i] + B06 * k6[i] + B07 * k7[i] + B08 * k8[i] + B09 * k9[i] +
@@ -443,7 +564,7 @@ int orc_integrate_dp853(int mode, int m, const int dm[3], const double *params,
             t = t + dt;
             /* the reference re-evaluates f(si) (= k1, :445) and f(sf) (:446);
              * only f(sf) is new */
-            RHS(sf, ke);
+            RHS(sf, ke, t);
             for (int i = 0; i < 6; i++) { ka[i] = dt * k1[i]; kb[i] = dt * ke[i]; }
             while (n <= m && t >= time[n - 1]) {                           /* :447-457 */
                 hermite_out(state + 8 * (int64_t)(n - 1), t, time[n - 1], dt, si, sf, ka, kb);
@@ -470,6 +591,41 @@ int orc_integrate_dp853(int mode, int m, const int dm[3], const double *params,
     return status;
 }
 
+int orc_integrate_dp853(int mode, int m, const int dm[3], const double *params,
+                        const double *time, const double *uu, const double *bb,
+                        double *state, int64_t cnt[3], int64_t max_steps)
+{
+    const fsrc F = fsrc_make(mode, dm, params, 0, NULL, uu, bb);
+    return dp853_run(&F, m, params, time, state, cnt, max_steps);
+}
+
+int orc_stage_nodes(int method, double *c, double *rowsum)
+{
+    if (method == 0) {
+        for (int j = 0; j < 6; j++) {
+            c[j] = CK_N[j];
+            rowsum[j] = 0.0;
+            for (int i = 0; i < j; i++) rowsum[j] += CK_A[j][i];
+        }
+        return 6;
+    }
+    static const double cn[12] = {0.0, DN02, DN03, DN04, DN05, DN06, DN07, DN08, DN09, DN10,
+                                  DN11, 1.0};
+    const double rs[12] = {
+        0.0, A0201, A0301 + A0302, A0401 + A0403, A0501 + A0503 + A0504,
+        A0601 + A0604 + A0605, A0701 + A0704 + A0705 + A0706,
+        A0801 + A0804 + A0805 + A0806 + A0807,
+        A0901 + A0904 + A0905 + A0906 + A0907 + A0908,
+        A1001 + A1004 + A1005 + A1006 + A1007 + A1008 + A1009,
+        A1101 + A1104 + A1105 + A1106 + A1107 + A1108 + A1109 + A1110,
+        A1201 + A1204 + A1205 + A1206 + A1207 + A1208 + A1209 + A1210 + A1211};
+    for (int j = 0; j < 12; j++) {
+        c[j] = cn[j];
+        rowsum[j] = rs[j];
+    }
+    return 12;
+}
+
 /* sources/particles.F90:337-356 */
 void orc_integrate(int method, int mode, int64_t np, int m, const int dm[3],
                    const double *params, const double *time, const double *uu,
@@ -488,6 +644,27 @@ void orc_integrate(int method, int mode, int64_t np, int m, const int dm[3],
         int s = method == 0
                     ? orc_integrate_rk5(mode, m, dm, params, time, uu, bb, st, cnt + 3 * n, max_steps)
                     : orc_integrate_dp853(mode, m, dm, params, time, uu, bb, st, cnt + 3 * n, max_steps);
+        if (status) status[n] = s;
+    }
+}
+
+void orc_integrate_series(int method, int mode, int64_t np, int m, const int dm[3],
+                          const double *params, const double *time, int ns, const double *st,
+                          const double *uu, const double *bb, double *state, int64_t *cnt,
+                          int32_t *status, int64_t max_steps, int nthreads)
+{
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+    (void)nthreads;
+#endif
+    const fsrc F = fsrc_make(mode, dm, params, ns, st, uu, bb);
+    int64_t n;
+#pragma omp parallel for schedule(static)
+    for (n = 0; n < np; n++) {
+        double *sp = state + (int64_t)8 * m * n;
+        int s = method == 0 ? rk5_run(&F, m, params, time, sp, cnt + 3 * n, max_steps)
+                            : dp853_run(&F, m, params, time, sp, cnt + 3 * n, max_steps);
         if (status) status[n] = s;
     }
 }

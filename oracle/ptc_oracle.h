@@ -64,6 +64,30 @@ void orc_integrate(int method, int mode, int64_t np, int m, const int dm[3],
                    const double *bb, double *state, int64_t *cnt, int32_t *status,
                    int64_t max_steps, int nthreads);
 
+/* ---- time-dependent fields (no counterpart in the reference, whose field
+ * is static and whose RHS takes no time) ----
+ * A series holds ns snapshots at strictly increasing times st; uu / bb are
+ * ns consecutive (dm1, dm2, dm3, 3) arrays.  At time t the fields are
+ * f_k + a (f_{k+1} - f_k) of the trilinear interpolants of snapshots k, k+1
+ * with st[k] <= t < st[k+1], a = (t - st[k]) / (st[k+1] - st[k]); snapshot 0
+ * alone at or before st[0], snapshot ns-1 alone at or after st[ns-1].
+ * Stage i of a step from t of size dt is at t + c_i dt (orc_stage_nodes). */
+void orc_interpolate_series(const int dm[3], const double xl[3], const double xs[3], int ns,
+                            const double *st, const double *uu, const double *bb,
+                            const double pos[3], double t, double u[3], double b[3]);
+void orc_acceleration_series(int mode, double qom, double vun, const int dm[3],
+                             const double xl[3], const double xs[3], int ns, const double *st,
+                             const double *uu, const double *bb, const double vec[6], double t,
+                             double acc[6]);
+/* orc_integrate in a series (ns >= 1; the initial step guess sees t = 0) */
+void orc_integrate_series(int method, int mode, int64_t np, int m, const int dm[3],
+                          const double *params, const double *time, int ns, const double *st,
+                          const double *uu, const double *bb, double *state, int64_t *cnt,
+                          int32_t *status, int64_t max_steps, int nthreads);
+/* the stage nodes c (6 for rk5, 12 for dop853) and the row sums of the
+ * stage matrix the oracle holds; returns the number of stages */
+int orc_stage_nodes(int method, double *c, double *rowsum);
+
 int orc_max_threads(void);
 
 /* status codes shared with the device integrator */

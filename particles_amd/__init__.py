@@ -62,6 +62,7 @@ def _load():
     sig = {
         "ptc_fields_create": ([vp, vp, vp, i32, i32, C.POINTER(vp)], i32),
         "ptc_fields_destroy": ([vp], i32),
+        "ptc_fields_series": ([vp, vp, i32, C.POINTER(vp)], i32),
         "ptc_fields_synthesize": ([vp, i32, vp, i32, vp, vp, vp, vp, i32, vp, vp, vp, vp, i32,
                                    C.POINTER(vp)], i32),
         "ptc_integrate": ([vp, i32, i64, i32, vp, i32, vp, vp, vp, vp], i32),
@@ -175,6 +176,27 @@ class Fields:
         self._h = h
         return self
 
+    @classmethod
+    def series(cls, snapshots, times):
+        """Time-dependent fields (ptc_fields_series): the snapshots (Fields
+        of equal shape, mode and precision) at strictly increasing times,
+        blended linearly in time at every stage.  The device arrays are
+        copied, so the snapshots may be closed afterwards."""
+        snaps = list(snapshots)
+        times = np.ascontiguousarray(times, dtype=np.float64)
+        if len(snaps) != times.size:
+            raise ValueError(f"{len(snaps)} snapshots but {times.size} times")
+        self = cls.__new__(cls)
+        self.dm = snaps[0].dm.copy() if snaps else np.zeros(3, np.int32)
+        self.mode = snaps[0].mode if snaps else "vb"
+        self.prec = snaps[0].prec if snaps else "fp64"
+        self.times = times
+        arr = (C.c_void_p * max(len(snaps), 1))(*[f.handle for f in snaps])
+        h = C.c_void_p()
+        _check(_lib.ptc_fields_series(arr, _ptr(times), len(snaps), C.byref(h)))
+        self._h = h
+        return self
+
     @property
     def handle(self):
         return self._h
@@ -198,7 +220,7 @@ class Fields:
 
 
 _PROBES = {"gather": (0, 3, 6), "rhs": (1, 6, 6), "dti5": (2, 6, 1), "dti8": (3, 6, 1),
-           "pow": (4, 2, 1), "features": (5, 10, 16)}
+           "pow": (4, 2, 1), "features": (5, 10, 16), "gather_t": (6, 4, 6), "rhs_t": (7, 7, 6)}
 
 
 def probe(fields: Fields, op: str, inputs, params) -> np.ndarray:
@@ -206,7 +228,8 @@ def probe(fields: Fields, op: str, inputs, params) -> np.ndarray:
     U or E, B; interpolate_lin), "rhs" (vec -> d vec/dt; acceleration),
     "dti5"/"dti8" (vec -> initial step; dti_guess), "pow" ((x, y) -> x**y as
     the step controller computes it), "features" ((vec, h, err, pi_ratio,
-    rejected) -> the step-size network's 16 inputs, FEATURES).  inputs is
+    rejected) -> the step-size network's 16 inputs, FEATURES), "gather_t" /
+    "rhs_t" ((pos, t) / (vec, t) in a snapshot series).  inputs is
     (n, k); returns (n, 6), (n, 16) or (n,)."""
     code, nin, nout = _PROBES[op]
     x = np.ascontiguousarray(inputs, dtype=np.float64).reshape(-1, nin)

@@ -210,7 +210,7 @@ static int fields_from_device_soa(const int dm[3], double *soa, int mode, int pr
             return hip_fail(e, "ptc_fields: bricks");
         }
     }
-    ptc_fields *f = new ptc_fields;
+    ptc_fields *f = new ptc_fields();
     f->d.dm[0] = dm[0];
     f->d.dm[1] = dm[1];
     f->d.dm[2] = dm[2];
@@ -338,12 +338,66 @@ int ptc_fields_synthesize(const int dm[3], int recipe, const double *scalars, in
     return fields_from_device_soa(dm, soa, mode, prec, out);
 }
 
+int ptc_fields_series(const ptc_fields *const *snaps, const double *times, int n,
+                      ptc_fields **out)
+{
+    if (!out) return fail(PTC_EARG, "ptc_fields_series: out is null");
+    *out = nullptr;
+    if (!snaps || !times || n < 1) return fail(PTC_EARG, "ptc_fields_series: need n >= 1 snapshots and times");
+    if (!snaps[0]) return fail(PTC_EARG, "ptc_fields_series: snapshot 0 is null");
+    const DeviceFields &f0 = snaps[0]->d;
+    for (int i = 0; i < n; i++) {
+        if (!snaps[i]) return fail(PTC_EARG, "ptc_fields_series: snapshot %d is null", i);
+        const DeviceFields &fi = snaps[i]->d;
+        if (fi.nsnap > 0) return fail(PTC_EARG, "ptc_fields_series: snapshot %d is a series", i);
+        if (fi.layout != LAYOUT_AOS)
+            return fail(PTC_EARG, "ptc_fields_series: snapshot %d is not in the aos layout", i);
+        if (fi.dm[0] != f0.dm[0] || fi.dm[1] != f0.dm[1] || fi.dm[2] != f0.dm[2] ||
+            fi.mode != f0.mode || fi.prec != f0.prec)
+            return fail(PTC_EARG, "ptc_fields_series: snapshot %d differs from snapshot 0 in "
+                                  "dimensions, mode or precision", i);
+        if (!std::isfinite(times[i]))
+            return fail(PTC_EARG, "ptc_fields_series: time %d is not finite", i);
+        if (i > 0 && !(times[i] > times[i - 1]))
+            return fail(PTC_EARG, "ptc_fields_series: times must increase strictly (%d)", i);
+    }
+    const int64_t stride = (f0.prec == PTC_FP32 ? 8 : 6) * f0.nodes;
+    const size_t bytes = (size_t)stride * (f0.prec == PTC_FP32 ? sizeof(float) : sizeof(double));
+    void *data = nullptr;
+    double *st = nullptr;
+    HIP_TRY(hipMalloc(&data, bytes * (size_t)n));
+    hipError_t e = hipMalloc(&st, sizeof(double) * (size_t)n);
+    for (int i = 0; e == hipSuccess && i < n; i++)
+        e = hipMemcpy((char *)data + bytes * (size_t)i, snaps[i]->d.data, bytes,
+                      hipMemcpyDeviceToDevice);
+    if (e == hipSuccess) e = hipMemcpy(st, times, sizeof(double) * (size_t)n, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)hipFree(data);
+        if (st) (void)hipFree(st);
+        return hip_fail(e, "ptc_fields_series");
+    }
+    ptc_fields *f = new ptc_fields();
+    f->d = f0;
+    f->d.data = data;
+    f->d.bricks = nullptr;
+    f->d.nsnap = n;
+    f->d.sstride = stride;
+    f->d.stime = st;
+    *out = f;
+    ok();
+    return PTC_OK;
+}
+
 int ptc_fields_destroy(ptc_fields *f)
 {
     if (!f) return PTC_OK;
     hipError_t e = hipFree(f->d.data);
     if (f->d.bricks) {
         hipError_t e2 = hipFree(f->d.bricks);
+        if (e == hipSuccess) e = e2;
+    }
+    if (f->d.stime) {
+        hipError_t e2 = hipFree((void *)f->d.stime);
         if (e == hipSuccess) e = e2;
     }
     delete f;
@@ -504,7 +558,8 @@ int ptc_particles_advance(ptc_particles *p, double t_stop)
     }
     p->sp.max_steps = o.max_steps;
     LaunchConfig lc{o.block, o.grid_blocks,
-                    p->fields->d.layout == LAYOUT_AOS ? o.cell_cache : 0, p->ctrl};
+                    p->fields->d.layout == LAYOUT_AOS && p->fields->d.nsnap == 0 ? o.cell_cache : 0,
+                    p->ctrl};
     if (p->ctrl != CTRL_PI && lc.cell_cache != 1)
         return fail(PTC_ESTATE, "record and learned step-size modes need the aos layout with "
                                 "cell_cache 1");
@@ -647,7 +702,8 @@ int ptc_particles_advance_graph(ptc_particles *p, int n, const double *t_stop)
     }
     p->sp.max_steps = o.max_steps;
     LaunchConfig lc{o.block, o.grid_blocks,
-                    p->fields->d.layout == LAYOUT_AOS ? o.cell_cache : 0, p->ctrl};
+                    p->fields->d.layout == LAYOUT_AOS && p->fields->d.nsnap == 0 ? o.cell_cache : 0,
+                    p->ctrl};
     if (p->ctrl != CTRL_PI && lc.cell_cache != 1)
         return fail(PTC_ESTATE, "record and learned step-size modes need the aos layout with "
                                 "cell_cache 1");
@@ -803,7 +859,7 @@ int ptc_probe(const ptc_fields *f, int op, int64_t n, const double *params, int 
               const double *in, double *out)
 {
     if (!f) return fail(PTC_EARG, "ptc_probe: fields handle is null");
-    if (op < PTC_PROBE_GATHER || op > PTC_PROBE_FEATURES)
+    if (op < PTC_PROBE_GATHER || op > PTC_PROBE_RHS_T)
         return fail(PTC_EARG, "ptc_probe: bad op %d", op);
     if (n < 0) return fail(PTC_EARG, "ptc_probe: negative count");
     if (n == 0) { ok(); return PTC_OK; }
@@ -811,15 +867,19 @@ int ptc_probe(const ptc_fields *f, int op, int64_t n, const double *params, int 
     StepParams sp;
     int rc = build_step_params(f, 1, params, nparams, sp);
     if (rc) return rc;
-    if (op == PTC_PROBE_FEATURES && f->d.layout != LAYOUT_AOS)
-        return fail(PTC_EARG, "ptc_probe: features need the aos layout");
-    const int nin = op == PTC_PROBE_GATHER    ? 3
+    if (op >= PTC_PROBE_FEATURES && f->d.layout != LAYOUT_AOS)
+        return fail(PTC_EARG, "ptc_probe: op %d needs the aos layout", op);
+    const int nin = op == PTC_PROBE_GATHER     ? 3
                     : op == PTC_PROBE_POW      ? 2
                     : op == PTC_PROBE_FEATURES ? 10
+                    : op == PTC_PROBE_GATHER_T ? 4
+                    : op == PTC_PROBE_RHS_T    ? 7
                                                : 6;
-    const int nout = (op == PTC_PROBE_GATHER || op == PTC_PROBE_RHS) ? 6
-                     : op == PTC_PROBE_FEATURES                       ? 16
-                                                                      : 1;
+    const int nout = (op == PTC_PROBE_GATHER || op == PTC_PROBE_RHS || op == PTC_PROBE_GATHER_T ||
+                      op == PTC_PROBE_RHS_T)
+                         ? 6
+                     : op == PTC_PROBE_FEATURES ? 16
+                                                : 1;
     double *d = nullptr;
     HIP_TRY(hipMalloc(&d, sizeof(double) * (nin + nout) * n));
     hipError_t e = hipMemcpy(d, in, sizeof(double) * nin * n, hipMemcpyHostToDevice);
@@ -843,6 +903,8 @@ int ptc_particles_record(ptc_particles *p, double rate, int64_t capacity)
         return fail(PTC_EARG, "ptc_particles_record: rate must be in (0, 1] (got %g)", rate);
     if (capacity < 0 || capacity > ((int64_t)1 << 32))
         return fail(PTC_EARG, "ptc_particles_record: bad capacity %lld", (long long)capacity);
+    if (capacity > 0 && p->fields->d.nsnap > 0)
+        return fail(PTC_ESTATE, "ptc_particles_record: needs a static field (not a series)");
     HIP_TRY(hipStreamSynchronize(p->stream));
     if (p->rec_block) {
         HIP_TRY(hipFree(p->rec_block));
@@ -911,8 +973,8 @@ int ptc_particles_use_mlp(ptc_particles *p, const ptc_mlp *net, double shift)
         ok();
         return PTC_OK;
     }
-    if (p->fields->d.layout != LAYOUT_AOS)
-        return fail(PTC_ESTATE, "ptc_particles_use_mlp: needs the aos field layout");
+    if (p->fields->d.layout != LAYOUT_AOS || p->fields->d.nsnap > 0)
+        return fail(PTC_ESTATE, "ptc_particles_use_mlp: needs a static field in the aos layout");
     if (!net->trained)
         return fail(PTC_ESTATE, "ptc_particles_use_mlp: the network still has its random "
                                 "initialisation (train it or set its parameters)");

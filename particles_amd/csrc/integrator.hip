@@ -138,6 +138,17 @@ hipError_t launch_integrate(int method, const DeviceFields &f, const StepParams 
                             hipStream_t stream)
 {
     if (f.prec == PTC_FP32) return launch_integrate_mixed(method, f, sp, p, t_stop, lc, stream);
+    if (f.nsnap > 0) {
+        // snapshot series: node rows, PI controller, no cell cache
+        if (f.layout != LAYOUT_AOS || lc.ctrl != CTRL_PI) return hipErrorInvalidValue;
+        if (method == PTC_RK5 && f.mode == PTC_MODE_VB)
+            return launch_t<PTC_RK5, PTC_MODE_VB, LAYOUT_AOS, 0, PTC_FP64, CTRL_PI, 0, 1>(f, sp, p, t_stop, lc, stream);
+        if (method == PTC_RK5)
+            return launch_t<PTC_RK5, PTC_MODE_EB, LAYOUT_AOS, 0, PTC_FP64, CTRL_PI, 0, 1>(f, sp, p, t_stop, lc, stream);
+        if (f.mode == PTC_MODE_VB)
+            return launch_t<PTC_DOP853, PTC_MODE_VB, LAYOUT_AOS, 0, PTC_FP64, CTRL_PI, 0, 1>(f, sp, p, t_stop, lc, stream);
+        return launch_t<PTC_DOP853, PTC_MODE_EB, LAYOUT_AOS, 0, PTC_FP64, CTRL_PI, 0, 1>(f, sp, p, t_stop, lc, stream);
+    }
     // FP64: the general geometry only -- without its divisions the FP64
     // kernels (already at 512 registers) spill twice as much and run slower
     const int geom = 0;
@@ -165,8 +176,8 @@ hipError_t launch_probe(const DeviceFields &f, const StepParams &sp, int op, int
                         const double *in, double *out, hipStream_t stream)
 {
     if (n <= 0) return hipSuccess;
-    if (op < 0 || op > 5) return hipErrorInvalidValue;
-    if (op == 5 && f.layout != LAYOUT_AOS) return hipErrorInvalidValue;
+    if (op < 0 || op > 7) return hipErrorInvalidValue;
+    if (op >= 5 && f.layout != LAYOUT_AOS) return hipErrorInvalidValue;
     if (f.prec == PTC_FP32) return launch_probe_mixed(f, sp, op, n, in, out, stream);
     if (f.mode == PTC_MODE_VB && f.layout == LAYOUT_SOA)
         return launch_probe_t<PTC_MODE_VB, LAYOUT_SOA>(f, sp, op, n, in, out, stream);

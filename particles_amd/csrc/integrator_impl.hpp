@@ -243,15 +243,94 @@ __device__ __forceinline__ void rhs(const DeviceFields &F, const StepParams &P, 
 // every time (SOA and brick layouts).
 // PREC = PTC_FP32 is the mixed-precision force: FP32 nodes, weights and
 // arithmetic, slopes returned as float (K = float).
-template <int MODE, int LAYOUT, int CACHE, int PREC, int GEOM = 0>
+// TIME = 1: the fields are a series of snapshots (F.nsnap, F.stime); the
+// caller sets the stage time with at() before each evaluation, and the
+// fields there are f_k + a (f_{k+1} - f_k), the two bracketing snapshots'
+// trilinear interpolants blended linearly in time (node rows, no cell cache).
+// With TIME = 0, at() does nothing and the stage times are never formed.
+template <int MODE, int LAYOUT, int CACHE, int PREC, int GEOM = 0, int TIME = 0>
 struct Force {
     using K = std::conditional_t<PREC == PTC_FP32, float, double>;
     const DeviceFields &F;
     const StepParams &P;
     uint32_t key0 = 0xffffffffu, key7 = 0xffffffffu;
     K v[CACHE == 1 ? 48 : 1];
+    double ts = 0.0;  // TIME: stage time of the next evaluation
+    int kh = 0;       // TIME: the lane's last snapshot interval, in [0, nsnap - 2]
 
     __device__ __forceinline__ Force(const DeviceFields &F_, const StepParams &P_) : F(F_), P(P_) {}
+
+    __device__ __forceinline__ void at(double t)
+    {
+        if constexpr (TIME == 1) ts = t;
+    }
+
+    // Snapshot interval of the stage time: k with stime[k] <= ts <
+    // stime[k+1] and a = (ts - stime[k]) / (stime[k+1] - stime[k]) (two =
+    // true).  At or before the first time (or a NaN time) snapshot 0 alone,
+    // at or after the last time the last snapshot alone (two = false).  The
+    // search walks from the lane's last interval: stage times move forward
+    // by less than a snapshot spacing, so it is nearly always 0 or 1 steps.
+    // Both walks stay in [0, nsnap - 2] because stime[0] < ts < stime[nsnap-1].
+    __device__ __forceinline__ int interval(double &a, bool &two)
+    {
+        const int ns = F.nsnap;
+        const double *__restrict__ st = F.stime;
+        a = 0.0;
+        two = false;
+        if (ns <= 1 || !(ts > st[0])) return 0;
+        if (ts >= st[ns - 1]) return ns - 1;
+        int k = kh;
+        while (ts < st[k]) k--;
+        while (ts >= st[k + 1]) k++;
+        kh = k;
+        a = (ts - st[k]) / (st[k + 1] - st[k]);
+        two = true;
+        return k;
+    }
+
+    // TIME: the interpolated fields (U or E, B) at s and the stage time
+    __device__ __forceinline__ void fields_at(const double *s, K *f)
+    {
+        static_assert(TIME == 1 && CACHE == 0 && LAYOUT == LAYOUT_AOS,
+                      "a snapshot series is read as node rows without the cell cache");
+        CornersT<K> c;
+        corners<K, GEOM>(P, s, c);
+        double a;
+        bool two;
+        const int k = interval(a, two);
+        if constexpr (PREC == PTC_FP32) {
+            const float *__restrict__ d = (const float *)F.data + (size_t)k * F.sstride;
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                float w[6];
+                load_node_f32(d, c.o[j], w);
+                accumulate_row(j, c.w[j], w, f);
+            }
+            if (two) {
+                const float *__restrict__ e = d + F.sstride;
+                float g[6];
+#pragma unroll
+                for (int j = 0; j < 8; j++) {
+                    float w[6];
+                    load_node_f32(e, c.o[j], w);
+                    accumulate_row(j, c.w[j], w, g);
+                }
+                const float af = float(a);
+#pragma unroll
+                for (int i = 0; i < 6; i++) f[i] = f[i] + af * (g[i] - f[i]);
+            }
+        } else {
+            const double *__restrict__ d = (const double *)F.data + (size_t)k * F.sstride;
+            gather_rows(d, c, f);
+            if (two) {
+                double g[6];
+                gather_rows(d + F.sstride, c, g);
+#pragma unroll
+                for (int i = 0; i < 6; i++) f[i] = f[i] + a * (g[i] - f[i]);
+            }
+        }
+    }
 
     // loads the corner rows of the cell with corner offsets o unless cached
     __device__ __forceinline__ void fill(const uint32_t *o)
@@ -305,7 +384,12 @@ struct Force {
 
     __device__ __forceinline__ void operator()(const double *s, K *a)
     {
-        if constexpr (PREC == PTC_FP32) {
+        if constexpr (TIME == 1) {
+            K f[6];
+            fields_at(s, f);
+            if constexpr (PREC == PTC_FP32) force_from_fields_f32<MODE>(P, s, f, a);
+            else force_from_fields<MODE, GEOM>(P, s, f, a);
+        } else if constexpr (PREC == PTC_FP32) {
             static_assert(LAYOUT == LAYOUT_AOS, "FP32 fields are stored as AOS nodes");
             if constexpr (CACHE == 0) {
                 rhs_f32<MODE, GEOM>(F, P, s, a);
@@ -506,10 +590,18 @@ __device__ __forceinline__ double dti_guess(const StepParams &P, int iord, const
 // sf, and the stage-6 slope k6 used by the dense output.  Stage slopes are
 // kept in the RHS's type K (double, or float in mixed precision); stage
 // points, sums and the error norm are always FP64.
+// Stage nodes c_i of the Cash-Karp tableau (the row sums of its stage
+// matrix): stage i is evaluated at t + c_i dt.  Used only with time-dependent
+// fields; the reference's fields are static and its RHS takes no time.
+#define CK_N2 (1.0 / 5.0)
+#define CK_N3 (3.0 / 10.0)
+#define CK_N4 (3.0 / 5.0)
+#define CK_N6 (7.0 / 8.0)
+
 template <class Rhs>
 __device__ __forceinline__ double attempt_rk5(Rhs &rhsf, const StepParams &P, const double *si,
-                                              const typename Rhs::K *f1, double dt, double *sf,
-                                              double *k6d)
+                                              const typename Rhs::K *f1, double t, double dt,
+                                              double *sf, double *k6d)
 {
     using K = typename Rhs::K;
     K k1[6], k2[6], k3[6], k4[6], k5[6], k6[6], a[6];
@@ -518,17 +610,20 @@ __device__ __forceinline__ double attempt_rk5(Rhs &rhsf, const StepParams &P, co
     for (int c = 0; c < 6; c++) k1[c] = K(dt * double(f1[c]));
 #pragma unroll
     for (int c = 0; c < 6; c++) s[c] = si[c] + CK_A21 * double(k1[c]);
+    rhsf.at(t + CK_N2 * dt);
     rhsf(s, a);
 #pragma unroll
     for (int c = 0; c < 6; c++) k2[c] = K(dt * double(a[c]));
 #pragma unroll
     for (int c = 0; c < 6; c++) s[c] = si[c] + CK_A31 * double(k1[c]) + CK_A32 * double(k2[c]);
+    rhsf.at(t + CK_N3 * dt);
     rhsf(s, a);
 #pragma unroll
     for (int c = 0; c < 6; c++) k3[c] = K(dt * double(a[c]));
 #pragma unroll
     for (int c = 0; c < 6; c++)
         s[c] = si[c] + CK_A41 * double(k1[c]) + CK_A42 * double(k2[c]) + CK_A43 * double(k3[c]);
+    rhsf.at(t + CK_N4 * dt);
     rhsf(s, a);
 #pragma unroll
     for (int c = 0; c < 6; c++) k4[c] = K(dt * double(a[c]));
@@ -536,6 +631,7 @@ __device__ __forceinline__ double attempt_rk5(Rhs &rhsf, const StepParams &P, co
     for (int c = 0; c < 6; c++)
         s[c] = si[c] + CK_A51 * double(k1[c]) + CK_A52 * double(k2[c]) + CK_A53 * double(k3[c]) +
                CK_A54 * double(k4[c]);
+    rhsf.at(t + dt);
     rhsf(s, a);
 #pragma unroll
     for (int c = 0; c < 6; c++) k5[c] = K(dt * double(a[c]));
@@ -543,6 +639,7 @@ __device__ __forceinline__ double attempt_rk5(Rhs &rhsf, const StepParams &P, co
     for (int c = 0; c < 6; c++)
         s[c] = si[c] + CK_A61 * double(k1[c]) + CK_A62 * double(k2[c]) + CK_A63 * double(k3[c]) +
                CK_A64 * double(k4[c]) + CK_A65 * double(k5[c]);
+    rhsf.at(t + CK_N6 * dt);
     rhsf(s, a);
 #pragma unroll
     for (int c = 0; c < 6; c++) k6[c] = K(dt * double(a[c]));
@@ -648,62 +745,77 @@ __device__ __forceinline__ double attempt_rk5(Rhs &rhsf, const StepParams &P, co
 #define ER11 0.8192320648511571246570742613e-01
 #define ER12 -0.2235530786388629525884427845e-01
 
+// Stage nodes c_i of DOP853 (public dop853.f; the row sums of the stage
+// matrix above), for time-dependent fields only: stage i at t + c_i dt.
+#define DN02 0.526001519587677318785587544488e-01
+#define DN03 0.789002279381515978178381316732e-01
+#define DN04 0.118350341907227396726757197510e+00
+#define DN05 0.281649658092772603273242802490e+00
+#define DN06 0.333333333333333333333333333333e+00
+#define DN07 0.25e+00
+#define DN08 0.307692307692307692307692307692e+00
+#define DN09 0.651282051282051282051282051282e+00
+#define DN10 0.6e+00
+#define DN11 0.857142857142857142857142857142e+00
+
 // k1 = f(si) is given (raw derivatives, no dt inside, as in the reference);
 // returns err and sf.  Slopes are stored in the RHS's type K; stage points,
 // sums and the error norm are FP64.
 template <class Rhs>
 __device__ __forceinline__ double attempt_dp853(Rhs &rhsf, const StepParams &P, const double *si,
-                                                const typename Rhs::K *k1, double dt, double *sf)
+                                                const typename Rhs::K *k1, double t, double dt,
+                                                double *sf)
 {
     using K = typename Rhs::K;
     K k2[6], k3[6], k4[6], k5[6], k6[6], k7[6], k8[6], k9[6], k10[6], k11[6], k12[6];
     double ss[6];
-#define RHS(out) rhsf(ss, out)
+#define RHS(out, node) rhsf.at(t + (node) * dt), rhsf(ss, out)
 #pragma unroll
     for (int i = 0; i < 6; i++) ss[i] = si[i] + dt * A0201 * double(k1[i]);
-    RHS(k2);
+    RHS(k2, DN02);
 #pragma unroll
     for (int i = 0; i < 6; i++) ss[i] = si[i] + dt * (A0301 * double(k1[i]) + A0302 * double(k2[i]));
-    RHS(k3);
+    RHS(k3, DN03);
 #pragma unroll
     for (int i = 0; i < 6; i++) ss[i] = si[i] + dt * (A0401 * double(k1[i]) + A0403 * double(k3[i]));
-    RHS(k4);
+    RHS(k4, DN04);
 #pragma unroll
     for (int i = 0; i < 6; i++) ss[i] = si[i] + dt * (A0501 * double(k1[i]) + A0503 * double(k3[i]) + A0504 * double(k4[i]));
-    RHS(k5);
+    RHS(k5, DN05);
 #pragma unroll
     for (int i = 0; i < 6; i++) ss[i] = si[i] + dt * (A0601 * double(k1[i]) + A0604 * double(k4[i]) + A0605 * double(k5[i]));
-    RHS(k6);
+    RHS(k6, DN06);
 #pragma unroll
     for (int i = 0; i < 6; i++)
         ss[i] = si[i] + dt * (A0701 * double(k1[i]) + A0704 * double(k4[i]) + A0705 * double(k5[i]) + A0706 * double(k6[i]));
-    RHS(k7);
+    RHS(k7, DN07);
 #pragma unroll
     for (int i = 0; i < 6; i++)
         ss[i] = si[i] + dt * (A0801 * double(k1[i]) + A0804 * double(k4[i]) + A0805 * double(k5[i]) + A0806 * double(k6[i]) +
                               A0807 * double(k7[i]));
-    RHS(k8);
+    RHS(k8, DN08);
 #pragma unroll
     for (int i = 0; i < 6; i++)
         ss[i] = si[i] + dt * (A0901 * double(k1[i]) + A0904 * double(k4[i]) + A0905 * double(k5[i]) + A0906 * double(k6[i]) +
                               A0907 * double(k7[i]) + A0908 * double(k8[i]));
-    RHS(k9);
+    RHS(k9, DN09);
 #pragma unroll
     for (int i = 0; i < 6; i++)
         ss[i] = si[i] + dt * (A1001 * double(k1[i]) + A1004 * double(k4[i]) + A1005 * double(k5[i]) + A1006 * double(k6[i]) +
                               A1007 * double(k7[i]) + A1008 * double(k8[i]) + A1009 * double(k9[i]));
-    RHS(k10);
+    RHS(k10, DN10);
 #pragma unroll
     for (int i = 0; i < 6; i++)
         ss[i] = si[i] + dt * (A1101 * double(k1[i]) + A1104 * double(k4[i]) + A1105 * double(k5[i]) + A1106 * double(k6[i]) +
                               A1107 * double(k7[i]) + A1108 * double(k8[i]) + A1109 * double(k9[i]) + A1110 * double(k10[i]));
-    RHS(k11);
+    RHS(k11, DN11);
 #pragma unroll
     for (int i = 0; i < 6; i++)
         ss[i] = si[i] + dt * (A1201 * double(k1[i]) + A1204 * double(k4[i]) + A1205 * double(k5[i]) + A1206 * double(k6[i]) +
                               A1207 * double(k7[i]) + A1208 * double(k8[i]) + A1209 * double(k9[i]) + A1210 * double(k10[i]) +
                               A1211 * double(k11[i]));
-    RHS(k12);
+    rhsf.at(t + dt);
+    rhsf(ss, k12);
 #undef RHS
 
     double err3 = 0.0, err5 = 0.0;
@@ -811,12 +923,16 @@ __device__ __forceinline__ bool record_sampled(int64_t idx, int64_t c, uint64_t 
 // after an accepted step the network proposes the next step, after a
 // rejection the PI rule does).  CTRL_LEARNED takes the network and a 64x72
 // BF16 buffer per wave in dynamic LDS.
-template <int METHOD, int MODE, int LAYOUT, int CACHE, int PREC, int CTRL, int GEOM>
+// TIME = 1 integrates in a snapshot series (fields interpolated in time at
+// each stage; PI controller, node rows, no cell cache).
+template <int METHOD, int MODE, int LAYOUT, int CACHE, int PREC, int CTRL, int GEOM, int TIME = 0>
 __global__ void __launch_bounds__(kBlock)
 integrate_kernel(const DeviceFields F, const StepParams P, const DeviceParticles S, double t_stop)
 {
     static_assert(CTRL == CTRL_PI || (CACHE == 1 && LAYOUT == LAYOUT_AOS),
                   "step-size features read the cell cache");
+    static_assert(TIME == 0 || (CTRL == CTRL_PI && CACHE == 0 && LAYOUT == LAYOUT_AOS),
+                  "a snapshot series runs with the PI controller on node rows");
     // the power's lookup tables (5 KB) in LDS: per-lane indices make every
     // lookup a divergent gather, which LDS serves far faster than memory
     __shared__ double s_logt[128][3];
@@ -832,7 +948,7 @@ integrate_kernel(const DeviceFields F, const StepParams P, const DeviceParticles
     __syncthreads();
     const PowTables T{s_logt, s_expt};
 
-    Force<MODE, LAYOUT, CACHE, PREC, GEOM> force(F, P);
+    Force<MODE, LAYOUT, CACHE, PREC, GEOM, TIME> force(F, P);
     using K = typename decltype(force)::K;
     const int lane = (int)__lane_id();
     const double t_end = t_stop < P.tmx ? t_stop : P.tmx;
@@ -902,6 +1018,7 @@ integrate_kernel(const DeviceFields F, const StepParams P, const DeviceParticles
             CTRL == CTRL_LEARNED ? want_feat
                                  : want_feat && record_sampled(idx, c_acc, S.rec_thresh);
         if (need_rhs) {
+            force.at(t);
             // (FP64 fields: the kernel is short of registers already, the
             // features are taken from the cache afterwards instead)
             if constexpr (CTRL == CTRL_PI || PREC != PTC_FP32) {
@@ -996,7 +1113,7 @@ integrate_kernel(const DeviceFields F, const StepParams P, const DeviceParticles
                 double dtn;
                 if constexpr (METHOD == PTC_RK5) {
                     double k6[6];
-                    const double del = attempt_rk5(force, P, si, f1, dt, sf, k6);
+                    const double del = attempt_rk5(force, P, si, f1, t, dt, sf, k6);
                     c_rhs += 5;
                     if (!isfinite(del) || !finite6(sf)) {
                         status = PTC_STATUS_NONFINITE;
@@ -1035,7 +1152,7 @@ integrate_kernel(const DeviceFields F, const StepParams P, const DeviceParticles
                       }
                     }
                 } else {
-                    const double err = attempt_dp853(force, P, si, f1, dt, sf);
+                    const double err = attempt_dp853(force, P, si, f1, t, dt, sf);
                     c_rhs += 11;
                     if (!isfinite(err) || !finite6(sf)) {
                         status = PTC_STATUS_NONFINITE;
@@ -1050,6 +1167,7 @@ integrate_kernel(const DeviceFields F, const StepParams P, const DeviceParticles
                             // the next step's stage 1
                             K fe[6];
                             double ka[6], kb[6];
+                            force.at(t);
                             force(sf, fe);
                             c_rhs++;
 #pragma unroll
@@ -1131,6 +1249,8 @@ integrate_kernel(const DeviceFields F, const StepParams P, const DeviceParticles
 //   op 2: dti_guess, order 5, vec(6) -> dt(1) (op 3: order 8)
 //   op 4: step-size power x^y, (x, y) -> (1)
 //   op 5: step-size features, (vec(6), h, err, pi_ratio, rejected) -> (16)
+//   op 6: gather at (pos(3), t) in a snapshot series -> f(6)   (node rows)
+//   op 7: RHS at (vec(6), t) in a snapshot series    -> acc(6) (node rows)
 template <int MODE, int LAYOUT, int PREC>
 __global__ void probe_kernel(const DeviceFields F, const StepParams P, int op, int64_t n,
                              const double *__restrict__ in, double *__restrict__ out)
@@ -1184,6 +1304,18 @@ __global__ void probe_kernel(const DeviceFields F, const StepParams P, int op, i
 #pragma unroll
             for (int k = 0; k < kFeatures; k++) out[kFeatures * i + k] = double(feat[k]);
         }
+    } else if (op == 6 || op == 7) {
+        if constexpr (LAYOUT == LAYOUT_AOS) {
+            Force<MODE, LAYOUT_AOS, 0, PREC, 0, 1> ft(F, P);
+            const int w = op == 6 ? 3 : 6;
+            const double *q = in + (size_t)(w + 1) * i;
+            for (int k = 0; k < 6; k++) v[k] = k < w ? q[k] : 0.0;
+            ft.at(q[w]);
+            if (op == 6) ft.fields_at(v, r);
+            else ft(v, r);
+#pragma unroll
+            for (int k = 0; k < 6; k++) out[6 * i + k] = double(r[k]);
+        }
     }
 }
 
@@ -1197,11 +1329,11 @@ hipError_t launch_probe_t(const DeviceFields &f, const StepParams &sp, int op, i
     return hipGetLastError();
 }
 
-template <int METHOD, int MODE, int LAYOUT, int CACHE, int PREC, int CTRL, int GEOM>
+template <int METHOD, int MODE, int LAYOUT, int CACHE, int PREC, int CTRL, int GEOM, int TIME = 0>
 hipError_t launch_t(const DeviceFields &f, const StepParams &sp, const DeviceParticles &p,
                     double t_stop, LaunchConfig lc, hipStream_t stream)
 {
-    auto kern = integrate_kernel<METHOD, MODE, LAYOUT, CACHE, PREC, CTRL, GEOM>;
+    auto kern = integrate_kernel<METHOD, MODE, LAYOUT, CACHE, PREC, CTRL, GEOM, TIME>;
     const size_t lds = CTRL == CTRL_LEARNED ? kNetLds + (size_t)(lc.block / 64) * sizeof(MlpWaveBuf) : 0;
     if (CTRL == CTRL_LEARNED && !p.net) return hipErrorInvalidValue;
     int grid = lc.grid_blocks;

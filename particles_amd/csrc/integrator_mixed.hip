@@ -14,6 +14,19 @@ hipError_t launch_integrate_mixed(int method, const DeviceFields &f, const StepP
 {
     if (f.prec != PTC_FP32 || f.layout != LAYOUT_AOS) return hipErrorInvalidValue;
     const int geom = sp.pow2 ? 1 : 0;
+    if (f.nsnap > 0) {
+        // snapshot series: PI controller, no cell cache
+        if (lc.ctrl != CTRL_PI) return hipErrorInvalidValue;
+#define SERIES(M, MO, G)                                                       \
+    if (method == M && f.mode == MO && geom == G)                              \
+        return launch_t<M, MO, LAYOUT_AOS, 0, PTC_FP32, CTRL_PI, G, 1>(f, sp, p, t_stop, lc, stream);
+        SERIES(PTC_RK5, PTC_MODE_VB, 0) SERIES(PTC_RK5, PTC_MODE_EB, 0)
+        SERIES(PTC_DOP853, PTC_MODE_VB, 0) SERIES(PTC_DOP853, PTC_MODE_EB, 0)
+        SERIES(PTC_RK5, PTC_MODE_VB, 1) SERIES(PTC_RK5, PTC_MODE_EB, 1)
+        SERIES(PTC_DOP853, PTC_MODE_VB, 1) SERIES(PTC_DOP853, PTC_MODE_EB, 1)
+#undef SERIES
+        return hipErrorInvalidValue;
+    }
 #define DISPATCH(M, MO, C, CT, G)                                              \
     if (method == M && f.mode == MO && lc.cell_cache == C && lc.ctrl == CT && geom == G) \
         return launch_t<M, MO, LAYOUT_AOS, C, PTC_FP32, CT, G>(f, sp, p, t_stop, lc, stream);

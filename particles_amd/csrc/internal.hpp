@@ -29,6 +29,12 @@ struct DeviceFields {
     int64_t nodes;
     void *data;   // SOA: 6 * nodes values; AOS/BRICK: nodes * 6 values
     void *bricks; // BRICK: nodes * 48 values (cell-major), else nullptr
+    // time series (AOS only): nsnap snapshots of node rows stored one after
+    // another, sstride values apart, at the strictly increasing times
+    // stime[nsnap] (device memory).  nsnap = 0: one static snapshot.
+    int nsnap;
+    int64_t sstride;
+    const double *stime;
 };
 
 // Geometry and controller constants of one integration (params vector of
@@ -83,7 +89,9 @@ struct LaunchConfig {
     int ctrl;          // CTRL_*; RECORD and LEARNED need the AOS layout with the cell cache
 };
 
-// Launches the integrator on `stream`; returns a hipError_t.
+// Launches the integrator on `stream`; returns a hipError_t.  A series of
+// snapshots (f.nsnap > 0) takes the time-interpolating kernels (PI
+// controller, no cell cache).
 hipError_t launch_integrate(int method, const DeviceFields &f, const StepParams &sp,
                             const DeviceParticles &p, double t_stop, const LaunchConfig &lc,
                             hipStream_t stream);

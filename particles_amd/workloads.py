@@ -146,6 +146,30 @@ def turbulent_vb(dm: int, seed: int, b0: float = 1.0, brms: float = 0.5, vrms: f
     return uu, bb
 
 
+def propagate(modes: Modes, t: float, speed: float) -> Modes:
+    """The modes as waves travelling along their wave vectors at `speed`
+    (box lengths per time unit): phase_m - 2 pi |n_m| speed t."""
+    kn = np.sqrt((modes.n ** 2).sum(1))
+    return Modes(n=modes.n, amp=modes.amp, pol=modes.pol,
+                 phase=modes.phase - TWO_PI * kn * speed * t)
+
+
+def turbulent_vb_series(dm: int, seed: int, times, speed: float = 0.01, b0: float = 1.0,
+                        brms: float = 0.5, vrms: float = 0.1, nmodes: int = 256):
+    """Snapshots of turbulent_vb at `times` with every mode propagating
+    (propagate); returns (uu, bb) Fortran (dm, dm, dm, 3, len(times))."""
+    mb = random_modes(seed, nmodes=nmodes, rms=brms, stream=1)
+    mv = random_modes(seed, nmodes=nmodes, rms=vrms, stream=2)
+    ns = len(times)
+    uu = np.empty((dm, dm, dm, 3, ns), dtype=np.float64, order="F")
+    bb = np.empty_like(uu, order="F")
+    for k, t in enumerate(times):
+        bb[..., k] = eval_modes(propagate(mb, t, speed), (dm, dm, dm))
+        bb[..., 2, k] += b0
+        uu[..., k] = eval_modes(propagate(mv, t, speed), (dm, dm, dm))
+    return uu, bb
+
+
 def sheet_profile(y: np.ndarray, delta: float) -> np.ndarray:
     """Periodic double current sheet: tanh((y-1/4)/d) - tanh((y-3/4)/d) - 1,
     which is tanh(y'/d) about the sheet at y = 1/4 and continuous across the
@@ -192,6 +216,26 @@ def turbulent_vb_device(dm: int, seed: int, prec: str = "fp64", b0: float = 1.0,
     mb = random_modes(seed, nmodes=nmodes, rms=brms, stream=1)
     mv = random_modes(seed, nmodes=nmodes, rms=vrms, stream=2)
     return Fields.synthesize((dm, dm, dm), "turbulent_vb", (b0,), mb, mv, prec=prec)
+
+
+def turbulent_vb_series_device(dm: int, seed: int, times, prec: str = "fp64",
+                               speed: float = 0.01, b0: float = 1.0, brms: float = 0.5,
+                               vrms: float = 0.1, nmodes: int = 256):
+    """turbulent_vb_series built on the device: one synthesised snapshot per
+    time, gathered into a series (particles_amd.Fields.series)."""
+    from . import Fields
+    mb = random_modes(seed, nmodes=nmodes, rms=brms, stream=1)
+    mv = random_modes(seed, nmodes=nmodes, rms=vrms, stream=2)
+    snaps = []
+    try:
+        for t in times:
+            snaps.append(Fields.synthesize((dm, dm, dm), "turbulent_vb", (b0,),
+                                           propagate(mb, t, speed), propagate(mv, t, speed),
+                                           prec=prec))
+        return Fields.series(snaps, times)
+    finally:
+        for f in snaps:
+            f.close()
 
 
 def reconnection_eb_device(dm: int, seed: int, prec: str = "fp64", delta: float = 0.02,
