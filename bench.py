@@ -22,6 +22,12 @@ Prints one JSON line.  `roofline` is the integrator kernel's algorithmic
 gather traffic (8 corners x 6 components x 8 B = 384 B per RHS evaluation,
 times the RHS evaluations the kernel counted) over its duration measured with
 HIP events on the batch's stream, against the MI355X HBM peak.
+With `--snapshots N` the field is time-dependent: N device-built snapshots
+of the same turbulent recipe, its modes travelling in time, spread over the
+benchmarked time and blended linearly in time at every stage; each RHS then
+reads two snapshots (768 B per evaluation in FP64).  The reference has no
+time-dependent field, so no CPU baseline or variants are run for it.
+
 `cpu_baseline` times the reference's own FP64 OpenMP integrator (compiled
 from its sources into oracle/_ref) -- or the C oracle where that library is
 absent -- on the host cores, on the first particles of the same workload.
@@ -69,7 +75,21 @@ def build_workload(args):
     if args.method:
         method = args.method
     uu = bb = None
-    if recipe == "turbulent" and not device:
+    if args.snapshots > 0:
+        if recipe != "turbulent":
+            raise SystemExit("--snapshots applies to the turbulent workloads")
+        # a snapshot series spanning the benchmarked time, every mode
+        # travelling at 0.01 box lengths per unit time, built on the device
+        times = np.linspace(0.0, (args.warmup + args.steps) * W.T0, args.snapshots)
+
+        def make_fields(prec):
+            return W.turbulent_vb_series_device(dm, seed=args.seed, times=times, prec=prec,
+                                                speed=0.01)
+        desc = (f"{dm}^3 VB turbulent field (B0=1, dB rms 0.5, V rms 0.1, 256 modes) as "
+                f"{args.snapshots} snapshots over {args.warmup + args.steps} T0 (modes "
+                "travelling at 0.01 per unit time), blended linearly in time at each stage, "
+                "built on the device")
+    elif recipe == "turbulent" and not device:
         uu, bb = W.turbulent_vb(dm, seed=args.seed)
 
         def make_fields(prec):
@@ -92,8 +112,9 @@ def build_workload(args):
                        f"step = advance all particles by 1 gyro-period of B0",
            "grid": dm, "particles": npart, "method": method, "tol": tol,
            "snapshots_every_T0": tmax / W.T0 / nsnap, "layout": args.layout,
-           "cell_cache": args.cell_cache, "field_precision": args.prec, "sort": args.sort,
-           "order": args.order}
+           "cell_cache": args.cell_cache if args.snapshots == 0 else 0,
+           "field_precision": args.prec, "sort": args.sort, "order": args.order,
+           "field_snapshots": args.snapshots}
     return make_fields, uu, bb, p, tg, st0, method, cfg
 
 
@@ -143,8 +164,9 @@ def traffic_from_profile(cfg):
     except (OSError, ValueError):
         return None
     keys = ("grid", "particles", "method", "tol", "layout", "cell_cache", "field_precision",
-            "sort", "order")
-    if any(prof.get("config", {}).get(k) != cfg.get(k) for k in keys):
+            "sort", "order", "field_snapshots")
+    if any(prof.get("config", {}).get(k, 0 if k == "field_snapshots" else None) != cfg.get(k)
+           for k in keys):
         return None
     return prof["hbm_bytes_per_launch"]
 
@@ -287,6 +309,8 @@ def main():
     ap.add_argument("--accuracy-particles", type=int, default=20000)
     ap.add_argument("--train-gamma", type=float, nargs=2, default=(1.0, 1.0e3))
     ap.add_argument("--train-steps", type=int, default=3000)
+    ap.add_argument("--snapshots", type=int, default=0,
+                    help="time-dependent field: this many snapshots over the run (0 = static)")
     ap.add_argument("--shift", type=float, default=None,
                     help="added to the network's log step factor (default: calibrated)")
     args = ap.parse_args()
@@ -338,11 +362,13 @@ def main():
     # algorithmic gather bytes per RHS: 8 corners x 6 components x 8 B (FP64)
     # or x 4 B (FP32 fields)
     bytes_per_rhs = BYTES_PER_RHS_FP64 if args.prec == "fp64" else BYTES_PER_RHS_FP64 // 2
+    if args.snapshots > 1:
+        bytes_per_rhs *= 2  # the two bracketing snapshots
     achieved = bytes_per_rhs * rhs / (kernel_ms * 1e-3) / 1e9
     batch.close()
 
     var = None
-    if not args.no_variants and args.prec == "fp64":
+    if not args.no_variants and args.prec == "fp64" and args.snapshots == 0:
         print("variants ...", file=sys.stderr, flush=True)
         var = variants(args, fields, make_fields, p, tg, st0, method, dict(acc=acc, wall=wall))
 
@@ -361,6 +387,8 @@ def main():
         "dtype": "f64", "data": "synthetic (seeded random-mode fields and particles)",
         "config": cfg,
         "variant": ("plain FP64, bit-identical to the reference" if uu is not None else
+                    "plain FP64, time-dependent field (bit-identical to the oracle; the "
+                    "reference's field is static)" if args.snapshots > 0 else
                     "plain FP64 (EB fields: bit-identical to the oracle; the reference has "
                     "no E-field form)"),
         "accepted": acc, "rejected": rej, "rejected_per_accepted": rej / max(acc, 1),

@@ -107,8 +107,11 @@ int ptc_fields_synthesize(const int dm[3], int recipe, const double *scalars, in
  * times[0] snapshot 0 alone, at or after times[n-1] snapshot n-1 alone.
  * Stage i of a step from t of size dt is at t + c_i dt, c_i the method's
  * published nodes (Cash-Karp 1990; Hairer-Norsett-Wanner dop853).  A
- * series integrates with the PI step-size controller and without the cell
- * cache; n = 1 is a static field taken through the series kernels. */
+ * series integrates with the PI step-size controller; with the cell cache
+ * (option cell_cache 1) each lane keeps the two snapshots' corner values of
+ * its last cell, the second in LDS (48 values per thread), where the device
+ * gives a block that much LDS.  n = 1 is a static field taken through the
+ * series kernels. */
 int ptc_fields_series(const ptc_fields *const *snaps, const double *times, int n,
                       ptc_fields **out);
 

@@ -558,7 +558,7 @@ int ptc_particles_advance(ptc_particles *p, double t_stop)
     }
     p->sp.max_steps = o.max_steps;
     LaunchConfig lc{o.block, o.grid_blocks,
-                    p->fields->d.layout == LAYOUT_AOS && p->fields->d.nsnap == 0 ? o.cell_cache : 0,
+                    p->fields->d.layout == LAYOUT_AOS ? o.cell_cache : 0,
                     p->ctrl};
     if (p->ctrl != CTRL_PI && lc.cell_cache != 1)
         return fail(PTC_ESTATE, "record and learned step-size modes need the aos layout with "
@@ -702,7 +702,7 @@ int ptc_particles_advance_graph(ptc_particles *p, int n, const double *t_stop)
     }
     p->sp.max_steps = o.max_steps;
     LaunchConfig lc{o.block, o.grid_blocks,
-                    p->fields->d.layout == LAYOUT_AOS && p->fields->d.nsnap == 0 ? o.cell_cache : 0,
+                    p->fields->d.layout == LAYOUT_AOS ? o.cell_cache : 0,
                     p->ctrl};
     if (p->ctrl != CTRL_PI && lc.cell_cache != 1)
         return fail(PTC_ESTATE, "record and learned step-size modes need the aos layout with "

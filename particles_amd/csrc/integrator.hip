@@ -139,15 +139,19 @@ hipError_t launch_integrate(int method, const DeviceFields &f, const StepParams 
 {
     if (f.prec == PTC_FP32) return launch_integrate_mixed(method, f, sp, p, t_stop, lc, stream);
     if (f.nsnap > 0) {
-        // snapshot series: node rows, PI controller, no cell cache
+        // snapshot series: node rows, PI controller; the cell cache keeps
+        // the second snapshot in LDS where the device has room for it
         if (f.layout != LAYOUT_AOS || lc.ctrl != CTRL_PI) return hipErrorInvalidValue;
-        if (method == PTC_RK5 && f.mode == PTC_MODE_VB)
-            return launch_t<PTC_RK5, PTC_MODE_VB, LAYOUT_AOS, 0, PTC_FP64, CTRL_PI, 0, 1>(f, sp, p, t_stop, lc, stream);
-        if (method == PTC_RK5)
-            return launch_t<PTC_RK5, PTC_MODE_EB, LAYOUT_AOS, 0, PTC_FP64, CTRL_PI, 0, 1>(f, sp, p, t_stop, lc, stream);
-        if (f.mode == PTC_MODE_VB)
-            return launch_t<PTC_DOP853, PTC_MODE_VB, LAYOUT_AOS, 0, PTC_FP64, CTRL_PI, 0, 1>(f, sp, p, t_stop, lc, stream);
-        return launch_t<PTC_DOP853, PTC_MODE_EB, LAYOUT_AOS, 0, PTC_FP64, CTRL_PI, 0, 1>(f, sp, p, t_stop, lc, stream);
+        const int c = lc.cell_cache == 1 && series_cache_fits(PTC_FP64, lc.block) ? 1 : 0;
+#define SERIES(M, MO, C)                                                       \
+    if (method == M && f.mode == MO && c == C)                                 \
+        return launch_t<M, MO, LAYOUT_AOS, C, PTC_FP64, CTRL_PI, 0, 1>(f, sp, p, t_stop, lc, stream);
+        SERIES(PTC_RK5, PTC_MODE_VB, 0) SERIES(PTC_RK5, PTC_MODE_EB, 0)
+        SERIES(PTC_DOP853, PTC_MODE_VB, 0) SERIES(PTC_DOP853, PTC_MODE_EB, 0)
+        SERIES(PTC_RK5, PTC_MODE_VB, 1) SERIES(PTC_RK5, PTC_MODE_EB, 1)
+        SERIES(PTC_DOP853, PTC_MODE_VB, 1) SERIES(PTC_DOP853, PTC_MODE_EB, 1)
+#undef SERIES
+        return hipErrorInvalidValue;
     }
     // FP64: the general geometry only -- without its divisions the FP64
     // kernels (already at 512 registers) spill twice as much and run slower

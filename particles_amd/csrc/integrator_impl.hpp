@@ -246,8 +246,12 @@ __device__ __forceinline__ void rhs(const DeviceFields &F, const StepParams &P, 
 // TIME = 1: the fields are a series of snapshots (F.nsnap, F.stime); the
 // caller sets the stage time with at() before each evaluation, and the
 // fields there are f_k + a (f_{k+1} - f_k), the two bracketing snapshots'
-// trilinear interpolants blended linearly in time (node rows, no cell cache).
-// With TIME = 0, at() does nothing and the stage times are never formed.
+// trilinear interpolants blended linearly in time (node rows).  With the
+// cell cache, snapshot k's 48 corner values sit in registers as for a static
+// field and snapshot k+1's in LDS (lds1: 48 values per lane, value-major,
+// lane-minor, so each lane reads its own column without bank conflicts);
+// the key adds the interval.  With TIME = 0, at() does nothing and the stage
+// times are never formed.
 template <int MODE, int LAYOUT, int CACHE, int PREC, int GEOM = 0, int TIME = 0>
 struct Force {
     using K = std::conditional_t<PREC == PTC_FP32, float, double>;
@@ -257,6 +261,8 @@ struct Force {
     K v[CACHE == 1 ? 48 : 1];
     double ts = 0.0;  // TIME: stage time of the next evaluation
     int kh = 0;       // TIME: the lane's last snapshot interval, in [0, nsnap - 2]
+    int kc = -1;      // TIME with the cache: 2 k + (snapshot k+1 loaded) of the cached cell
+    K *lds1 = nullptr;  // TIME with the cache: this block's snapshot-k+1 columns
 
     __device__ __forceinline__ Force(const DeviceFields &F_, const StepParams &P_) : F(F_), P(P_) {}
 
@@ -289,17 +295,84 @@ struct Force {
         return k;
     }
 
+    // TIME with the cache: loads the corner rows of snapshot k (registers)
+    // and, if two, of snapshot k+1 (LDS), unless the cell and interval are
+    // cached.  A pair cached for interval k also serves snapshot k alone.
+    __device__ __forceinline__ void fill_pair(const uint32_t *o, int k, bool two)
+    {
+        const int key = 2 * k + (two ? 1 : 0);
+        if (o[0] == key0 && o[7] == key7 && (key == kc || (!two && key + 1 == kc))) return;
+        const int bd = (int)blockDim.x, tid = (int)threadIdx.x;
+        if constexpr (PREC == PTC_FP32) {
+            const float *__restrict__ d = (const float *)F.data + (size_t)k * F.sstride;
+#pragma unroll
+            for (int j = 0; j < 8; j++) load_node_f32(d, o[j], &v[6 * j]);
+            if (two) {
+#pragma unroll
+                for (int j = 0; j < 8; j++) {
+                    float w[6];
+                    load_node_f32(d + F.sstride, o[j], w);
+#pragma unroll
+                    for (int i = 0; i < 6; i++) lds1[(6 * j + i) * bd + tid] = w[i];
+                }
+            }
+        } else {
+            const double *__restrict__ d = (const double *)F.data + (size_t)k * F.sstride;
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                const double2 *__restrict__ q = (const double2 *)(d + (size_t)o[j] * 6);
+                const double2 x0 = q[0], x1 = q[1], x2 = q[2];
+                v[6 * j + 0] = x0.x; v[6 * j + 1] = x0.y; v[6 * j + 2] = x1.x;
+                v[6 * j + 3] = x1.y; v[6 * j + 4] = x2.x; v[6 * j + 5] = x2.y;
+            }
+            if (two) {
+#pragma unroll
+                for (int j = 0; j < 8; j++) {
+                    const double2 *__restrict__ q =
+                        (const double2 *)(d + F.sstride + (size_t)o[j] * 6);
+                    const double2 x0 = q[0], x1 = q[1], x2 = q[2];
+                    lds1[(6 * j + 0) * bd + tid] = x0.x;
+                    lds1[(6 * j + 1) * bd + tid] = x0.y;
+                    lds1[(6 * j + 2) * bd + tid] = x1.x;
+                    lds1[(6 * j + 3) * bd + tid] = x1.y;
+                    lds1[(6 * j + 4) * bd + tid] = x2.x;
+                    lds1[(6 * j + 5) * bd + tid] = x2.y;
+                }
+            }
+        }
+        key0 = o[0];
+        key7 = o[7];
+        kc = key;
+    }
+
     // TIME: the interpolated fields (U or E, B) at s and the stage time
     __device__ __forceinline__ void fields_at(const double *s, K *f)
     {
-        static_assert(TIME == 1 && CACHE == 0 && LAYOUT == LAYOUT_AOS,
-                      "a snapshot series is read as node rows without the cell cache");
+        static_assert(TIME == 1 && LAYOUT == LAYOUT_AOS, "a snapshot series is read as node rows");
         CornersT<K> c;
         corners<K, GEOM>(P, s, c);
         double a;
         bool two;
         const int k = interval(a, two);
-        if constexpr (PREC == PTC_FP32) {
+        if constexpr (CACHE == 1) {
+            fill_pair(c.o, k, two);
+#pragma unroll
+            for (int j = 0; j < 8; j++) accumulate_row(j, c.w[j], &v[6 * j], f);
+            if (two) {
+                const int bd = (int)blockDim.x, tid = (int)threadIdx.x;
+                K g[6];
+#pragma unroll
+                for (int j = 0; j < 8; j++) {
+                    K w[6];
+#pragma unroll
+                    for (int i = 0; i < 6; i++) w[i] = lds1[(6 * j + i) * bd + tid];
+                    accumulate_row(j, c.w[j], w, g);
+                }
+                const K ak = K(a);
+#pragma unroll
+                for (int i = 0; i < 6; i++) f[i] = f[i] + ak * (g[i] - f[i]);
+            }
+        } else if constexpr (PREC == PTC_FP32) {
             const float *__restrict__ d = (const float *)F.data + (size_t)k * F.sstride;
 #pragma unroll
             for (int j = 0; j < 8; j++) {
@@ -931,7 +1004,7 @@ integrate_kernel(const DeviceFields F, const StepParams P, const DeviceParticles
 {
     static_assert(CTRL == CTRL_PI || (CACHE == 1 && LAYOUT == LAYOUT_AOS),
                   "step-size features read the cell cache");
-    static_assert(TIME == 0 || (CTRL == CTRL_PI && CACHE == 0 && LAYOUT == LAYOUT_AOS),
+    static_assert(TIME == 0 || (CTRL == CTRL_PI && LAYOUT == LAYOUT_AOS),
                   "a snapshot series runs with the PI controller on node rows");
     // the power's lookup tables (5 KB) in LDS: per-lane indices make every
     // lookup a divergent gather, which LDS serves far faster than memory
@@ -950,6 +1023,7 @@ integrate_kernel(const DeviceFields F, const StepParams P, const DeviceParticles
 
     Force<MODE, LAYOUT, CACHE, PREC, GEOM, TIME> force(F, P);
     using K = typename decltype(force)::K;
+    if constexpr (TIME == 1 && CACHE == 1) force.lds1 = (K *)s_dyn;
     const int lane = (int)__lane_id();
     const double t_end = t_stop < P.tmx ? t_stop : P.tmx;
     const int64_t N = S.np;
@@ -1319,6 +1393,25 @@ __global__ void probe_kernel(const DeviceFields F, const StepParams P, int op, i
     }
 }
 
+// dynamic LDS of a cached series kernel: snapshot k+1's 48 corner values
+// per thread
+inline size_t series_cache_lds(int prec, int block)
+{
+    return (size_t)48 * (prec == PTC_FP32 ? sizeof(float) : sizeof(double)) * (size_t)block;
+}
+
+// whether the device gives one block that much dynamic LDS (next to the
+// kernel's 5 KB of static tables)
+inline bool series_cache_fits(int prec, int block)
+{
+    int dev = 0, max_lds = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return false;
+    if (hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) !=
+        hipSuccess)
+        return false;
+    return series_cache_lds(prec, block) + 8192 <= (size_t)max_lds;
+}
+
 template <int MODE, int LAYOUT, int PREC = PTC_FP64>
 hipError_t launch_probe_t(const DeviceFields &f, const StepParams &sp, int op, int64_t n,
                           const double *in, double *out, hipStream_t stream)
@@ -1334,7 +1427,14 @@ hipError_t launch_t(const DeviceFields &f, const StepParams &sp, const DevicePar
                     double t_stop, LaunchConfig lc, hipStream_t stream)
 {
     auto kern = integrate_kernel<METHOD, MODE, LAYOUT, CACHE, PREC, CTRL, GEOM, TIME>;
-    const size_t lds = CTRL == CTRL_LEARNED ? kNetLds + (size_t)(lc.block / 64) * sizeof(MlpWaveBuf) : 0;
+    const size_t lds = CTRL == CTRL_LEARNED ? kNetLds + (size_t)(lc.block / 64) * sizeof(MlpWaveBuf)
+                       : TIME == 1 && CACHE == 1 ? series_cache_lds(PREC, lc.block)
+                                                 : 0;
+    if (lds > 65536) {
+        hipError_t e = hipFuncSetAttribute((const void *)kern,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
     if (CTRL == CTRL_LEARNED && !p.net) return hipErrorInvalidValue;
     int grid = lc.grid_blocks;
     if (grid <= 0) {

@@ -15,15 +15,18 @@ hipError_t launch_integrate_mixed(int method, const DeviceFields &f, const StepP
     if (f.prec != PTC_FP32 || f.layout != LAYOUT_AOS) return hipErrorInvalidValue;
     const int geom = sp.pow2 ? 1 : 0;
     if (f.nsnap > 0) {
-        // snapshot series: PI controller, no cell cache
+        // snapshot series: PI controller; the cell cache keeps the second
+        // snapshot in LDS where the device has room for it
         if (lc.ctrl != CTRL_PI) return hipErrorInvalidValue;
-#define SERIES(M, MO, G)                                                       \
-    if (method == M && f.mode == MO && geom == G)                              \
-        return launch_t<M, MO, LAYOUT_AOS, 0, PTC_FP32, CTRL_PI, G, 1>(f, sp, p, t_stop, lc, stream);
-        SERIES(PTC_RK5, PTC_MODE_VB, 0) SERIES(PTC_RK5, PTC_MODE_EB, 0)
-        SERIES(PTC_DOP853, PTC_MODE_VB, 0) SERIES(PTC_DOP853, PTC_MODE_EB, 0)
-        SERIES(PTC_RK5, PTC_MODE_VB, 1) SERIES(PTC_RK5, PTC_MODE_EB, 1)
-        SERIES(PTC_DOP853, PTC_MODE_VB, 1) SERIES(PTC_DOP853, PTC_MODE_EB, 1)
+        const int c = lc.cell_cache == 1 && series_cache_fits(PTC_FP32, lc.block) ? 1 : 0;
+#define SERIES(M, MO, C, G)                                                    \
+    if (method == M && f.mode == MO && c == C && geom == G)                    \
+        return launch_t<M, MO, LAYOUT_AOS, C, PTC_FP32, CTRL_PI, G, 1>(f, sp, p, t_stop, lc, stream);
+#define SERIES_MODES(C, G)                                                     \
+    SERIES(PTC_RK5, PTC_MODE_VB, C, G) SERIES(PTC_RK5, PTC_MODE_EB, C, G)      \
+    SERIES(PTC_DOP853, PTC_MODE_VB, C, G) SERIES(PTC_DOP853, PTC_MODE_EB, C, G)
+        SERIES_MODES(0, 0) SERIES_MODES(0, 1) SERIES_MODES(1, 0) SERIES_MODES(1, 1)
+#undef SERIES_MODES
 #undef SERIES
         return hipErrorInvalidValue;
     }

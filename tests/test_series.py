@@ -198,9 +198,11 @@ def device_series(P, uu, bb, times, mode="vb", prec="fp64"):
 @pytest.mark.gpu
 @pytest.mark.parametrize("method", ["rk5", "dop853"])
 @pytest.mark.parametrize("mode", ["vb", "eb"])
-def test_device_series_matches_oracle(oracle, method, mode):
+@pytest.mark.parametrize("cache", [0, 1])
+def test_device_series_matches_oracle(oracle, method, mode, cache):
     """Three propagating turbulent snapshots; the run goes past the last
-    snapshot time (held constant there)."""
+    snapshot time (held constant there).  With the cell cache (second
+    snapshot in LDS) also at 64 threads per block."""
     import particles_amd as P
     tt = TIMES[:3]
     uu, bb = W.turbulent_vb_series(24, seed=11, times=tt, speed=0.02, nmodes=128)
@@ -209,15 +211,26 @@ def test_device_series_matches_oracle(oracle, method, mode):
     p = W.params_vector(tmax=8 * W.T0, dtmax=1.0, atol=1e-8, rtol=1e-8)
     tg = W.time_grid(8, p[2])
     st0 = W.initial_state(256, len(tg), seed=12, gamma_range=(1.0, 1e3))
-    say(f"[series] {method} {mode} 24^3 x3 np=256 t=8 T0")
+    say(f"[series] {method} {mode} cache {cache} 24^3 x3 np=256 t=8 T0")
     so, co, sto = oracle.integrate_series(method, uu, bb, tt, p, tg, st0, mode=mode)
-    with device_series(P, uu, bb, tt, mode=mode) as f:
-        st, cnt, status = P.integrate(f, method, p, tg, st0)
-    assert (status == sto).all() and (status == 0).all()
-    same = np.array([np.array_equal(st[..., i], so[..., i]) and np.array_equal(cnt[:, i], co[:, i])
-                     for i in range(st.shape[2])])
-    say(f"  bit-exact {same.sum()}/{len(same)}; accepted {cnt[1].sum()} vs {co[1].sum()}")
-    assert same.all()
+    old = P.get_option("cell_cache"), P.get_option("block")
+    try:
+        P.set_option("cell_cache", cache)
+        runs = []
+        with device_series(P, uu, bb, tt, mode=mode) as f:
+            for block in ((256, 64) if cache else (256,)):
+                P.set_option("block", block)
+                runs.append((block, P.integrate(f, method, p, tg, st0)))
+    finally:
+        P.set_option("cell_cache", old[0])
+        P.set_option("block", old[1])
+    for block, (st, cnt, status) in runs:
+        assert (status == sto).all() and (status == 0).all()
+        same = np.array([np.array_equal(st[..., i], so[..., i]) and
+                         np.array_equal(cnt[:, i], co[:, i]) for i in range(st.shape[2])])
+        say(f"  block {block}: bit-exact {same.sum()}/{len(same)}; accepted {cnt[1].sum()} "
+            f"vs {co[1].sum()}")
+        assert same.all()
 
 
 @pytest.mark.gpu
