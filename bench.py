@@ -25,7 +25,8 @@ HIP events on the batch's stream, against the MI355X HBM peak.
 With `--snapshots N` the field is time-dependent: N device-built snapshots
 of the same turbulent recipe, its modes travelling in time, spread over the
 benchmarked time and blended linearly in time at every stage; each RHS then
-reads two snapshots (768 B per evaluation in FP64).  The reference has no
+reads two snapshots (768 B per evaluation in FP64), the second kept in LDS
+by the cell cache.  The reference has no
 time-dependent field, so no CPU baseline or variants are run for it.
 
 `cpu_baseline` times the reference's own FP64 OpenMP integrator (compiled
@@ -78,15 +79,15 @@ def build_workload(args):
     if args.snapshots > 0:
         if recipe != "turbulent":
             raise SystemExit("--snapshots applies to the turbulent workloads")
-        # a snapshot series spanning the benchmarked time, every mode
+        # a snapshot series every `snapshot_every` T0 from t = 0, every mode
         # travelling at 0.01 box lengths per unit time, built on the device
-        times = np.linspace(0.0, (args.warmup + args.steps) * W.T0, args.snapshots)
+        times = np.arange(args.snapshots) * args.snapshot_every * W.T0
 
         def make_fields(prec):
             return W.turbulent_vb_series_device(dm, seed=args.seed, times=times, prec=prec,
                                                 speed=0.01)
         desc = (f"{dm}^3 VB turbulent field (B0=1, dB rms 0.5, V rms 0.1, 256 modes) as "
-                f"{args.snapshots} snapshots over {args.warmup + args.steps} T0 (modes "
+                f"{args.snapshots} snapshots {args.snapshot_every:g} T0 apart (modes "
                 "travelling at 0.01 per unit time), blended linearly in time at each stage, "
                 "built on the device")
     elif recipe == "turbulent" and not device:
@@ -112,9 +113,10 @@ def build_workload(args):
                        f"step = advance all particles by 1 gyro-period of B0",
            "grid": dm, "particles": npart, "method": method, "tol": tol,
            "snapshots_every_T0": tmax / W.T0 / nsnap, "layout": args.layout,
-           "cell_cache": args.cell_cache if args.snapshots == 0 else 0,
-           "field_precision": args.prec, "sort": args.sort, "order": args.order,
-           "field_snapshots": args.snapshots}
+           "cell_cache": args.cell_cache, "field_precision": args.prec, "sort": args.sort,
+           "order": args.order, "field_snapshots": args.snapshots}
+    if args.snapshots > 0:
+        cfg["snapshot_every_T0"] = args.snapshot_every
     return make_fields, uu, bb, p, tg, st0, method, cfg
 
 
@@ -154,21 +156,24 @@ def cpu_baseline(uu, bb, p, tg, st0, method, target_s):
 
 
 def traffic_from_profile(cfg):
-    """HBM bytes per integrator launch of this workload from the committed
-    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (scaled by a copy-kernel
-    calibration), or None when no summary matches the configuration."""
-    path = os.path.join(ROOT, "profiles", "integrate_dop853_turbulent256.json")
-    try:
-        with open(path) as fh:
-            prof = json.load(fh)
-    except (OSError, ValueError):
-        return None
+    """HBM bytes per integrator launch of this workload from a committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (profiles/*.json, scaled by a
+    copy-kernel calibration), or None when no summary matches the
+    configuration."""
+    import glob
     keys = ("grid", "particles", "method", "tol", "layout", "cell_cache", "field_precision",
-            "sort", "order", "field_snapshots")
-    if any(prof.get("config", {}).get(k, 0 if k == "field_snapshots" else None) != cfg.get(k)
-           for k in keys):
-        return None
-    return prof["hbm_bytes_per_launch"]
+            "sort", "order", "field_snapshots", "snapshot_every_T0")
+    defaults = {"field_snapshots": 0}
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.json"))):
+        try:
+            with open(path) as fh:
+                prof = json.load(fh)
+        except (OSError, ValueError):
+            continue
+        pc = prof.get("config", {})
+        if all(pc.get(k, defaults.get(k)) == cfg.get(k, defaults.get(k)) for k in keys):
+            return prof["hbm_bytes_per_launch"]
+    return None
 
 
 def run_variant(fields, method, p, tg, st, steps, warmup, net=None, shift=0.0):
@@ -310,10 +315,24 @@ def main():
     ap.add_argument("--train-gamma", type=float, nargs=2, default=(1.0, 1.0e3))
     ap.add_argument("--train-steps", type=int, default=3000)
     ap.add_argument("--snapshots", type=int, default=0,
-                    help="time-dependent field: this many snapshots over the run (0 = static)")
+                    help="time-dependent field: this many snapshots from t = 0 (0 = static)")
+    ap.add_argument("--snapshot-every", type=float, default=2.0,
+                    help="time-dependent field: snapshot spacing in gyro-periods of B0")
     ap.add_argument("--shift", type=float, default=None,
                     help="added to the network's log step factor (default: calibrated)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only copy a 1 GiB float64 device tensor three times (the known-size "
+                         "copy that scales the rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        a = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print("copied 3 x 1 GiB", flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}): the particle batch would "
               "shard by contiguous particle ranges with the fields replicated, and no such run "
