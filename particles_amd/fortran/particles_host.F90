@@ -19,6 +19,11 @@
 !!     particles.F90:365-371.
 !! Extra command-line switches: --mixed (FP32 fields, mixed precision),
 !! --dry-run (stop before the integration; needs no GPU).
+!! Extra keys (the reference's field is static): field_snapshots = N > 1
+!! reads N field snapshots, snapshot s from <name>_<ssss>.fits[.gz]
+!! (s = 0001 ..), at times (s - 1) * field_dt in the unit of tmax, blended
+!! linearly in time at every stage (ptc_fields_series); default 1, the
+!! reference's single static field.
 !! Differences on purpose: "zmax" sets zmax (the reference stores it into
 !! zmax_domain, particles.F90:149); an odd number of particles gets normal
 !! momenta too (the reference's split of the array needs an even count);
@@ -173,7 +178,12 @@ program particles_host
 
   character(len=512) :: pfile = './params.in', arg, path
   logical  :: ok, dry_run = .false., mixed = .false., exists
-  integer  :: i, k, iret, nargs, naxis, bitpix, imethod, nbad
+  integer  :: i, k, iret, nargs, naxis, bitpix, imethod, nbad, s
+  integer  :: nfs = 1
+  real(8)  :: fdt = 0.0d0
+  character(len=64) :: fbase(6)
+  type(c_ptr), allocatable :: snaps(:)
+  real(8), allocatable :: stimes(:)
   integer(c_int64_t) :: dims(8), nn, n
   integer(8) :: t1, t2, rate
   integer(c_int) :: dm(3)
@@ -247,6 +257,12 @@ program particles_host
   do k = 1, 6
     call get_param(trim(fkeys(k)), fnames(k))
   end do
+  call get_param('field_snapshots', nfs)
+  call get_param('field_dt', fdt)
+  if (nfs < 1 .or. (nfs > 1 .and. .not. fdt > 0.0d0)) then
+    write(*, '(a)') ' field_snapshots must be >= 1, and field_dt > 0 for more than one'
+    stop 1
+  end if
 
   ! the seeding region lies inside the domain
   xmin = max(xmin, xmin_domain)
@@ -309,28 +325,19 @@ program particles_host
   write(*, "(3x,'facmin facmax = ',2es13.5)") facmin, facmax
   write(*, "(3x,'safe beta     = ',2es13.5)") safe, beta
   write(*, "(3x,'precision     = ',a)") merge('mixed (FP32 fields)', 'FP64               ', mixed)
+  if (nfs > 1) write(*, "(3x,'field snaps   = ',i0,' every ',es13.5)") nfs, fdt
   write(*, '(a)') ' Output:'
   write(*, "(3x,'time scale    = ',a)") trim(tscale)
   write(*, "(3x,'nsteps        = ',i0)") nsteps
   write(*, "(3x,'tmin tmax     = ',2es13.5)") tmin, tmax
 
-  ! ---- fields (velx .. magz in datadir, .fits or .fits.gz) ----
+  ! ---- fields (velx .. magz in datadir, .fits or .fits.gz; snapshot s of
+  !      a series in <name>_<ssss>.fits[.gz]) ----
   if (len_trim(datadir) > 0) then
     if (datadir(len_trim(datadir):len_trim(datadir)) /= '/') datadir = trim(datadir) // '/'
   end if
-  do k = 1, 6
-    path = trim(datadir) // trim(fnames(k)) // '.fits'
-    inquire(file = trim(path), exist = exists)
-    if (.not. exists) then
-      path = trim(path) // '.gz'
-      inquire(file = trim(path), exist = exists)
-    end if
-    if (.not. exists) then
-      write(*, '(a)') ' missing field file ' // trim(datadir) // trim(fnames(k)) // '.fits[.gz]'
-      stop 1
-    end if
-    fnames(k) = trim(path)
-  end do
+  fbase = fnames
+  call field_paths(1)
   iret = ptc_fits_dims(trim(fnames(1)) // c_null_char, naxis, dims, bitpix)
   if (iret /= 0 .or. naxis /= 3) then
     write(*, '(a)') ' cannot read the field dimensions: ' // ptc_fits_last_error()
@@ -339,18 +346,20 @@ program particles_host
   dm = int(dims(1:3), c_int)
   allocate(uu(dm(1), dm(2), dm(3), 3), bb(dm(1), dm(2), dm(3), 3))
   nn = int(dm(1), c_int64_t) * dm(2) * dm(3)
-  do k = 1, 3
-    iret = ptc_fits_read(trim(fnames(k)) // c_null_char, nn, uu(:,:,:,k))
-    if (iret == 0) iret = ptc_fits_read(trim(fnames(k + 3)) // c_null_char, nn, bb(:,:,:,k))
-    if (iret /= 0) then
-      write(*, '(a)') ' cannot read a field: ' // ptc_fits_last_error()
-      stop 1
-    end if
+  do s = 2, nfs
+    ! every snapshot present and of the first one's shape
+    call field_paths(s)
+    do k = 1, 6
+      iret = ptc_fits_dims(trim(fnames(k)) // c_null_char, naxis, dims, bitpix)
+      if (iret /= 0 .or. naxis /= 3 .or. any(int(dims(1:3), c_int) /= dm)) then
+        write(*, '(a)') ' field snapshot ' // trim(fnames(k)) // ' does not match the first'
+        stop 1
+      end if
+    end do
   end do
-  uu = vunit * uu
-  bb = bunit * bb
+  call read_snapshot(1)
   write(*, "(' Field data     = ',i0,' x ',i0,' x ',i0,', ',f10.3,' GB')") dm, &
-       16.0d0 * 3 * real(nn, 8) / 1024.0d0**3
+       16.0d0 * 3 * real(nn, 8) * nfs / 1024.0d0**3
 
   ! ---- snapshot times ----
   allocate(time(nsteps), counter(3, nparticles), status(nparticles))
@@ -421,10 +430,33 @@ contains
   ! ---- integration on the GPU ----
   write(*, '(a)') ' Integrating trajectories...'
   call system_clock(t1)
-  iret = ptc_fields_create(dm, uu, bb, PTC_MODE_VB, merge(PTC_FP32, PTC_FP64, mixed), fh)
-  if (iret /= PTC_OK) then
-    write(*, '(a)') ' ptc_fields_create: ' // ptc_last_error()
-    stop 2
+  if (nfs == 1) then
+    iret = ptc_fields_create(dm, uu, bb, PTC_MODE_VB, merge(PTC_FP32, PTC_FP64, mixed), fh)
+    if (iret /= PTC_OK) then
+      write(*, '(a)') ' ptc_fields_create: ' // ptc_last_error()
+      stop 2
+    end if
+  else
+    ! one snapshot on the host at a time: read, upload, next; then the
+    ! series copies them into one block and the single handles go
+    allocate(snaps(nfs), stimes(nfs))
+    do s = 1, nfs
+      if (s > 1) call read_snapshot(s)
+      stimes(s) = (s - 1) * fdt
+      iret = ptc_fields_create(dm, uu, bb, PTC_MODE_VB, merge(PTC_FP32, PTC_FP64, mixed), snaps(s))
+      if (iret /= PTC_OK) then
+        write(*, '(a)') ' ptc_fields_create: ' // ptc_last_error()
+        stop 2
+      end if
+    end do
+    iret = ptc_fields_series(snaps, stimes, int(nfs, c_int), fh)
+    if (iret /= PTC_OK) then
+      write(*, '(a)') ' ptc_fields_series: ' // ptc_last_error()
+      stop 2
+    end if
+    do s = 1, nfs
+      iret = ptc_fields_destroy(snaps(s))
+    end do
   end if
   iret = ptc_integrate(fh, imethod, int(nparticles, c_int64_t), nsteps, params, size(params), &
                        time, state, counter, c_loc(status))
@@ -447,6 +479,48 @@ contains
                  int(nparticles, c_int64_t)], state, 'TEMP', temp, 'Initial thermal temperature')
   call system_clock(t2)
   write(*, "(' Data stored in ',f10.4,' seconds.')") real(t2 - t1, 8) / rate
+  end subroutine
+
+  subroutine field_paths(snap)
+  ! fnames = the six files of snapshot `snap` (the plain names for a static
+  ! field), .fits or else .fits.gz
+  integer, intent(in) :: snap
+  character(len=16) :: suffix
+  integer :: j
+  logical :: there
+  suffix = ''
+  if (nfs > 1) write(suffix, "('_',i4.4)") snap
+  do j = 1, 6
+    path = trim(datadir) // trim(fbase(j)) // trim(suffix) // '.fits'
+    inquire(file = trim(path), exist = there)
+    if (.not. there) then
+      path = trim(path) // '.gz'
+      inquire(file = trim(path), exist = there)
+    end if
+    if (.not. there) then
+      write(*, '(a)') ' missing field file ' // trim(datadir) // trim(fbase(j)) // trim(suffix) &
+                      // '.fits[.gz]'
+      stop 1
+    end if
+    fnames(j) = trim(path)
+  end do
+  end subroutine
+
+  subroutine read_snapshot(snap)
+  ! uu, bb = snapshot `snap` in code units (fields.F90:63-74)
+  integer, intent(in) :: snap
+  integer :: j
+  call field_paths(snap)
+  do j = 1, 3
+    iret = ptc_fits_read(trim(fnames(j)) // c_null_char, nn, uu(:,:,:,j))
+    if (iret == 0) iret = ptc_fits_read(trim(fnames(j + 3)) // c_null_char, nn, bb(:,:,:,j))
+    if (iret /= 0) then
+      write(*, '(a)') ' cannot read a field: ' // ptc_fits_last_error()
+      stop 1
+    end if
+  end do
+  uu = vunit * uu
+  bb = bunit * bb
   end subroutine
 
   function ptc_version_string() result(v)

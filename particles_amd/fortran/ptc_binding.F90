@@ -22,6 +22,7 @@ module ptc_binding
   integer(c_int), parameter :: PTC_RK5 = 0, PTC_DOP853 = 1
   integer(c_int), parameter :: PTC_MODE_VB = 0, PTC_MODE_EB = 1
   integer(c_int), parameter :: PTC_FP64 = 0, PTC_FP32 = 1
+  integer(c_int), parameter :: PTC_INIT_MAXWELL = 0, PTC_INIT_GAMMA = 1, PTC_INIT_GAMMA_LOG = 2
 
   interface
 
@@ -36,6 +37,15 @@ module ptc_binding
     integer(c_int) function ptc_fields_destroy(f) bind(C, name="ptc_fields_destroy")
       import :: c_int, c_ptr
       type(c_ptr), value :: f
+    end function
+
+    ! time-dependent fields: n snapshot handles at strictly increasing times
+    integer(c_int) function ptc_fields_series(snaps, times, n, out) bind(C, name="ptc_fields_series")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), intent(in)    :: snaps(*)
+      real(c_double), intent(in) :: times(*)
+      integer(c_int), value      :: n
+      type(c_ptr), intent(out)   :: out
     end function
 
     integer(c_int) function ptc_integrate(f, method, np, m, params, nparams, time, state, counter, &
@@ -75,6 +85,43 @@ module ptc_binding
     integer(c_int) function ptc_particles_destroy(p) bind(C, name="ptc_particles_destroy")
       import :: c_int, c_ptr
       type(c_ptr), value :: p
+    end function
+
+    integer(c_int) function ptc_particles_generate(p, kind, seed, lo, hi, a, b) &
+                                                   bind(C, name="ptc_particles_generate")
+      import :: c_int, c_int64_t, c_double, c_ptr
+      type(c_ptr), value         :: p
+      integer(c_int), value      :: kind
+      integer(c_int64_t), value  :: seed
+      real(c_double), intent(in) :: lo(3), hi(3)
+      real(c_double), value      :: a, b
+    end function
+
+    integer(c_int) function ptc_particles_moments(p, moments) bind(C, name="ptc_particles_moments")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value          :: p
+      real(c_double), intent(out) :: moments(24)
+    end function
+
+    integer(c_int) function ptc_particles_advance_graph(p, n, t_stop) &
+                                                        bind(C, name="ptc_particles_advance_graph")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value         :: p
+      integer(c_int), value      :: n
+      real(c_double), intent(in) :: t_stop(*)
+    end function
+
+    integer(c_int) function ptc_particles_autotune(p, t_end, chosen) bind(C, name="ptc_particles_autotune")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value           :: p
+      real(c_double), value        :: t_end
+      integer(c_int), intent(out)  :: chosen(3)
+    end function
+
+    integer(c_int) function ptc_particles_totals(p, totals) bind(C, name="ptc_particles_totals")
+      import :: c_int, c_int64_t, c_ptr
+      type(c_ptr), value              :: p
+      integer(c_int64_t), intent(out) :: totals(5)
     end function
 
     integer(c_int) function ptc_set_option(key, value) bind(C, name="ptc_set_option")

@@ -4,7 +4,9 @@ files around the device integrator.  CPU: a dry run (no GPU) checks the
 parsed parameters and derived quantities against the reference's formulas
 (sources/particles.F90:163-214,240-280,317-333).  GPU: a full run writes
 counter.fits, time.fits and state.fits, and its snapshots equal the device
-integrator's on the initial states it wrote."""
+integrator's on the initial states it wrote; with field_snapshots = 3 (a
+time-dependent field, no counterpart in the reference) they equal the
+library's series integration of the same snapshots."""
 import os
 import re
 import subprocess
@@ -137,6 +139,81 @@ def test_full_run_writes_reference_files(tmp_path):
     st0[6:, 0, :] = 0.0
     with P.Fields(vunit * uu, 1.0 * bb) as f:
         st, cnt, status = P.integrate(f, "dop853", p, time, st0)
+    assert (status == 0).all()
+    assert np.array_equal(st[:, 1:, :], state[:, 1:, :])
+    assert np.array_equal(cnt.astype(np.float64), counter)
+
+
+# ---- field snapshot series (field_snapshots / field_dt) ----
+
+SERIES_KEYS = "field_snapshots = 3\nfield_dt = 4.0\n"
+
+
+def _setup_series(tmp, params_text, dm=12):
+    times = np.arange(3) * 4.0
+    uu, bb = W.turbulent_vb_series(dm, seed=6, times=times, speed=0.05, nmodes=24)
+    data = os.path.join(tmp, "data")
+    os.makedirs(data, exist_ok=True)
+    names = ("velx", "vely", "velz", "magx", "magy", "magz")
+    for s in range(3):
+        for k, n in enumerate(names):
+            arr = uu[..., k, s] if k < 3 else bb[..., k - 3, s]
+            fits.write_f64(os.path.join(data, f"{n}_{s + 1:04d}.fits"), np.asfortranarray(arr))
+    with open(os.path.join(tmp, "params.in"), "w") as fh:
+        fh.write(params_text)
+    return uu, bb, times
+
+
+def test_dry_run_series_checks_every_snapshot(tmp_path):
+    tmp = str(tmp_path)
+    _setup_series(tmp, PARAMS + SERIES_KEYS)
+    r = _run(tmp, "--dry-run")
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert re.search(r"field snaps\s*=\s*3 every\s+4\.0", r.stdout)
+    # a missing file of the third snapshot
+    os.remove(os.path.join(tmp, "data", "magy_0003.fits"))
+    r = _run(tmp, "--dry-run")
+    assert r.returncode != 0 and "missing field file data/magy_0003.fits" in r.stdout
+    # a snapshot of another shape
+    fits.write_f64(os.path.join(tmp, "data", "magy_0003.fits"), np.zeros((12, 12, 6), order="F"))
+    r = _run(tmp, "--dry-run")
+    assert r.returncode != 0 and "does not match the first" in r.stdout
+    # several snapshots need a spacing
+    with open(os.path.join(tmp, "params.in"), "w") as fh:
+        fh.write(PARAMS + "field_snapshots = 3\n")
+    r = _run(tmp, "--dry-run")
+    assert r.returncode != 0 and "field_dt > 0" in r.stdout
+
+
+@pytest.mark.gpu
+def test_full_run_with_snapshot_series(tmp_path):
+    import particles_amd as P
+    tmp = str(tmp_path)
+    text = PARAMS.replace("particle_type = electron", "particle_type = proton").replace(
+        "tunit       = 2.0e-6", "tunit       = 1.0e-4").replace(
+        "nparticles  = 9", "nparticles  = 64") + SERIES_KEYS
+    uu, bb, times = _setup_series(tmp, text)
+    r = _run(tmp)
+    print(r.stdout[-600:])
+    assert r.returncode == 0, r.stdout + r.stderr
+    state = fits.read(os.path.join(tmp, "state.fits"))
+    counter = fits.read(os.path.join(tmp, "counter.fits"))
+    time = fits.read(os.path.join(tmp, "time.fits"))
+    vunit = 3.0e5 / C_LIGHT
+    qom = 1.0e-4 * QE * 1.0e-4 / MP
+    p = W.params_vector(qom=qom, vun=vunit, tmax=10.0, dtmax=0.5, atol=1e-9, rtol=1e-9,
+                        xl=(-0.5, 0, 0), xs=(1.0, 1.0, 1.0))
+    st0 = np.zeros_like(state, order="F")
+    st0[:, 0, :] = state[:, 0, :]
+    st0[6:, 0, :] = 0.0
+    snaps = [P.Fields(np.asfortranarray(vunit * uu[..., s]), np.asfortranarray(bb[..., s]))
+             for s in range(3)]
+    try:
+        with P.Fields.series(snaps, times) as f:
+            st, cnt, status = P.integrate(f, "dop853", p, time, st0)
+    finally:
+        for g in snaps:
+            g.close()
     assert (status == 0).all()
     assert np.array_equal(st[:, 1:, :], state[:, 1:, :])
     assert np.array_equal(cnt.astype(np.float64), counter)
