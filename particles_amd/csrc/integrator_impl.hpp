@@ -13,6 +13,12 @@ namespace ptc {
 namespace {
 
 constexpr int kBlock = 256;
+#ifndef SERIES_F32_REGS
+#define SERIES_F32_REGS 1
+#endif
+#ifndef SERIES_F64_REGS
+#define SERIES_F64_REGS 0
+#endif
 
 // ---------------------------------------------------------------------------
 // trilinear gather, sources/interpolations.F90:104-226
@@ -248,9 +254,12 @@ __device__ __forceinline__ void rhs(const DeviceFields &F, const StepParams &P, 
 // fields there are f_k + a (f_{k+1} - f_k), the two bracketing snapshots'
 // trilinear interpolants blended linearly in time (node rows).  With the
 // cell cache, snapshot k's 48 corner values sit in registers as for a static
-// field and snapshot k+1's in LDS (lds1: 48 values per lane, value-major,
-// lane-minor, so each lane reads its own column without bank conflicts);
-// the key adds the interval.  With TIME = 0, at() does nothing and the stage
+// field and snapshot k+1's in LDS (FP64) or registers too (FP32): in LDS
+// each wave owns a 48 x 64 slab, value j of lane l at slab[64 j + l], so a
+// read of one value by the wave covers 64 consecutive elements (no bank
+// conflicts) and the 48 reads differ by compile-time offsets from one
+// per-lane address (no block size in the address); the key adds the
+// interval.  With TIME = 0, at() does nothing and the stage
 // times are never formed.
 template <int MODE, int LAYOUT, int CACHE, int PREC, int GEOM = 0, int TIME = 0>
 struct Force {
@@ -262,7 +271,12 @@ struct Force {
     double ts = 0.0;  // TIME: stage time of the next evaluation
     int kh = 0;       // TIME: the lane's last snapshot interval, in [0, nsnap - 2]
     int kc = -1;      // TIME with the cache: 2 k + (snapshot k+1 loaded) of the cached cell
-    K *lds1 = nullptr;  // TIME with the cache: this block's snapshot-k+1 columns
+    K *lds1 = nullptr;  // TIME with the cache: this lane's snapshot-k+1 column (stride 64)
+    // TIME with the cache in FP32: snapshot k+1 in registers instead (the
+    // FP32 kernel has room for 48 more floats; FP64 does not)
+    static constexpr bool kSecondInRegs =
+        TIME == 1 && CACHE == 1 && (PREC == PTC_FP32 ? SERIES_F32_REGS : SERIES_F64_REGS);
+    K v1[kSecondInRegs ? 48 : 1];
 
     __device__ __forceinline__ Force(const DeviceFields &F_, const StepParams &P_) : F(F_), P(P_) {}
 
@@ -302,7 +316,6 @@ struct Force {
     {
         const int key = 2 * k + (two ? 1 : 0);
         if (o[0] == key0 && o[7] == key7 && (key == kc || (!two && key + 1 == kc))) return;
-        const int bd = (int)blockDim.x, tid = (int)threadIdx.x;
         if constexpr (PREC == PTC_FP32) {
             const float *__restrict__ d = (const float *)F.data + (size_t)k * F.sstride;
 #pragma unroll
@@ -313,7 +326,10 @@ struct Force {
                     float w[6];
                     load_node_f32(d + F.sstride, o[j], w);
 #pragma unroll
-                    for (int i = 0; i < 6; i++) lds1[(6 * j + i) * bd + tid] = w[i];
+                    for (int i = 0; i < 6; i++) {
+                        if constexpr (kSecondInRegs) v1[6 * j + i] = w[i];
+                        else lds1[(6 * j + i) * 64] = w[i];
+                    }
                 }
             }
         } else {
@@ -331,12 +347,12 @@ struct Force {
                     const double2 *__restrict__ q =
                         (const double2 *)(d + F.sstride + (size_t)o[j] * 6);
                     const double2 x0 = q[0], x1 = q[1], x2 = q[2];
-                    lds1[(6 * j + 0) * bd + tid] = x0.x;
-                    lds1[(6 * j + 1) * bd + tid] = x0.y;
-                    lds1[(6 * j + 2) * bd + tid] = x1.x;
-                    lds1[(6 * j + 3) * bd + tid] = x1.y;
-                    lds1[(6 * j + 4) * bd + tid] = x2.x;
-                    lds1[(6 * j + 5) * bd + tid] = x2.y;
+                    const double w[6] = {x0.x, x0.y, x1.x, x1.y, x2.x, x2.y};
+#pragma unroll
+                    for (int i = 0; i < 6; i++) {
+                        if constexpr (kSecondInRegs) v1[6 * j + i] = w[i];
+                        else lds1[(6 * j + i) * 64] = w[i];
+                    }
                 }
             }
         }
@@ -359,13 +375,15 @@ struct Force {
 #pragma unroll
             for (int j = 0; j < 8; j++) accumulate_row(j, c.w[j], &v[6 * j], f);
             if (two) {
-                const int bd = (int)blockDim.x, tid = (int)threadIdx.x;
                 K g[6];
 #pragma unroll
                 for (int j = 0; j < 8; j++) {
                     K w[6];
 #pragma unroll
-                    for (int i = 0; i < 6; i++) w[i] = lds1[(6 * j + i) * bd + tid];
+                    for (int i = 0; i < 6; i++) {
+                        if constexpr (kSecondInRegs) w[i] = v1[6 * j + i];
+                        else w[i] = lds1[(6 * j + i) * 64];
+                    }
                     accumulate_row(j, c.w[j], w, g);
                 }
                 const K ak = K(a);
@@ -1023,7 +1041,8 @@ integrate_kernel(const DeviceFields F, const StepParams P, const DeviceParticles
 
     Force<MODE, LAYOUT, CACHE, PREC, GEOM, TIME> force(F, P);
     using K = typename decltype(force)::K;
-    if constexpr (TIME == 1 && CACHE == 1) force.lds1 = (K *)s_dyn;
+    if constexpr (TIME == 1 && CACHE == 1)
+        force.lds1 = (K *)s_dyn + (threadIdx.x >> 6) * (48 * 64) + (threadIdx.x & 63);
     const int lane = (int)__lane_id();
     const double t_end = t_stop < P.tmx ? t_stop : P.tmx;
     const int64_t N = S.np;
@@ -1397,7 +1416,8 @@ __global__ void probe_kernel(const DeviceFields F, const StepParams P, int op, i
 // per thread
 inline size_t series_cache_lds(int prec, int block)
 {
-    return (size_t)48 * (prec == PTC_FP32 ? sizeof(float) : sizeof(double)) * (size_t)block;
+    const size_t waves = ((size_t)block + 63) / 64;
+    return (size_t)48 * 64 * (prec == PTC_FP32 ? sizeof(float) : sizeof(double)) * waves;
 }
 
 // whether the device gives one block that much dynamic LDS (next to the
@@ -1428,8 +1448,10 @@ hipError_t launch_t(const DeviceFields &f, const StepParams &sp, const DevicePar
 {
     auto kern = integrate_kernel<METHOD, MODE, LAYOUT, CACHE, PREC, CTRL, GEOM, TIME>;
     const size_t lds = CTRL == CTRL_LEARNED ? kNetLds + (size_t)(lc.block / 64) * sizeof(MlpWaveBuf)
-                       : TIME == 1 && CACHE == 1 ? series_cache_lds(PREC, lc.block)
-                                                 : 0;
+                       : TIME == 1 && CACHE == 1 &&
+                                 !(PREC == PTC_FP32 ? SERIES_F32_REGS : SERIES_F64_REGS)
+                           ? series_cache_lds(PREC, lc.block)
+                           : 0;
     if (lds > 65536) {
         hipError_t e = hipFuncSetAttribute((const void *)kern,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
