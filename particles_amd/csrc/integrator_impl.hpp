@@ -261,13 +261,17 @@ __device__ __forceinline__ void rhs(const DeviceFields &F, const StepParams &P, 
 // per-lane address (no block size in the address); the key adds the
 // interval.  With TIME = 0, at() does nothing and the stage
 // times are never formed.
-template <int MODE, int LAYOUT, int CACHE, int PREC, int GEOM = 0, int TIME = 0>
+// SPLIT > 0 (static FP64 field, PI controller): the last SPLIT of the 48
+// cached values live in LDS (lds1, the per-wave slab layout below) instead
+// of registers.
+template <int MODE, int LAYOUT, int CACHE, int PREC, int GEOM = 0, int TIME = 0, int SPLIT = 0>
 struct Force {
     using K = std::conditional_t<PREC == PTC_FP32, float, double>;
+    static constexpr int kRegVals = CACHE == 1 ? 48 - SPLIT : 1;
     const DeviceFields &F;
     const StepParams &P;
     uint32_t key0 = 0xffffffffu, key7 = 0xffffffffu;
-    K v[CACHE == 1 ? 48 : 1];
+    K v[kRegVals > 0 ? kRegVals : 1];
     double ts = 0.0;  // TIME: stage time of the next evaluation
     int kh = 0;       // TIME: the lane's last snapshot interval, in [0, nsnap - 2]
     int kc = -1;      // TIME with the cache: 2 k + (snapshot k+1 loaded) of the cached cell
@@ -438,12 +442,32 @@ struct Force {
             for (int j = 0; j < 8; j++) {
                 const double2 *__restrict__ q = (const double2 *)(d + (size_t)o[j] * 6);
                 const double2 x0 = q[0], x1 = q[1], x2 = q[2];
-                v[6 * j + 0] = x0.x; v[6 * j + 1] = x0.y; v[6 * j + 2] = x1.x;
-                v[6 * j + 3] = x1.y; v[6 * j + 4] = x2.x; v[6 * j + 5] = x2.y;
+                const double w[6] = {x0.x, x0.y, x1.x, x1.y, x2.x, x2.y};
+#pragma unroll
+                for (int i = 0; i < 6; i++) set_cached(6 * j + i, w[i]);
             }
         }
         key0 = o[0];
         key7 = o[7];
+    }
+
+    // cached value q (0..47): registers, or the LDS slab for the last SPLIT
+    __device__ __forceinline__ K cached(int q) const
+    {
+        if constexpr (SPLIT > 0) {
+            if (q >= kRegVals) return lds1[(q - kRegVals) * 64];
+        }
+        return v[q];
+    }
+    __device__ __forceinline__ void set_cached(int q, K x)
+    {
+        if constexpr (SPLIT > 0) {
+            if (q >= kRegVals) {
+                lds1[(q - kRegVals) * 64] = x;
+                return;
+            }
+        }
+        v[q] = x;
     }
 
     // the RHS at s (bit-identical to operator()), also returning the
@@ -502,7 +526,12 @@ struct Force {
             fill(c.o);
             double f[6];
 #pragma unroll
-            for (int j = 0; j < 8; j++) accumulate_row(j, c.w[j], &v[6 * j], f);
+            for (int j = 0; j < 8; j++) {
+                double w[6];
+#pragma unroll
+                for (int i = 0; i < 6; i++) w[i] = cached(6 * j + i);
+                accumulate_row(j, c.w[j], w, f);
+            }
             force_from_fields<MODE, GEOM>(P, s, f, a);
         }
     }
@@ -1014,8 +1043,21 @@ __device__ __forceinline__ bool record_sampled(int64_t idx, int64_t c, uint64_t 
 // after an accepted step the network proposes the next step, after a
 // rejection the PI rule does).  CTRL_LEARNED takes the network and a 64x72
 // BF16 buffer per wave in dynamic LDS.
+// Cell-cache values the static FP64 PI kernels keep in LDS (per-wave slabs,
+// 64 doubles per value) instead of registers.  DOP853 is at the 512-register
+// limit: with all 48 values in registers it spills 212 B per lane; with the
+// last 16 in LDS it spills 36 B and runs 1.09x faster on the 256^3
+// turbulent workload (8 to 20 measured alike, 32 or more far slower: the
+// sums then wait on too many LDS reads).  RK5 spills 52 B and gains nothing.
+constexpr int static_split(int method, int ctrl, int cache, int prec, int time)
+{
+    return method == PTC_DOP853 && ctrl == CTRL_PI && cache == 1 && prec == PTC_FP64 && time == 0
+               ? 16
+               : 0;
+}
+
 // TIME = 1 integrates in a snapshot series (fields interpolated in time at
-// each stage; PI controller, node rows, no cell cache).
+// each stage; PI controller, node rows, cell cache as described at Force).
 template <int METHOD, int MODE, int LAYOUT, int CACHE, int PREC, int CTRL, int GEOM, int TIME = 0>
 __global__ void __launch_bounds__(kBlock)
 integrate_kernel(const DeviceFields F, const StepParams P, const DeviceParticles S, double t_stop)
@@ -1039,10 +1081,13 @@ integrate_kernel(const DeviceFields F, const StepParams P, const DeviceParticles
     __syncthreads();
     const PowTables T{s_logt, s_expt};
 
-    Force<MODE, LAYOUT, CACHE, PREC, GEOM, TIME> force(F, P);
+    constexpr int kSplit = static_split(METHOD, CTRL, CACHE, PREC, TIME);
+    Force<MODE, LAYOUT, CACHE, PREC, GEOM, TIME, kSplit> force(F, P);
     using K = typename decltype(force)::K;
     if constexpr (TIME == 1 && CACHE == 1)
         force.lds1 = (K *)s_dyn + (threadIdx.x >> 6) * (48 * 64) + (threadIdx.x & 63);
+    if constexpr (kSplit > 0)
+        force.lds1 = (K *)s_dyn + (threadIdx.x >> 6) * (kSplit * 64) + (threadIdx.x & 63);
     const int lane = (int)__lane_id();
     const double t_end = t_stop < P.tmx ? t_stop : P.tmx;
     const int64_t N = S.np;
@@ -1447,7 +1492,9 @@ hipError_t launch_t(const DeviceFields &f, const StepParams &sp, const DevicePar
                     double t_stop, LaunchConfig lc, hipStream_t stream)
 {
     auto kern = integrate_kernel<METHOD, MODE, LAYOUT, CACHE, PREC, CTRL, GEOM, TIME>;
+    constexpr int kSplit = static_split(METHOD, CTRL, CACHE, PREC, TIME);
     const size_t lds = CTRL == CTRL_LEARNED ? kNetLds + (size_t)(lc.block / 64) * sizeof(MlpWaveBuf)
+                       : kSplit > 0 ? (size_t)kSplit * 64 * sizeof(double) * (size_t)((lc.block + 63) / 64)
                        : TIME == 1 && CACHE == 1 &&
                                  !(PREC == PTC_FP32 ? SERIES_F32_REGS : SERIES_F64_REGS)
                            ? series_cache_lds(PREC, lc.block)
