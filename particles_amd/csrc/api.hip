@@ -424,10 +424,11 @@ static int build_step_params(const ptc_fields *f, int m, const double *params, i
     sp.beta = params[10];
     sp.expo = 2.0e-01 * sp.beta - 1.25e-01;  // integrations.F90:348
     // a power of two scales exactly: division by it equals multiplication by
-    // its (exact) reciprocal
+    // its reciprocal (both round the same real number), when that reciprocal
+    // is itself a normal double
     auto pow2 = [](double x) {
         int e = 0;
-        return x != 0.0 && std::fabs(std::frexp(x, &e)) == 0.5;
+        return std::isnormal(x) && std::isnormal(1.0 / x) && std::fabs(std::frexp(x, &e)) == 0.5;
     };
     sp.pow2 = pow2(params[1]);
     sp.rvun = 1.0 / params[1];
