@@ -293,9 +293,11 @@ struct Force {
     // stime[k+1] and a = (ts - stime[k]) / (stime[k+1] - stime[k]) (two =
     // true).  At or before the first time (or a NaN time) snapshot 0 alone,
     // at or after the last time the last snapshot alone (two = false).  The
-    // search walks from the lane's last interval: stage times move forward
-    // by less than a snapshot spacing, so it is nearly always 0 or 1 steps.
-    // Both walks stay in [0, nsnap - 2] because stime[0] < ts < stime[nsnap-1].
+    // search starts from the lane's last interval (stage times move forward
+    // by less than a snapshot spacing, so that or the next one nearly
+    // always holds) and bisects otherwise (a lane that picked up a particle
+    // at another time).  Every index stays in [0, nsnap - 1] because
+    // stime[0] < ts < stime[nsnap - 1].
     __device__ __forceinline__ int interval(double &a, bool &two)
     {
         const int ns = F.nsnap;
@@ -305,8 +307,19 @@ struct Force {
         if (ns <= 1 || !(ts > st[0])) return 0;
         if (ts >= st[ns - 1]) return ns - 1;
         int k = kh;
-        while (ts < st[k]) k--;
-        while (ts >= st[k + 1]) k++;
+        if (!(st[k] <= ts && ts < st[k + 1])) {
+            if (k + 2 < ns && st[k + 1] <= ts && ts < st[k + 2]) {
+                k = k + 1;
+            } else {
+                int lo = 0, hi = ns - 1;  // st[lo] <= ts < st[hi]
+                while (hi - lo > 1) {
+                    const int mid = (lo + hi) >> 1;
+                    if (st[mid] <= ts) lo = mid;
+                    else hi = mid;
+                }
+                k = lo;
+            }
+        }
         kh = k;
         a = (ts - st[k]) / (st[k + 1] - st[k]);
         two = true;
