@@ -234,6 +234,23 @@ def test_device_series_matches_oracle(oracle, method, mode, cache):
 
 
 @pytest.mark.gpu
+def test_device_series_many_snapshots_matches_oracle(oracle):
+    """33 snapshots over the run: lanes that pick up a new particle start
+    from an interval far from their last one (the lookup's bisection)."""
+    import particles_amd as P
+    tt = np.linspace(0.0, 6.0, 33) * W.T0
+    uu, bb = W.turbulent_vb_series(12, seed=23, times=tt, speed=0.05, nmodes=48)
+    p = W.params_vector(tmax=6 * W.T0, dtmax=1.0, atol=1e-8, rtol=1e-8)
+    tg = W.time_grid(6, p[2])
+    st0 = W.initial_state(512, len(tg), seed=24, gamma_range=(1.0, 1e3))
+    so, co, sto = oracle.integrate_series("dop853", uu, bb, tt, p, tg, st0)
+    with device_series(P, uu, bb, tt) as f:
+        st, cnt, status = P.integrate(f, "dop853", p, tg, st0)
+    assert (status == sto).all() and (status == 0).all()
+    assert np.array_equal(st, so) and np.array_equal(cnt, co)
+
+
+@pytest.mark.gpu
 def test_device_series_probes_match_oracle(oracle):
     import particles_amd as P
     rng = np.random.default_rng(13)
