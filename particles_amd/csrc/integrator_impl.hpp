@@ -13,12 +13,6 @@ namespace ptc {
 namespace {
 
 constexpr int kBlock = 256;
-#ifndef SERIES_F32_REGS
-#define SERIES_F32_REGS 1
-#endif
-#ifndef SERIES_F64_REGS
-#define SERIES_F64_REGS 0
-#endif
 
 // ---------------------------------------------------------------------------
 // trilinear gather, sources/interpolations.F90:104-226
@@ -279,7 +273,7 @@ struct Force {
     // TIME with the cache in FP32: snapshot k+1 in registers instead (the
     // FP32 kernel has room for 48 more floats; FP64 does not)
     static constexpr bool kSecondInRegs =
-        TIME == 1 && CACHE == 1 && (PREC == PTC_FP32 ? SERIES_F32_REGS : SERIES_F64_REGS);
+        TIME == 1 && CACHE == 1 && PREC == PTC_FP32;
     K v1[kSecondInRegs ? 48 : 1];
 
     __device__ __forceinline__ Force(const DeviceFields &F_, const StepParams &P_) : F(F_), P(P_) {}
@@ -1508,8 +1502,7 @@ hipError_t launch_t(const DeviceFields &f, const StepParams &sp, const DevicePar
     constexpr int kSplit = static_split(METHOD, CTRL, CACHE, PREC, TIME);
     const size_t lds = CTRL == CTRL_LEARNED ? kNetLds + (size_t)(lc.block / 64) * sizeof(MlpWaveBuf)
                        : kSplit > 0 ? (size_t)kSplit * 64 * sizeof(double) * (size_t)((lc.block + 63) / 64)
-                       : TIME == 1 && CACHE == 1 &&
-                                 !(PREC == PTC_FP32 ? SERIES_F32_REGS : SERIES_F64_REGS)
+                       : TIME == 1 && CACHE == 1 && PREC == PTC_FP64
                            ? series_cache_lds(PREC, lc.block)
                            : 0;
     if (lds > 65536) {

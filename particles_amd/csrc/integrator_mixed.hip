@@ -15,11 +15,10 @@ hipError_t launch_integrate_mixed(int method, const DeviceFields &f, const StepP
     if (f.prec != PTC_FP32 || f.layout != LAYOUT_AOS) return hipErrorInvalidValue;
     const int geom = sp.pow2 ? 1 : 0;
     if (f.nsnap > 0) {
-        // snapshot series: PI controller; the cell cache keeps the second
-        // snapshot in LDS where the device has room for it
+        // snapshot series: PI controller; the cell cache keeps both
+        // snapshots' corner values in registers
         if (lc.ctrl != CTRL_PI) return hipErrorInvalidValue;
-        const int c =
-            lc.cell_cache == 1 && (SERIES_F32_REGS || series_cache_fits(PTC_FP32, lc.block)) ? 1 : 0;
+        const int c = lc.cell_cache == 1 ? 1 : 0;
 #define SERIES(M, MO, C, G)                                                    \
     if (method == M && f.mode == MO && c == C && geom == G)                    \
         return launch_t<M, MO, LAYOUT_AOS, C, PTC_FP32, CTRL_PI, G, 1>(f, sp, p, t_stop, lc, stream);
