@@ -91,7 +91,7 @@ struct LaunchConfig {
 
 // Launches the integrator on `stream`; returns a hipError_t.  A series of
 // snapshots (f.nsnap > 0) takes the time-interpolating kernels (PI
-// controller, no cell cache).
+// controller).
 hipError_t launch_integrate(int method, const DeviceFields &f, const StepParams &sp,
                             const DeviceParticles &p, double t_stop, const LaunchConfig &lc,
                             hipStream_t stream);
