@@ -46,7 +46,8 @@ def _final_error(state, ref):
 def train_controller(fields: Fields, method: str, params, time, state_train, *,
                      fields_fp64: Fields | None = None, horizon: float | None = None,
                      steps: int = 3000, batch: int = 16384, lr: float = 2e-3, alpha: float = 1.0,
-                     shifts=(-0.35, -0.2, -0.1, 0.0, 0.1), capacity: int = 60_000_000,
+                     shifts=(-0.45, -0.4, -0.35, -0.3, -0.25, -0.2, -0.15, -0.1, 0.0, 0.1),
+                     capacity: int = 60_000_000,
                      seed: int = 1) -> Controller:
     """Records, trains and calibrates a learned controller for `fields` (the
     precision it will run with).  `state_train` is (8, m, n) like every
