@@ -1,0 +1,60 @@
+"""bench.py's contract on the CPU: the workload configurations it reports
+and the committed rocprofv3 traffic figures it attaches to them (a figure
+only for the configuration it was measured on), and the refusal of more than
+one GPU."""
+import json
+import os
+import subprocess
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, ROOT)
+import bench  # noqa: E402
+
+
+def _cfg(argv):
+    import argparse
+    ap = argparse.ArgumentParser()
+    for a, d in (("--workload", "turbulent256"), ("--layout", "aos"), ("--order", "cost"),
+                 ("--prec", "fp64"), ("--method", "")):
+        ap.add_argument(a, default=d)
+    for a, d in (("--particles", 0), ("--grid", 0), ("--seed", 1), ("--cell-cache", 1),
+                 ("--sort", 1), ("--snapshots", 0), ("--steps", 10), ("--warmup", 2)):
+        ap.add_argument(a, type=int, default=d)
+    ap.add_argument("--snapshot-every", type=float, default=2.0)
+    args = ap.parse_args(argv)
+    if args.workload == "turbulent256" and args.snapshots == 0:
+        # the static default builds its field on the host: skip that here
+        cfg = {"grid": 256, "particles": 100000, "method": "dop853", "tol": 1e-8,
+               "layout": args.layout, "cell_cache": args.cell_cache, "field_precision": args.prec,
+               "sort": args.sort, "order": args.order, "field_snapshots": 0}
+        return cfg
+    return bench.build_workload(args)[-1]
+
+
+def test_profiles_name_their_configuration():
+    names = sorted(f for f in os.listdir(os.path.join(ROOT, "profiles")) if f.endswith(".json"))
+    assert names, "no traffic figures committed"
+    for n in names:
+        with open(os.path.join(ROOT, "profiles", n)) as fh:
+            prof = json.load(fh)
+        assert prof["hbm_bytes_per_launch"] == pytest.approx(prof["read_bytes"] + prof["write_bytes"])
+        assert prof["config"]["grid"] > 0 and "source" in prof
+
+
+def test_traffic_matches_only_its_configuration():
+    static = bench.traffic_from_profile(_cfg([]))
+    series = bench.traffic_from_profile(_cfg(["--snapshots", "8"]))
+    assert static is not None and series is not None and series > static
+    assert bench.traffic_from_profile(_cfg(["--snapshots", "8", "--snapshot-every", "1"])) is None
+    assert bench.traffic_from_profile(_cfg(["--layout", "soa"])) is None
+    assert bench.traffic_from_profile(_cfg(["--order", "spatial"])) is None
+    assert bench.traffic_from_profile(_cfg(["--prec", "fp32"])) is None
+
+
+def test_more_than_one_gpu_is_refused():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2 and "one GPU" in r.stderr
