@@ -167,8 +167,9 @@ int ptc_particles_moments(ptc_particles *p, double moments[24]);
  * ptc_particles_last_kernel_ms then reports the whole graph. */
 int ptc_particles_advance_graph(ptc_particles *p, int n, const double *t_stop);
 /* Autotuner: integrates the batch up to t_end in 1 + 6 equal segments, the
- * first to warm up, then one per candidate launch setting (threads per
- * block 256 / 128 / 64, work-queue order by cost / spatial / none), and
+ * first to warm up, then one per candidate launch setting (work-queue order
+ * by cost at 256 / 128 / 64 threads per block, spatial at 256, index order
+ * at 256 / 128), and
  * keeps for this batch the one with the most RHS evaluations per kernel
  * millisecond.  The segments are real progress: every setting gives the
  * same results.  chosen (may be NULL) = block, sort cadence, order (1
