@@ -4,8 +4,8 @@ the CPU oracle on the same seeded inputs.
 Bar: the device path runs the reference's FP64 operation order without FMA
 contraction, and its step-size powers reproduce the host libm's pow, so
 snapshots and step counters are bit-identical to the reference's.  Each test
-also reports the deviation against the north-star tolerance (1e-8 relative
-in position and momentum), which bit-identity implies.
+also reports the deviation against a tolerance of 1e-8 relative in position
+and momentum, which bit-identity implies.
 """
 import math
 
