@@ -283,7 +283,38 @@ def test_identical_snapshot_series_equals_static_device():
         s1, c1, _ = P.integrate(f, "dop853", p, tg, st0)
         with P.Fields.series([f, f], [0.0, W.T0]) as fs:
             s2, c2, _ = P.integrate(fs, "dop853", p, tg, st0)
+        with P.Fields.series([f], [0.0]) as f1:  # one snapshot through the series kernels
+            s3, c3, _ = P.integrate(f1, "dop853", p, tg, st0)
     assert np.array_equal(s1, s2) and np.array_equal(c1, c2)
+    assert np.array_equal(s1, s3) and np.array_equal(c1, c3)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_device_series_general_geometry(oracle, prec):
+    """A domain that is neither at the origin nor of power-of-two size (the
+    general, dividing geometry kernels) and a non-unit velocity unit: FP64
+    bit-identical to the oracle, FP32 within the mixed-precision bound."""
+    import particles_amd as P
+    tt = TIMES[:3]
+    uu, bb = W.turbulent_vb_series(16, seed=25, times=tt, speed=0.02, nmodes=64)
+    xl, xs = (-0.5, 0.2, 0.1), (1.5, 2.0, 0.75)
+    p = W.params_vector(qom=1.0, vun=0.9, tmax=(2 if prec == "fp64" else 0.1) * W.T0,
+                        dtmax=1.0, atol=1e-8, rtol=1e-8, xl=xl, xs=xs)
+    tg = W.time_grid(3, p[2])
+    st0 = W.initial_state(256, len(tg), seed=26, gamma_range=(1.0, 1e3), xl=xl, xs=xs)
+    so, co, sto = oracle.integrate_series("dop853", uu, bb, tt, p, tg, st0)
+    with device_series(P, uu, bb, tt, prec=prec) as f:
+        st, cnt, status = P.integrate(f, "dop853", p, tg, st0)
+    assert (status == 0).all() and (sto == 0).all()
+    if prec == "fp64":
+        assert np.array_equal(st, so) and np.array_equal(cnt, co)
+    else:
+        d = st[:6] - so[:6]
+        dev = np.maximum(np.sqrt((d[:3] ** 2).sum(0)) / np.maximum(np.sqrt((so[:3] ** 2).sum(0)), 1),
+                         np.sqrt((d[3:6] ** 2).sum(0)) / np.maximum(np.sqrt((so[3:6] ** 2).sum(0)), 1)).max(0)
+        say(f"[mixed series, general geometry] deviation median {np.median(dev):.2e} max {dev.max():.2e}")
+        assert np.median(dev) < 1e-6 and dev.max() < 1e-3
 
 
 @pytest.mark.gpu
