@@ -393,8 +393,8 @@ def test_spatial_order_keeps_results():
                     b.advance(k * 0.5 * W.T0)
                 out.append(b.download())
         finally:
-            P.set_option("sort", 0)
-            P.set_option("order", "spatial")
+            P.set_option("sort", 1)
+            P.set_option("order", "cost")
     for s, c, status in out[1:]:
         assert np.array_equal(s, out[0][0]) and np.array_equal(c, out[0][1])
         assert (status == 0).all()
