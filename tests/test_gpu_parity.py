@@ -130,6 +130,79 @@ def test_device_matches_oracle_turbulent_10_gyroperiods(oracle, method):
     assert ex.all()
 
 
+def test_cpu_config_full_size_matches_oracle(oracle):
+    """The reference's CPU-runnable case at its full size: 1,000 protons
+    (gamma 10, isotropic, uniform in the unit box) in the 64^3 turbulent
+    field (B0 = 1, dB rms 0.5, V rms 0.1, 256 modes), Cash-Karp RK5(4) at
+    atol = rtol = 1e-6 to t = 100 gyro-periods of B0, ten snapshots.  Every
+    snapshot and step counter bit-identical to the oracle (bound: 1e-8
+    relative in position and momentum); where the compiled reference is
+    present, its first 64 particles bit-identical to it as well."""
+    import os
+    from oracle.oracle import REF_SO, Reference
+    uu, bb = W.turbulent_vb(64, seed=1)
+    p = W.params_vector(tmax=100 * W.T0, dtmax=W.T0, atol=1e-6, rtol=1e-6)
+    tg = W.time_grid(10, p[2])
+    st0 = W.initial_state(1000, len(tg), seed=2, gamma=10.0)
+    say("[config 0] turbulent 64^3 rk5 tol 1e-6 np=1000 t=100 T0")
+    with P.Fields(uu, bb) as f:
+        st, cnt, status = P.integrate(f, "rk5", p, tg, st0)
+    so, co, _ = oracle.integrate("rk5", uu, bb, p, tg, st0)
+    ex = exact_particles(st, cnt, so, co)
+    dev = deviation(st, so)
+    say(f"  bit-exact {ex.sum()}/{len(ex)}, max deviation {dev.max():.3e}, "
+        f"accepted {cnt[1].sum()}, rejected {cnt[2].sum()}")
+    assert (status == 0).all()
+    assert dev.max() <= PARITY_TOL
+    assert ex.all()
+    if os.path.exists(REF_SO):
+        sr, cr = Reference().integrate("rk5", uu, bb, p, tg, st0[:, :, :64])
+        assert np.array_equal(st[:, :, :64], sr) and np.array_equal(cnt[:, :64], cr)
+        say("  first 64 particles bit-identical to the compiled reference")
+
+
+def test_large_field_offsets(oracle):
+    """A 512^3 field: its node rows span 6.4 GB (FP64) and 4.3 GB (FP32),
+    beyond 32-bit byte offsets.  Gathers and RHS at positions over the whole
+    box, the far corner and its periodic faces included, are bit-identical to
+    the oracle in FP64 and within 1e-6 of the field scale in FP32, and a
+    short DOP853 run of 64 particles is bit-identical to the oracle."""
+    n = 512
+    rng = np.random.default_rng(31)
+    uu = np.empty((n, n, n, 3), dtype=np.float64, order="F")
+    bb = np.empty((n, n, n, 3), dtype=np.float64, order="F")
+    for arr, off in ((uu, 0.0), (bb, 1.0)):
+        for c in range(3):
+            # a sum of three random profiles, one per axis: every node distinct
+            ax, ay, az = (0.1 * rng.normal(size=n) for _ in range(3))
+            arr[..., c] = ax[:, None, None] + ay[None, :, None]
+            arr[..., c] += az[None, None, :] + (off if c == 2 else 0.0)
+    pos = rng.uniform(0.0, 1.0, size=(3000, 3))
+    pos[:1000] = 1.0 - rng.uniform(0.0, 2.0 / n, size=(1000, 3))  # the last cells
+    pos[1000:1100] = np.round(pos[1000:1100] * n) / n              # nodes and faces
+    vec = np.hstack([pos, rng.normal(size=(3000, 3))])
+    p = W.params_vector(tmax=0.2 * W.T0, dtmax=1.0, atol=1e-8, rtol=1e-8)
+    tg = W.time_grid(2, p[2])
+    st0 = W.initial_state(64, len(tg), seed=32, gamma_range=(1.0, 100.0))
+    st0[0:3, 0, :32] = 1.0 - rng.uniform(0.0, 2.0 / n, size=(3, 32))
+    gu = np.array([np.concatenate(oracle.interpolate_lin(uu, bb, x)) for x in pos])
+    ga = np.array([oracle.acceleration(uu, bb, v) for v in vec])
+    so, co, _ = oracle.integrate("dop853", uu, bb, p, tg, st0)
+    say(f"[large] {n}^3 fp64: {len(pos)} probes, 64 particles")
+    with P.Fields(uu, bb) as f:
+        g = P.probe(f, "gather", pos, p)
+        a = P.probe(f, "rhs", vec, p)
+        st, cnt, status = P.integrate(f, "dop853", p, tg, st0)
+    assert np.array_equal(g, gu) and np.array_equal(a, ga)
+    assert (status == 0).all() and np.array_equal(st, so) and np.array_equal(cnt, co)
+    say(f"[large] {n}^3 fp32 probes")
+    with P.Fields(uu, bb, prec="fp32") as f:
+        g32 = P.probe(f, "gather", pos, p)
+    err = np.abs(g32 - gu).max() / np.abs(gu).max()
+    say(f"  fp32 gather max error {err:.2e} of the field scale")
+    assert err < 1e-6
+
+
 def test_eb_mode_matches_oracle(oracle):
     ee, bb = W.reconnection_eb(32, seed=3, nmodes=64)
     p = W.params_vector(tmax=2 * W.T0, dtmax=1.0, atol=1e-8, rtol=1e-8)
