@@ -114,7 +114,7 @@ def build_workload(args):
            "grid": dm, "particles": npart, "method": method, "tol": tol,
            "snapshots_every_T0": tmax / W.T0 / nsnap, "layout": args.layout,
            "cell_cache": args.cell_cache, "field_precision": args.prec, "sort": args.sort,
-           "order": args.order, "field_snapshots": args.snapshots}
+           "order": args.order, "field_snapshots": args.snapshots, "tail": args.tail}
     if args.snapshots > 0:
         cfg["snapshot_every_T0"] = args.snapshot_every
     return make_fields, uu, bb, p, tg, st0, method, cfg
@@ -162,8 +162,8 @@ def traffic_from_profile(cfg):
     configuration."""
     import glob
     keys = ("grid", "particles", "method", "tol", "layout", "cell_cache", "field_precision",
-            "sort", "order", "field_snapshots", "snapshot_every_T0")
-    defaults = {"field_snapshots": 0}
+            "sort", "order", "field_snapshots", "snapshot_every_T0", "tail")
+    defaults = {"field_snapshots": 0, "tail": "0"}
     for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.json"))):
         try:
             with open(path) as fh:
@@ -171,9 +171,18 @@ def traffic_from_profile(cfg):
         except (OSError, ValueError):
             continue
         pc = prof.get("config", {})
-        if all(pc.get(k, defaults.get(k)) == cfg.get(k, defaults.get(k)) for k in keys):
+        if all(_norm(pc, k, defaults) == _norm(cfg, k, defaults) for k in keys):
             return prof["hbm_bytes_per_launch"]
     return None
+
+
+def _norm(cfg, key, defaults):
+    """A configuration value for matching; the tail setting only selects
+    kernels for FP64 static fields in node rows, elsewhere it is moot."""
+    if key == "tail" and (cfg.get("field_snapshots", 0) or cfg.get("field_precision") != "fp64"
+                          or cfg.get("layout") != "aos"):
+        return None
+    return cfg.get(key, defaults.get(key))
 
 
 def run_variant(fields, method, p, tg, st, steps, warmup, net=None, shift=0.0):
@@ -297,6 +306,9 @@ def main():
     ap.add_argument("--layout", default="aos", choices=["aos", "soa", "brick"])
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--cell-cache", type=int, default=1, choices=[0, 1])
+    ap.add_argument("--tail", default="auto",
+                    help="FP64 static runs: hand a wave's last particles (at most this many) to "
+                         "the group kernel once the queue is empty; 'group': that kernel only")
     ap.add_argument("--grid-blocks", type=int, default=0)
     ap.add_argument("--sort", type=int, default=1,
                     help="re-order the work queue every N steps (0 = never)")
@@ -349,6 +361,7 @@ def main():
     P.set_option("grid_blocks", args.grid_blocks)
     P.set_option("sort", args.sort)
     P.set_option("order", args.order)
+    P.set_option("tail", args.tail)
     fields = make_fields(args.prec)
     batch = P.Particles(fields, method, p, tg, st0)
     print(f"setup {time.perf_counter() - t_setup:.1f}s: {cfg['workload']}", file=sys.stderr,

@@ -166,15 +166,17 @@ int ptc_particles_moments(ptc_particles *p, double moments[24]);
  * is computed once, before the graph, and kept for all n intervals.
  * ptc_particles_last_kernel_ms then reports the whole graph. */
 int ptc_particles_advance_graph(ptc_particles *p, int n, const double *t_stop);
-/* Autotuner: integrates the batch up to t_end in 1 + 6 equal segments, the
+/* Autotuner: integrates the batch up to t_end in 1 + 7 equal segments, the
  * first to warm up, then one per candidate launch setting (work-queue order
- * by cost at 256 / 128 / 64 threads per block, spatial at 256, index order
- * at 256 / 128), and
- * keeps for this batch the one with the most RHS evaluations per kernel
- * millisecond.  The segments are real progress: every setting gives the
- * same results.  chosen (may be NULL) = block, sort cadence, order (1
- * spatial, 2 cost). */
-int ptc_particles_autotune(ptc_particles *p, double t_end, int chosen[3]);
+ * by cost at 256 threads per block with one lane per particle, with the
+ * group kernel alone and with the last 32768 particles handed to it ("tail"
+ * option); cost order at 128 / 64 threads per block, spatial order at 256,
+ * index order at 256), and keeps for this batch the one with the most RHS
+ * evaluations per kernel millisecond.  The segments are real progress: every
+ * setting gives the same results.  chosen (may be NULL) = block, sort
+ * cadence, order (1 spatial, 2 cost), tail (0 lane per particle, -1 group
+ * kernel alone, N > 0 hand-over count). */
+int ptc_particles_autotune(ptc_particles *p, double t_end, int chosen[4]);
 /* totals[0..4] = sum of all / accepted / rejected steps, number of RHS
  * evaluations, number of particles still running; reduced on the device. */
 int ptc_particles_totals(ptc_particles *p, int64_t totals[5]);
@@ -303,6 +305,19 @@ int ptc_particles_use_mlp(ptc_particles *p, const ptc_mlp *net, double shift);
  *                 "spatial": Morton order of the particles' cells
  *                 (neighbouring lanes gather from neighbouring memory).
  *                 Results are identical for every order.
+ *   "tail"        FP64 static fields with the PI controller: which kernel
+ *                 integrates.  "0": one lane per particle (all 64 lanes of
+ *                 a wave carry particles, each lane all six components);
+ *                 "group": the group kernel, eight lanes per particle (one
+ *                 component per lane; fewer instructions per step and two
+ *                 waves per SIMD, so single trajectories and gathers from
+ *                 large fields finish sooner); N > 0: one lane per particle
+ *                 until the queue is empty and at most N particles are in
+ *                 flight, then the group kernel finishes them; "auto"
+ *                 (default): the group kernel for RK5, for batches of at
+ *                 most 32768 particles and for fields over 2 GB, else
+ *                 one lane per particle handing the last 32768 over.
+ *                 Results are identical for every setting.
  * Takes effect for the next create/advance. */
 int ptc_set_option(const char *key, const char *value);
 /* Reads an option back as text into buf (size len). */

@@ -39,7 +39,7 @@ struct ptc_particles {
     size_t order_bytes;
     int64_t advances;     // advance calls so far (sort cadence)
     int tuned;            // launch settings below chosen by ptc_particles_autotune
-    int tuned_block, tuned_sort, tuned_order;
+    int tuned_block, tuned_sort, tuned_order, tuned_tail;
 };
 
 struct ptc_mlp {
@@ -89,6 +89,7 @@ struct Options {
     int cell_cache = 1;
     int sort = 1;   // re-order the work queue every `sort` advances (0 = never)
     int order = 2;  // the order then: 1 spatial (Morton), 2 cost (most work first)
+    int tail = kTailAuto;  // LaunchConfig::tail, or chosen per batch
 };
 
 std::mutex g_opt_mu;
@@ -147,6 +148,18 @@ int ptc_set_option(const char *key, const char *value)
         if (!strcmp(value, "spatial")) g_opt.order = 1;
         else if (!strcmp(value, "cost")) g_opt.order = 2;
         else return fail(PTC_EARG, "order must be spatial or cost (got %s)", value);
+    } else if (!strcmp(key, "tail")) {
+        if (!strcmp(value, "group")) {
+            g_opt.tail = kTailGroupOnly;
+        } else if (!strcmp(value, "auto")) {
+            g_opt.tail = kTailAuto;
+        } else {
+            long v = strtol(value, &end, 10);
+            if (*end || v < 0 || v > (1l << 30))
+                return fail(PTC_EARG, "tail must be a particle count >= 0 or group (got %s)",
+                            value);
+            g_opt.tail = (int)v;
+        }
     } else {
         return fail(PTC_EARG, "unknown option %s", key);
     }
@@ -166,6 +179,11 @@ int ptc_get_option(const char *key, char *buf, int len)
     else if (!strcmp(key, "cell_cache")) snprintf(buf, len, "%d", o.cell_cache);
     else if (!strcmp(key, "sort")) snprintf(buf, len, "%d", o.sort);
     else if (!strcmp(key, "order")) snprintf(buf, len, "%s", o.order == 2 ? "cost" : "spatial");
+    else if (!strcmp(key, "tail")) {
+        if (o.tail == kTailGroupOnly) snprintf(buf, len, "group");
+        else if (o.tail == kTailAuto) snprintf(buf, len, "auto");
+        else snprintf(buf, len, "%d", o.tail);
+    }
     else return fail(PTC_EARG, "unknown option %s", key);
     ok();
     return PTC_OK;
@@ -406,6 +424,24 @@ int ptc_fields_destroy(ptc_fields *f)
     return PTC_OK;
 }
 
+// The "auto" tail setting for a batch (measured on the MI355X, every choice
+// giving the same results): the group kernel alone for Cash-Karp (its
+// trajectories differ most in length, and the lane-per-particle launch is set
+// by the slowest), for batches that leave most lanes of the lane-per-particle
+// kernel idle, and for fields beyond 2 GB (the gathers then miss the caches
+// and the group kernel's two waves per SIMD hide their latency better);
+// otherwise lane per particle, handing the last 32768 particles over.
+static int resolve_tail(int tail, const ptc_particles *p)
+{
+    if (tail != kTailAuto) return tail;
+    const DeviceFields &f = p->fields->d;
+    if (f.prec != PTC_FP64 || f.nsnap > 0 || f.layout != LAYOUT_AOS || p->ctrl != CTRL_PI) return 0;
+    const double bytes = 48.0 * (double)f.nodes;
+    if (p->method == PTC_RK5 || p->d.np <= 32768 || bytes > 2.0 * 1024 * 1024 * 1024)
+        return kTailGroupOnly;
+    return 32768;
+}
+
 static int build_step_params(const ptc_fields *f, int m, const double *params, int nparams,
                              StepParams &sp)
 {
@@ -556,11 +592,12 @@ int ptc_particles_advance(ptc_particles *p, double t_stop)
         o.block = p->tuned_block;
         o.sort = p->tuned_sort;
         o.order = p->tuned_order;
+        o.tail = p->tuned_tail;
     }
     p->sp.max_steps = o.max_steps;
     LaunchConfig lc{o.block, o.grid_blocks,
                     p->fields->d.layout == LAYOUT_AOS ? o.cell_cache : 0,
-                    p->ctrl};
+                    p->ctrl, resolve_tail(o.tail, p)};
     if (p->ctrl != CTRL_PI && lc.cell_cache != 1)
         return fail(PTC_ESTATE, "record and learned step-size modes need the aos layout with "
                                 "cell_cache 1");
@@ -589,7 +626,7 @@ int ptc_particles_advance(ptc_particles *p, double t_stop)
         p->d.order = nullptr;
     }
     p->advances++;
-    HIP_TRY(launch_zero_i32(p->d.qcount, 1, p->stream));
+    HIP_TRY(launch_zero_i32(p->d.qcount, 3, p->stream));
     HIP_TRY(hipEventRecord(p->ev0, p->stream));
     HIP_TRY(launch_integrate(p->method, p->fields->d, p->sp, p->d, t_stop, lc, p->stream));
     HIP_TRY(hipEventRecord(p->ev1, p->stream));
@@ -700,11 +737,12 @@ int ptc_particles_advance_graph(ptc_particles *p, int n, const double *t_stop)
         o.block = p->tuned_block;
         o.sort = p->tuned_sort;
         o.order = p->tuned_order;
+        o.tail = p->tuned_tail;
     }
     p->sp.max_steps = o.max_steps;
     LaunchConfig lc{o.block, o.grid_blocks,
                     p->fields->d.layout == LAYOUT_AOS ? o.cell_cache : 0,
-                    p->ctrl};
+                    p->ctrl, resolve_tail(o.tail, p)};
     if (p->ctrl != CTRL_PI && lc.cell_cache != 1)
         return fail(PTC_ESTATE, "record and learned step-size modes need the aos layout with "
                                 "cell_cache 1");
@@ -739,7 +777,7 @@ int ptc_particles_advance_graph(ptc_particles *p, int n, const double *t_stop)
     HIP_TRY(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal));
     hipError_t e = hipSuccess;
     for (int k = 0; k < n && e == hipSuccess; k++) {
-        e = launch_zero_i32(p->d.qcount, 1, p->stream);
+        e = launch_zero_i32(p->d.qcount, 3, p->stream);
         if (e == hipSuccess) e = launch_integrate(p->method, p->fields->d, p->sp, p->d, t_stop[k],
                                                   lc, p->stream);
     }
@@ -761,7 +799,7 @@ int ptc_particles_advance_graph(ptc_particles *p, int n, const double *t_stop)
     return PTC_OK;
 }
 
-int ptc_particles_autotune(ptc_particles *p, double t_end, int chosen[3])
+int ptc_particles_autotune(ptc_particles *p, double t_end, int chosen[4])
 {
     if (!p) return fail(PTC_EARG, "ptc_particles_autotune: null handle");
     if (!std::isfinite(t_end)) return fail(PTC_EARG, "ptc_particles_autotune: t_end not finite");
@@ -773,13 +811,15 @@ int ptc_particles_autotune(ptc_particles *p, double t_end, int chosen[3])
     HIP_TRY(hipMemcpy(ht.data(), p->d.t, sizeof(double) * ht.size(), hipMemcpyDeviceToHost));
     double t0 = ht[0];
     for (double v : ht) t0 = v < t0 ? v : t0;
-    struct Cand { int block, sort, order; };
-    const Cand cand[] = {{256, 1, 2}, {128, 1, 2}, {64, 1, 2}, {256, 1, 1}, {256, 0, 2}, {128, 0, 2}};
+    struct Cand { int block, sort, order, tail; };
+    const Cand cand[] = {{256, 1, 2, 0},     {256, 1, 2, kTailGroupOnly}, {256, 1, 2, 32768},
+                         {128, 1, 2, 0},     {64, 1, 2, 0},               {256, 1, 1, 0},
+                         {256, 0, 2, 0}};
     const int nc = (int)(sizeof cand / sizeof cand[0]);
     if (!(t_end > t0)) return fail(PTC_EARG, "ptc_particles_autotune: t_end must exceed the "
                                              "batch's time %g", t0);
     const int saved = p->tuned;
-    const int sb = p->tuned_block, ss = p->tuned_sort, so = p->tuned_order;
+    const int sb = p->tuned_block, ss = p->tuned_sort, so = p->tuned_order, st = p->tuned_tail;
     double best = -1.0;
     int pick = 0;
     int64_t tot[5];
@@ -790,13 +830,14 @@ int ptc_particles_autotune(ptc_particles *p, double t_end, int chosen[3])
         p->tuned_block = c.block;
         p->tuned_sort = c.sort;
         p->tuned_order = c.order;
+        p->tuned_tail = c.tail;
         int rc = ptc_particles_totals(p, tot);
         const int64_t rhs0 = tot[3];
         if (!rc) rc = ptc_particles_advance(p, t0 + (t_end - t0) * (k + 1) / (nc + 1));
         if (!rc) rc = ptc_particles_totals(p, tot);
         if (rc) {
             p->tuned = saved;
-            p->tuned_block = sb; p->tuned_sort = ss; p->tuned_order = so;
+            p->tuned_block = sb; p->tuned_sort = ss; p->tuned_order = so; p->tuned_tail = st;
             return rc;
         }
         const double rate = p->last_ms > 0.0 ? (double)(tot[3] - rhs0) / p->last_ms : 0.0;
@@ -809,10 +850,12 @@ int ptc_particles_autotune(ptc_particles *p, double t_end, int chosen[3])
     p->tuned_block = cand[pick].block;
     p->tuned_sort = cand[pick].sort;
     p->tuned_order = cand[pick].order;
+    p->tuned_tail = cand[pick].tail;
     if (chosen) {
         chosen[0] = p->tuned_block;
         chosen[1] = p->tuned_sort;
         chosen[2] = p->tuned_order;
+        chosen[3] = p->tuned_tail;
     }
     ok();
     return PTC_OK;

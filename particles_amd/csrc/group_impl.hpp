@@ -1,0 +1,517 @@
+// Group kernel: eight lanes integrate one particle, lane c of the group
+// holding component c (0..5) of the state and its slopes, lanes 6 and 7
+// mirroring components 0 and 1.  Same arithmetic as integrate_kernel, per
+// component, and every sum over components (error norms, the momentum's
+// norm) is formed by every lane of the group from the six values shuffled
+// across it, in the same order, so every lane of a group holds the same
+// step-control state and the results are bit-identical to the lane-per-
+// particle kernel and to the reference (integrations.F90:57-200, 224-505,
+// 524-589; accelerations.F90:59-96; interpolations.F90:104-226).
+//
+// Why: one lane carrying a whole particle issues every component's
+// arithmetic in sequence, so the slowest trajectory of a launch runs at one
+// wave's issue rate and sets the launch time once the work queue is empty.
+// A group of eight issues about a third of the instructions per attempt, so
+// those last trajectories finish sooner.  The lane-per-particle kernel hands
+// its particles over once the queue is exhausted and a wave has few lanes
+// left (DeviceParticles::park); this kernel then finishes them.  It can also
+// run a whole launch (few particles, where every launch is tail).
+//
+// FP64 fields in node rows, static field, PI controller.
+#pragma once
+#include "integrator_impl.hpp"
+
+namespace ptc {
+namespace {
+
+constexpr int kGroup = 8;
+
+// component j of x across the group starting at lane base
+__device__ __forceinline__ double gsh(double x, int base, int j) { return __shfl(x, base + j); }
+
+// Component c of the Lorentz force (accelerations.F90:59-96, the operations
+// of force_from_fields) from the momentum (u0, u1, u2) of the stage point and
+// the interpolated fields (U or E: f0..f2, B: f3..f5), known on every lane.
+// The operands are scalars selected per lane: selected from an array, they
+// would be indexed per lane and go through scratch.
+template <int MODE, int GEOM>
+__device__ __forceinline__ double force_component(const StepParams &P, double u0, double u1,
+                                                  double u2, double f0, double f1, double f2,
+                                                  double f3, double f4, double f5, int c)
+{
+    const double g = sqrt(u0 * u0 + u1 * u1 + u2 * u2 + 1.0);
+    const double v0 = u0 / g, v1 = u1 / g, v2 = u2 / g;
+    const double vc = c == 0 ? v0 : c == 1 ? v1 : v2;
+    const double lin = GEOM ? vc * P.rvun : vc / P.vun;
+    const bool is3 = c == 3, is4 = c == 4;
+    const double b = is3 ? f5 : is4 ? f3 : f4;
+    const double e = is3 ? f4 : is4 ? f5 : f3;
+    double rot;
+    if constexpr (MODE == PTC_MODE_EB) {
+        // E + v x B
+        const double a = is3 ? v1 : is4 ? v2 : v0;
+        const double d = is3 ? v2 : is4 ? v0 : v1;
+        const double ec = is3 ? f0 : is4 ? f1 : f2;
+        rot = P.qom * (ec + (a * b - d * e));
+    } else {
+        // (v - U) x B
+        const double w0 = v0 - f0, w1 = v1 - f1, w2 = v2 - f2;
+        const double a = is3 ? w1 : is4 ? w2 : w0;
+        const double d = is3 ? w2 : is4 ? w0 : w1;
+        rot = P.qom * (a * b - d * e);
+    }
+    return c < 3 ? lin : rot;
+}
+
+// true if every component of x in the group is finite
+__device__ __forceinline__ bool group_finite(double x, int base)
+{
+    const uint64_t bad = __ballot(!isfinite(x));
+    return ((bad >> base) & 0xffull) == 0;
+}
+
+// The RHS as the group evaluates it: each lane keeps component comp of the
+// eight corner rows of the cell it last gathered from (the lane-per-particle
+// kernel's cell cache, split over the group).
+template <int MODE, int GEOM>
+struct GroupForce {
+    const DeviceFields &F;
+    const StepParams &P;
+    const int base, comp;
+    uint32_t key0 = 0xffffffffu, key7 = 0xffffffffu;
+    double v[8];
+
+    __device__ __forceinline__ GroupForce(const DeviceFields &F_, const StepParams &P_, int b, int c)
+        : F(F_), P(P_), base(b), comp(c)
+    {
+    }
+
+    // component comp of f(s), s given by its component comp on each lane
+    __device__ __forceinline__ double operator()(double s)
+    {
+        double x[6];
+#pragma unroll
+        for (int j = 0; j < 6; j++) x[j] = gsh(s, base, j);
+        Corners c;
+        corners<double, GEOM>(P, x, c);
+        if (c.o[0] != key0 || c.o[7] != key7) {
+            const double *__restrict__ d = (const double *)F.data + comp;
+#pragma unroll
+            for (int j = 0; j < 8; j++) v[j] = d[(size_t)c.o[j] * 6];
+            key0 = c.o[0];
+            key7 = c.o[7];
+        }
+        // interpolations.F90:212-220, corners summed left to right
+        double f = c.w[0] * v[0];
+#pragma unroll
+        for (int j = 1; j < 8; j++) f = f + c.w[j] * v[j];
+        return force_component<MODE, GEOM>(P, x[3], x[4], x[5], gsh(f, base, 0), gsh(f, base, 1),
+                                           gsh(f, base, 2), gsh(f, base, 3), gsh(f, base, 4),
+                                           gsh(f, base, 5), comp);
+    }
+};
+
+// dti_guess (integrations.F90:524-589) with the component sums formed over
+// the group; ki = f(si)
+__device__ __forceinline__ double group_dti_guess(const StepParams &P, int iord, int base, double si,
+                                                  double ki, const PowTables &T)
+{
+    const double sr = P.atol + P.rtol * fabs(si);
+    const double q0 = si / sr, q1 = ki / sr;
+    double s = 0.0, q, d0, d1, d2, dti, dtf;
+#pragma unroll
+    for (int j = 0; j < 6; j++) { q = gsh(q0, base, j); s = s + q * q; }
+    d0 = sqrt(s / 6.0);
+    s = 0.0;
+#pragma unroll
+    for (int j = 0; j < 6; j++) { q = gsh(q1, base, j); s = s + q * q; }
+    d1 = sqrt(s / 6.0);
+    dti = (fmin(d0, d1) < 1.0e-10) ? 1.0e-06 : 1.0e-01 * (d0 / d1);
+    const double sf = si + dti * ki;
+    const double q2 = (sf - si) / sr;
+    s = 0.0;
+#pragma unroll
+    for (int j = 0; j < 6; j++) { q = gsh(q2, base, j); s = s + q * q; }
+    d2 = sqrt(s / 6.0) / dti;
+    if (d1 > d2) d2 = d1;
+    if (d2 <= 1.0e-15) {
+        dtf = 1.0e-03 * dti;
+        if (dtf < 1.0e-06) dtf = 1.0e-06;
+    } else {
+        dtf = ptc_pow_with(1.0e-02 / d2, 1.0 / (double)iord, T.log, T.exp);
+    }
+    dti = 1.0e+02 * dti;
+    if (dtf < dti) dti = dtf;
+    if (P.dtm < dti) dti = P.dtm;
+    return dti;
+}
+
+// Cash-Karp attempt (integrations.F90:131-148) on one component; returns del
+template <class GF>
+__device__ __forceinline__ double group_attempt_rk5(GF &rhsf, const StepParams &P, int base,
+                                                    double si, double f1, double dt, double &sf,
+                                                    double &k6d)
+{
+    const double k1 = dt * f1;
+    double s = si + CK_A21 * k1;
+    const double k2 = dt * rhsf(s);
+    s = si + CK_A31 * k1 + CK_A32 * k2;
+    const double k3 = dt * rhsf(s);
+    s = si + CK_A41 * k1 + CK_A42 * k2 + CK_A43 * k3;
+    const double k4 = dt * rhsf(s);
+    s = si + CK_A51 * k1 + CK_A52 * k2 + CK_A53 * k3 + CK_A54 * k4;
+    const double k5 = dt * rhsf(s);
+    s = si + CK_A61 * k1 + CK_A62 * k2 + CK_A63 * k3 + CK_A64 * k4 + CK_A65 * k5;
+    const double k6 = dt * rhsf(s);
+
+    const double eps = 2.2250738585072014e-308;  // tiny(1d0), integrations.F90:99
+    double sc = 0.0;
+    sc = sc + CK_C1 * k1;
+    sc = sc + 0.0 * k2;
+    sc = sc + CK_C3 * k3;
+    sc = sc + CK_C4 * k4;
+    sc = sc + 0.0 * k5;
+    sc = sc + CK_C6 * k6;
+    sf = si + sc;
+    double se = 0.0;
+    se = se + CK_D1 * k1;
+    se = se + 0.0 * k2;
+    se = se + CK_D3 * k3;
+    se = se + CK_D4 * k4;
+    se = se + CK_D5 * k5;
+    se = se + CK_D6 * k6;
+    const double qc = fabs(se / (fabs(si) + fabs(k1) + eps));
+    double del = 0.0;
+#pragma unroll
+    for (int j = 0; j < 6; j++) {
+        const double q = gsh(qc, base, j);
+        if (j == 0 || q > del) del = q;
+    }
+    k6d = k6;
+    return del / P.atol;
+}
+
+// DOP853 attempt (integrations.F90:365-437) on one component; k1 = f(si);
+// returns err
+template <class GF>
+__device__ __forceinline__ double group_attempt_dp853(GF &rhsf, const StepParams &P, int base,
+                                                      double si, double k1, double dt, double &sf)
+{
+    double ss = si + dt * A0201 * k1;
+    const double k2 = rhsf(ss);
+    ss = si + dt * (A0301 * k1 + A0302 * k2);
+    const double k3 = rhsf(ss);
+    ss = si + dt * (A0401 * k1 + A0403 * k3);
+    const double k4 = rhsf(ss);
+    ss = si + dt * (A0501 * k1 + A0503 * k3 + A0504 * k4);
+    const double k5 = rhsf(ss);
+    ss = si + dt * (A0601 * k1 + A0604 * k4 + A0605 * k5);
+    const double k6 = rhsf(ss);
+    ss = si + dt * (A0701 * k1 + A0704 * k4 + A0705 * k5 + A0706 * k6);
+    const double k7 = rhsf(ss);
+    ss = si + dt * (A0801 * k1 + A0804 * k4 + A0805 * k5 + A0806 * k6 + A0807 * k7);
+    const double k8 = rhsf(ss);
+    ss = si + dt * (A0901 * k1 + A0904 * k4 + A0905 * k5 + A0906 * k6 + A0907 * k7 + A0908 * k8);
+    const double k9 = rhsf(ss);
+    ss = si + dt * (A1001 * k1 + A1004 * k4 + A1005 * k5 + A1006 * k6 + A1007 * k7 + A1008 * k8 +
+                    A1009 * k9);
+    const double k10 = rhsf(ss);
+    ss = si + dt * (A1101 * k1 + A1104 * k4 + A1105 * k5 + A1106 * k6 + A1107 * k7 + A1108 * k8 +
+                    A1109 * k9 + A1110 * k10);
+    const double k11 = rhsf(ss);
+    ss = si + dt * (A1201 * k1 + A1204 * k4 + A1205 * k5 + A1206 * k6 + A1207 * k7 + A1208 * k8 +
+                    A1209 * k9 + A1210 * k10 + A1211 * k11);
+    const double k12 = rhsf(ss);
+
+    const double kb = DB01 * k1 + DB06 * k6 + DB07 * k7 + DB08 * k8 + DB09 * k9 + DB10 * k10 +
+                      DB11 * k11 + DB12 * k12;
+    sf = si + dt * kb;
+    const double a = fabs(si), b = fabs(sf);
+    const double sr = P.atol + P.rtol * (a > b ? a : b);
+    const double e3 = (kb - BH1 * k1 - BH2 * k9 - BH3 * k12) / sr;
+    const double e5 = (ER01 * k1 + ER06 * k6 + ER07 * k7 + ER08 * k8 + ER09 * k9 + ER10 * k10 +
+                       ER11 * k11 + ER12 * k12) / sr;
+    double err3 = 0.0, err5 = 0.0;
+#pragma unroll
+    for (int j = 0; j < 6; j++) {
+        const double q = gsh(e3, base, j);
+        err3 = err3 + q * q;
+    }
+#pragma unroll
+    for (int j = 0; j < 6; j++) {
+        const double q = gsh(e5, base, j);
+        err5 = err5 + q * q;
+    }
+    const double deno = err5 + 1.0e-02 * err3;
+    return deno <= 0.0 ? fabs(dt) * err5 / sqrt(6.0) : fabs(dt) * err5 / sqrt(6.0 * deno);
+}
+
+// cubic Hermite snapshot (integrations.F90:158-168 / :447-457) of one
+// component; the group's lanes 0..5 write components 1..6 of state(:, n),
+// lane 0 also gamma and dt
+__device__ __forceinline__ void group_hermite_out(double *st, int comp, int cl, int base, double t,
+                                                  double tn, double dt, double si, double sf,
+                                                  double ka, double kb)
+{
+    const double fr = (t - tn) / dt, fl = 1.0 - fr;
+    const double ds = sf - si;
+    const double o = fr * si + fl * sf + fl * fr * ((ka - ds) * fr + (ds - kb) * fl);
+    const double o3 = gsh(o, base, 3), o4 = gsh(o, base, 4), o5 = gsh(o, base, 5);
+    if (cl < 6) st[comp] = o;
+    if (cl == 0) {
+        st[6] = sqrt(o3 * o3 + o4 * o4 + o5 * o5 + 1.0);
+        st[7] = dt;
+    }
+}
+
+// The same particle loop as integrate_kernel (PI controller, static FP64
+// field in node rows), one particle per group of eight lanes, particles
+// taken from the second work-queue head S.qcount[1].  Particles whose
+// status is not running, or that were started and already reached the end
+// of the segment, are skipped.
+template <int METHOD, int MODE, int GEOM>
+__global__ void __launch_bounds__(kBlock)
+integrate_group_kernel(const DeviceFields F, const StepParams P, const DeviceParticles S,
+                       double t_stop)
+{
+    __shared__ double s_logt[128][3];
+    __shared__ unsigned long long s_expt[256];
+    for (int i = threadIdx.x; i < 128 * 3; i += blockDim.x)
+        (&s_logt[0][0])[i] = (&ptc_pow_log_tab[0][0])[i];
+    for (int i = threadIdx.x; i < 256; i += blockDim.x) s_expt[i] = ptc_pow_exp_tab[i];
+    __syncthreads();
+    const PowTables T{s_logt, s_expt};
+
+    const int lane = (int)__lane_id();
+    const int base = lane & ~(kGroup - 1), cl = lane & (kGroup - 1);
+    const int comp = cl < 6 ? cl : cl - 6;
+    const double t_end = t_stop < P.tmx ? t_stop : P.tmx;
+    const int64_t N = S.np;
+    GroupForce<MODE, GEOM> force(F, P, base, comp);
+    // this lane's state plane, chosen once (indexing the kernel argument's
+    // pointer array by a per-lane value would copy it to scratch)
+    double *xc = S.x[0];
+    xc = comp == 1 ? S.x[1] : xc;
+    xc = comp == 2 ? S.x[2] : xc;
+    xc = comp == 3 ? S.x[3] : xc;
+    xc = comp == 4 ? S.x[4] : xc;
+    xc = comp == 5 ? S.x[5] : xc;
+
+    bool active = false, exhausted = false, need_rhs = false, fresh = false;
+    int64_t idx = 0;
+    double si = 0.0, f1 = 0.0;
+    double t = 0.0, dt = 0.0, errold = 1.0e-04, tnext = 0.0;
+    int32_t n = 2, flags = 0, status = PTC_STATUS_RUNNING;
+    int64_t c_all = 0, c_acc = 0, c_rej = 0, c_rhs = 0;
+
+    for (;;) {
+        // ---- refill groups without a particle (one atomic per wave) ----
+        const bool want = !active && !exhausted;
+        const uint64_t wmask = __ballot(want && cl == 0);
+        if (wmask) {
+            const int first = __ffsll((unsigned long long)wmask) - 1;
+            int b0 = 0;
+            if (lane == first) b0 = atomicAdd(S.qcount + 1, __popcll(wmask));
+            b0 = __shfl(b0, first);
+            if (want) {
+                const int64_t my = (int64_t)b0 + __popcll(wmask & ((1ull << base) - 1ull));
+                if (my >= N) {
+                    exhausted = true;
+                } else {
+                    idx = S.order ? (int64_t)S.order[my] : my;
+                    status = S.status[idx];
+                    t = S.t[idx];
+                    flags = S.flags[idx];
+                    if (status == PTC_STATUS_RUNNING && (t < t_end || !(flags & FLAG_STARTED))) {
+                        active = true;
+                        fresh = true;
+                        need_rhs = true;
+                        si = xc[idx];
+                        dt = S.dt[idx];
+                        errold = S.errold[idx];
+                        n = S.nout[idx];
+                        c_all = S.cnt[idx];
+                        c_acc = S.cnt[N + idx];
+                        c_rej = S.cnt[2 * N + idx];
+                        c_rhs = S.nrhs[idx];
+                    }
+                }
+            }
+        }
+        if (!__any(active)) {
+            if (__all(exhausted)) break;
+            continue;
+        }
+
+        // ---- f(si) for new particles and after accepted steps ----
+        if (need_rhs) {
+            f1 = force(si);
+            c_rhs++;
+            need_rhs = false;
+        }
+
+        bool finish = false;
+        if (fresh) {
+            fresh = false;
+            if (!(flags & FLAG_STARTED)) {
+                // integrations.F90:120-127 / :352-359
+                dt = group_dti_guess(P, METHOD == PTC_RK5 ? 5 : 8, base, si, f1, T);
+                const double u3 = gsh(si, base, 3), u4 = gsh(si, base, 4), u5 = gsh(si, base, 5);
+                if (cl == 0) {
+                    double *o = S.out + (size_t)8 * P.m * idx;
+                    o[6] = sqrt(u3 * u3 + u4 * u4 + u5 * u5 + 1.0);
+                    o[7] = dt;
+                }
+                t = 0.0;
+                n = 2;
+                errold = 1.0e-04;
+                flags = FLAG_STARTED;
+                c_all = c_acc = c_rej = 0;
+            }
+            tnext = n <= P.m ? S.time[n - 1] : 0.0;
+            if (!(t < t_end)) {
+                if (!(t < P.tmx)) status = PTC_STATUS_DONE;
+                finish = true;
+            }
+        }
+
+        if (active && !finish) {
+            if (P.max_steps > 0 && c_all >= P.max_steps) {
+                status = PTC_STATUS_MAXSTEPS;
+                finish = true;
+            } else if (!isfinite(dt)) {
+                status = PTC_STATUS_NONFINITE;
+                finish = true;
+            } else if (!(dt > 0.0) || t + dt == t) {
+                status = PTC_STATUS_DTSMALL;
+                finish = true;
+            } else {
+                double sf = 0.0, dtn = 0.0;
+                if constexpr (METHOD == PTC_RK5) {
+                    double k6 = 0.0;
+                    const double del = group_attempt_rk5(force, P, base, si, f1, dt, sf, k6);
+                    c_rhs += 5;
+                    if (!isfinite(del) || !group_finite(sf, base)) {
+                        status = PTC_STATUS_NONFINITE;
+                        finish = true;
+                    } else {
+                        // integrations.F90:177,186
+                        const bool accept = del <= 1.0;
+                        const double q =
+                            P.safe * ptc_pow_with(del, accept ? -0.20 : -0.25, T.log, T.exp);
+                        if (accept) {
+                            t = t + dt;
+                            if (n <= P.m && t >= tnext) {
+                                const double k1 = dt * f1;
+                                do {
+                                    group_hermite_out(
+                                        S.out + (size_t)8 * ((n - 1) + (size_t)P.m * idx), comp,
+                                        cl, base, t, tnext, dt, si, sf, k1, k6);
+                                    n++;
+                                    tnext = n <= P.m ? S.time[n - 1] : 0.0;
+                                } while (n <= P.m && t >= tnext);
+                            }
+                            si = sf;
+                            c_acc++;
+                            dtn = dt * (P.fcmx < q ? P.fcmx : q);
+                            need_rhs = true;
+                        } else {
+                            c_rej++;
+                            dtn = dt * (P.fcmn > q ? P.fcmn : q);
+                        }
+                    }
+                } else {
+                    const double err = group_attempt_dp853(force, P, base, si, f1, dt, sf);
+                    c_rhs += 11;
+                    if (!isfinite(err) || !group_finite(sf, base)) {
+                        status = PTC_STATUS_NONFINITE;
+                        finish = true;
+                    } else {
+                        // integrations.F90:466,486
+                        const double qe = P.safe * ptc_pow_with(err, P.expo, T.log, T.exp);
+                        if (err <= 1.0) {
+                            t = t + dt;
+                            if (n <= P.m && t >= tnext) {
+                                const double fe = force(sf);
+                                c_rhs++;
+                                const double ka = dt * f1, kb = dt * fe;
+                                do {
+                                    group_hermite_out(
+                                        S.out + (size_t)8 * ((n - 1) + (size_t)P.m * idx), comp,
+                                        cl, base, t, tnext, dt, si, sf, ka, kb);
+                                    n++;
+                                    tnext = n <= P.m ? S.time[n - 1] : 0.0;
+                                } while (n <= P.m && t >= tnext);
+                                f1 = fe;
+                            } else {
+                                need_rhs = true;
+                            }
+                            si = sf;
+                            c_acc++;
+                            double q = qe;
+                            if (P.beta != 0.0) q = q * ptc_pow_with(errold, P.beta, T.log, T.exp);
+                            if (q < P.fcmn) q = P.fcmn;
+                            if (flags & FLAG_REJECTED) dtn = dt * (q < 1.0 ? q : 1.0);
+                            else dtn = dt * (q < P.fcmx ? q : P.fcmx);
+                            errold = err > 1.0e-04 ? err : 1.0e-04;
+                            flags &= ~FLAG_REJECTED;
+                        } else {
+                            c_rej++;
+                            dtn = dt * (P.fcmn > qe ? P.fcmn : qe);
+                            flags |= FLAG_REJECTED;
+                        }
+                    }
+                }
+                if (!finish) {
+                    c_all++;
+                    dt = dtn < P.dtm ? dtn : P.dtm;
+                    if (!(t < t_end)) {
+                        if (!(t < P.tmx)) status = PTC_STATUS_DONE;
+                        finish = true;
+                    }
+                }
+            }
+        }
+
+        if (active && finish) {
+            if (cl < 6) xc[idx] = si;
+            if (cl == 0) {
+                S.t[idx] = t;
+                S.dt[idx] = dt;
+                S.errold[idx] = errold;
+                S.nout[idx] = n;
+                S.flags[idx] = flags;
+                S.status[idx] = status;
+                S.cnt[idx] = c_all;
+                S.cnt[N + idx] = c_acc;
+                S.cnt[2 * N + idx] = c_rej;
+                S.nrhs[idx] = c_rhs;
+            }
+            active = false;
+            need_rhs = false;
+        }
+    }
+}
+
+template <int METHOD, int MODE, int GEOM>
+hipError_t launch_group_t(const DeviceFields &f, const StepParams &sp, const DeviceParticles &p,
+                          double t_stop, int block, hipStream_t stream)
+{
+    auto kern = integrate_group_kernel<METHOD, MODE, GEOM>;
+    int dev = 0, cus = 0, per_cu = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e != hipSuccess) return e;
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, block, 0);
+    if (e != hipSuccess) return e;
+    int64_t grid = (int64_t)cus * (per_cu > 0 ? per_cu : 1);
+    const int64_t need = (p.np * kGroup + block - 1) / block;
+    if (grid > need) grid = need;
+    if (grid < 1) grid = 1;
+    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(block), 0, stream, f, sp, p, t_stop);
+    return hipGetLastError();
+}
+
+}  // namespace
+}  // namespace ptc

@@ -24,7 +24,7 @@
 //     to the next step's stage 1) -- f(sf) is evaluated once and reused;
 //   * dti_guess evaluates f at a trial point and never uses it (:576);
 //   * a rejected attempt reuses f(si) (stage 1 does not depend on dt).
-#include "integrator_impl.hpp"
+#include "group_impl.hpp"
 
 namespace ptc {
 namespace {
@@ -133,6 +133,13 @@ __global__ void to_nodes_f32_kernel(const double *__restrict__ soa, int64_t node
 
 }  // namespace
 
+static hipError_t launch_integrate_lanes(int method, const DeviceFields &f, const StepParams &sp,
+                                         const DeviceParticles &p, double t_stop,
+                                         const LaunchConfig &lc, hipStream_t stream);
+static hipError_t launch_integrate_park(int method, const DeviceFields &f, const StepParams &sp,
+                                        const DeviceParticles &p, double t_stop,
+                                        const LaunchConfig &lc, hipStream_t stream);
+
 hipError_t launch_integrate(int method, const DeviceFields &f, const StepParams &sp,
                             const DeviceParticles &p, double t_stop, const LaunchConfig &lc,
                             hipStream_t stream)
@@ -153,6 +160,55 @@ hipError_t launch_integrate(int method, const DeviceFields &f, const StepParams 
 #undef SERIES
         return hipErrorInvalidValue;
     }
+    // FP64 static field in node rows with the PI controller: the group kernel
+    // (eight lanes per particle) takes the tail of the launch, or all of it
+    if (lc.tail != 0 && f.layout == LAYOUT_AOS && lc.ctrl == CTRL_PI) {
+        const int g = sp.pow2 ? 1 : 0;
+        DeviceParticles q = p;
+        q.park = lc.tail > 0 ? lc.tail : 0;
+        if (lc.tail != kTailGroupOnly) {
+            const hipError_t e = launch_integrate_park(method, f, sp, q, t_stop, lc, stream);
+            if (e != hipSuccess) return e;
+        }
+#define GROUP(M, MO, G)                                                        \
+    if (method == M && f.mode == MO && g == G)                                 \
+        return launch_group_t<M, MO, G>(f, sp, q, t_stop, kBlock, stream);
+        GROUP(PTC_RK5, PTC_MODE_VB, 0) GROUP(PTC_RK5, PTC_MODE_EB, 0)
+        GROUP(PTC_DOP853, PTC_MODE_VB, 0) GROUP(PTC_DOP853, PTC_MODE_EB, 0)
+        GROUP(PTC_RK5, PTC_MODE_VB, 1) GROUP(PTC_RK5, PTC_MODE_EB, 1)
+        GROUP(PTC_DOP853, PTC_MODE_VB, 1) GROUP(PTC_DOP853, PTC_MODE_EB, 1)
+#undef GROUP
+        return hipErrorInvalidValue;
+    }
+    DeviceParticles q = p;
+    q.park = 0;
+    return launch_integrate_lanes(method, f, sp, q, t_stop, lc, stream);
+}
+
+// the lane-per-particle kernels that hand their last particles over
+static hipError_t launch_integrate_park(int method, const DeviceFields &f, const StepParams &sp,
+                                        const DeviceParticles &p, double t_stop,
+                                        const LaunchConfig &lc, hipStream_t stream)
+{
+    const int geom = sp.pow2 && method == PTC_DOP853 && lc.cell_cache == 1 ? 1 : 0;
+#define PARKED(M, MO, C, G)                                                    \
+    if (method == M && f.mode == MO && lc.cell_cache == C && geom == G)        \
+        return launch_t<M, MO, LAYOUT_AOS, C, PTC_FP64, CTRL_PI, G, 0, 1>(f, sp, p, t_stop, lc, \
+                                                                          stream);
+    PARKED(PTC_RK5, PTC_MODE_VB, 1, 0) PARKED(PTC_RK5, PTC_MODE_EB, 1, 0)
+    PARKED(PTC_DOP853, PTC_MODE_VB, 1, 0) PARKED(PTC_DOP853, PTC_MODE_EB, 1, 0)
+    PARKED(PTC_DOP853, PTC_MODE_VB, 1, 1) PARKED(PTC_DOP853, PTC_MODE_EB, 1, 1)
+    PARKED(PTC_RK5, PTC_MODE_VB, 0, 0) PARKED(PTC_RK5, PTC_MODE_EB, 0, 0)
+    PARKED(PTC_DOP853, PTC_MODE_VB, 0, 0) PARKED(PTC_DOP853, PTC_MODE_EB, 0, 0)
+#undef PARKED
+    return hipErrorInvalidValue;
+}
+
+// the lane-per-particle kernels of a static FP64 field
+static hipError_t launch_integrate_lanes(int method, const DeviceFields &f, const StepParams &sp,
+                                         const DeviceParticles &p, double t_stop,
+                                         const LaunchConfig &lc, hipStream_t stream)
+{
     // FP64 power-of-two geometry (box sizes and velocity unit powers of two:
     // the six divisions by them become exact multiplications): DOP853 with
     // the cell cache and the PI controller only, where it spills less and

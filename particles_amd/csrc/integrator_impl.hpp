@@ -1065,10 +1065,20 @@ constexpr int static_split(int method, int ctrl, int cache, int prec, int time)
 
 // TIME = 1 integrates in a snapshot series (fields interpolated in time at
 // each stage; PI controller, node rows, cell cache as described at Force).
-template <int METHOD, int MODE, int LAYOUT, int CACHE, int PREC, int CTRL, int GEOM, int TIME = 0>
+// PARK = 1 (static FP64 node rows, PI controller): once the queue is empty
+// and at most S.park particles are still in flight (S.qcount[2] counts the
+// finished ones), every wave stores its particles as at the end of a segment
+// and stops; the group kernel (group_impl.hpp) finishes them.  A separate
+// instantiation, because the extra live state changes the register
+// allocation of the kernels that do not hand over.
+template <int METHOD, int MODE, int LAYOUT, int CACHE, int PREC, int CTRL, int GEOM, int TIME = 0,
+          int PARK = 0>
 __global__ void __launch_bounds__(kBlock)
 integrate_kernel(const DeviceFields F, const StepParams P, const DeviceParticles S, double t_stop)
 {
+    static_assert(PARK == 0 || (CTRL == CTRL_PI && TIME == 0 && PREC == PTC_FP64 &&
+                                LAYOUT == LAYOUT_AOS),
+                  "the group kernel takes static FP64 node rows with the PI controller");
     static_assert(CTRL == CTRL_PI || (CACHE == 1 && LAYOUT == LAYOUT_AOS),
                   "step-size features read the cell cache");
     static_assert(TIME == 0 || (CTRL == CTRL_PI && LAYOUT == LAYOUT_AOS),
@@ -1145,6 +1155,31 @@ integrate_kernel(const DeviceFields F, const StepParams P, const DeviceParticles
                         c_rej = S.cnt[2 * N + idx];
                         c_rhs = S.nrhs[idx];
                     }
+                }
+            }
+        }
+        if constexpr (PARK == 1) {
+            if (__any(exhausted)) {
+                int done = 0;
+                if (lane == 0)
+                    done = __hip_atomic_load(S.qcount + 2, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT);
+                done = __shfl(done, 0);
+                if (N - done <= (int64_t)S.park) {
+                    if (active) {
+#pragma unroll
+                        for (int c = 0; c < 6; c++) S.x[c][idx] = si[c];
+                        S.t[idx] = t;
+                        S.dt[idx] = dt;
+                        S.errold[idx] = errold;
+                        S.nout[idx] = n;
+                        S.flags[idx] = flags;
+                        S.cnt[idx] = c_all;
+                        S.cnt[N + idx] = c_acc;
+                        S.cnt[2 * N + idx] = c_rej;
+                        S.nrhs[idx] = c_rhs;
+                    }
+                    break;
                 }
             }
         }
@@ -1368,6 +1403,11 @@ integrate_kernel(const DeviceFields F, const StepParams P, const DeviceParticles
             }
         }
 
+        if constexpr (PARK == 1) {
+            const uint64_t fm = __ballot(active && finish);
+            if (fm && lane == __ffsll((unsigned long long)fm) - 1)
+                atomicAdd(S.qcount + 2, __popcll(fm));
+        }
         if (active && finish) {
 #pragma unroll
             for (int c = 0; c < 6; c++) S.x[c][idx] = si[c];
@@ -1494,11 +1534,12 @@ hipError_t launch_probe_t(const DeviceFields &f, const StepParams &sp, int op, i
     return hipGetLastError();
 }
 
-template <int METHOD, int MODE, int LAYOUT, int CACHE, int PREC, int CTRL, int GEOM, int TIME = 0>
+template <int METHOD, int MODE, int LAYOUT, int CACHE, int PREC, int CTRL, int GEOM, int TIME = 0,
+          int PARK = 0>
 hipError_t launch_t(const DeviceFields &f, const StepParams &sp, const DeviceParticles &p,
                     double t_stop, LaunchConfig lc, hipStream_t stream)
 {
-    auto kern = integrate_kernel<METHOD, MODE, LAYOUT, CACHE, PREC, CTRL, GEOM, TIME>;
+    auto kern = integrate_kernel<METHOD, MODE, LAYOUT, CACHE, PREC, CTRL, GEOM, TIME, PARK>;
     constexpr int kSplit = static_split(METHOD, CTRL, CACHE, PREC, TIME);
     const size_t lds = CTRL == CTRL_LEARNED ? kNetLds + (size_t)(lc.block / 64) * sizeof(MlpWaveBuf)
                        : kSplit > 0 ? (size_t)kSplit * 64 * sizeof(double) * (size_t)((lc.block + 63) / 64)

@@ -64,7 +64,10 @@ struct DeviceParticles {
     double *out;         // state(8, m, np), Fortran order
     const double *time;  // time(m)
     const int32_t *order;// work-queue permutation (nullptr = identity)
-    int32_t *qcount;     // work-queue head
+    int32_t *qcount;     // [0] work-queue head of the lane-per-particle kernel,
+                         // [1] of the group kernel, [2] particles the former finished
+    int32_t park;        // hand every particle in flight to the group kernel once the
+                         // queue is empty and at most this many are left (0 = never)
     // step-size controller: the reference's PI rule (CTRL_PI), the same with
     // sampled training records (CTRL_RECORD), or network proposals after
     // accepted steps with the PI rule after rejections (CTRL_LEARNED)
@@ -87,7 +90,12 @@ struct LaunchConfig {
     int grid_blocks;   // 0 = occupancy-derived
     int cell_cache;    // keep the last cell's corner values per lane (AOS only): 0 off, 1 on
     int ctrl;          // CTRL_*; RECORD and LEARNED need the AOS layout with the cell cache
+    int tail;          // FP64 static PI runs: 0 lane per particle only; N > 0 hand the
+                       // particles in flight to the group kernel once the queue is empty
+                       // and at most N are left; kTailGroupOnly: the group kernel only
 };
+constexpr int kTailGroupOnly = -1;
+constexpr int kTailAuto = -2;  // option value: chosen per batch (ptc_particles_advance)
 
 // Launches the integrator on `stream`; returns a hipError_t.  A series of
 // snapshots (f.nsnap > 0) takes the time-interpolating kernels (PI

@@ -49,8 +49,17 @@ def test_options_roundtrip_and_validation():
             P.set_option("block", "100")
         with pytest.raises(P.PtcError):
             P.set_option("no_such_option", "1")
+        assert P.get_option("tail") == "auto"
+        for v in ("0", "16", "8192", "group", "auto"):
+            P.set_option("tail", v)
+            assert P.get_option("tail") == v
+        for bad in ("-1", "2000000000", "x"):
+            with pytest.raises(P.PtcError):
+                P.set_option("tail", bad)
+        assert P.get_option("tail") == "auto"
     finally:
         P.set_option("layout", old)
+        P.set_option("tail", "auto")
 
 
 def test_fields_argument_validation_before_device():

@@ -17,7 +17,7 @@ import bench  # noqa: E402
 def _cfg(argv):
     import argparse
     ap = argparse.ArgumentParser()
-    for a, d in (("--workload", "turbulent256"), ("--layout", "aos"), ("--order", "cost"),
+    for a, d in (("--workload", "turbulent256"), ("--layout", "aos"), ("--order", "cost"), ("--tail", "auto"),
                  ("--prec", "fp64"), ("--method", "")):
         ap.add_argument(a, default=d)
     for a, d in (("--particles", 0), ("--grid", 0), ("--seed", 1), ("--cell-cache", 1),
@@ -29,7 +29,7 @@ def _cfg(argv):
         # the static default builds its field on the host: skip that here
         cfg = {"grid": 256, "particles": 100000, "method": "dop853", "tol": 1e-8,
                "layout": args.layout, "cell_cache": args.cell_cache, "field_precision": args.prec,
-               "sort": args.sort, "order": args.order, "field_snapshots": 0}
+               "sort": args.sort, "order": args.order, "field_snapshots": 0, "tail": args.tail}
         return cfg
     return bench.build_workload(args)[-1]
 
@@ -52,6 +52,8 @@ def test_traffic_matches_only_its_configuration():
     assert bench.traffic_from_profile(_cfg(["--layout", "soa"])) is None
     assert bench.traffic_from_profile(_cfg(["--order", "spatial"])) is None
     assert bench.traffic_from_profile(_cfg(["--prec", "fp32"])) is None
+    assert bench.traffic_from_profile(_cfg(["--tail", "16"])) is None
+    assert bench.traffic_from_profile(_cfg(["--tail", "0"])) is None
 
 
 def test_more_than_one_gpu_is_refused():

@@ -253,6 +253,7 @@ def test_cell_cache_modes_identical(oracle, method):
     st0 = W.initial_state(500, len(tg), seed=13, gamma_range=(1.0, 1000.0))
     so, co, _ = oracle.integrate(method, uu, bb, p, tg, st0)
     try:
+        P.set_option("tail", 0)  # the lane-per-particle kernel's cache
         for mode in (0, 1):
             P.set_option("cell_cache", mode)
             say(f"[cache] {method} cell_cache={mode}")
@@ -262,6 +263,7 @@ def test_cell_cache_modes_identical(oracle, method):
             assert np.array_equal(st, so) and np.array_equal(cnt, co), mode
     finally:
         P.set_option("cell_cache", 1)
+        P.set_option("tail", "auto")
 
 
 def test_launch_shape_does_not_change_results():
@@ -273,6 +275,7 @@ def test_launch_shape_does_not_change_results():
     st0 = W.initial_state(1000, len(tg), seed=10, gamma_range=(1.0, 1000.0))
     outs = []
     try:
+        P.set_option("tail", 0)  # the lane-per-particle kernel's launch shape
         for block, grid in ((256, 0), (64, 3), (128, 1)):
             P.set_option("block", block)
             P.set_option("grid_blocks", grid)
@@ -282,6 +285,7 @@ def test_launch_shape_does_not_change_results():
     finally:
         P.set_option("block", 256)
         P.set_option("grid_blocks", 0)
+        P.set_option("tail", "auto")
     for o in outs[1:]:
         assert np.array_equal(o[0], outs[0][0]) and np.array_equal(o[1], outs[0][1])
 
@@ -512,3 +516,85 @@ def test_graph_advances_equal_eager_advances():
             got = b.download()
     assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1])
     assert (got[2] == 0).all()
+
+
+@pytest.fixture(params=["0", "group", "600", "100"])
+def tail(request):
+    """Which kernel integrates: "0" one lane per particle, "group" the group
+    kernel (eight lanes per particle) alone, "N" the former handing every
+    particle still in flight to the latter once the work queue is empty and
+    at most N are left."""
+    P.set_option("tail", request.param)
+    yield request.param
+    P.set_option("tail", "auto")
+
+
+@pytest.mark.parametrize("method", ["rk5", "dop853"])
+def test_group_kernel_matches_oracle(oracle, method, tail):
+    """The group kernel forms every per-component operation as the lane-per-
+    particle kernel does and every sum over components in the same order on
+    every lane: snapshots and counters bit-identical to the oracle, in the
+    unit box (power-of-two geometry) and in a shifted box of size 2.5 x 3 x
+    1.75 with velocity unit 0.7, in VB and EB modes, in one call or in
+    segments."""
+    uu, bb = W.turbulent_vb(32, seed=41, nmodes=64)
+    p = W.params_vector(tmax=2 * W.T0, dtmax=1.0, atol=1e-8, rtol=1e-8)
+    tg = W.time_grid(4, p[2])
+    st0 = W.initial_state(700, len(tg), seed=42, gamma_range=(1.0, 1000.0))
+    say(f"[group] {method} tail={tail} unit box")
+    so, co, _ = oracle.integrate(method, uu, bb, p, tg, st0)
+    with P.Fields(uu, bb) as f:
+        st, cnt, status = P.integrate(f, method, p, tg, st0)
+        assert (status == 0).all()
+        assert np.array_equal(st, so) and np.array_equal(cnt, co)
+        with P.Particles(f, method, p, tg, st0) as b:
+            for t_stop in (0.3 * W.T0, 0.9 * W.T0, 1.4 * W.T0):
+                b.advance(t_stop)
+            b.advance()
+            s2, c2, st2 = b.download()
+        assert np.array_equal(s2, so) and np.array_equal(c2, co) and (st2 == 0).all()
+    g = W.params_vector(vun=0.7, tmax=1 * W.T0, dtmax=1.0, atol=1e-8, rtol=1e-8,
+                        xl=(-1.0, 0.5, 2.0), xs=(2.5, 3.0, 1.75))
+    tg = W.time_grid(3, g[2])
+    sg = W.initial_state(300, len(tg), seed=43, gamma_range=(1.0, 100.0),
+                         xl=(-1.0, 0.5, 2.0), xs=(2.5, 3.0, 1.75))
+    say(f"[group] {method} tail={tail} general box")
+    so, co, _ = oracle.integrate(method, uu, bb, g, tg, sg)
+    with P.Fields(uu, bb) as f:
+        st, cnt, status = P.integrate(f, method, g, tg, sg)
+    assert np.array_equal(st, so) and np.array_equal(cnt, co)
+    ee, be = W.reconnection_eb(32, seed=44, nmodes=64)
+    say(f"[group] {method} tail={tail} EB")
+    so, co, _ = oracle.integrate(method, ee, be, p, W.time_grid(4, p[2]), st0, mode="eb")
+    with P.Fields(ee, be, mode="eb") as f:
+        st, cnt, status = P.integrate(f, method, p, W.time_grid(4, p[2]), st0)
+    assert np.array_equal(st, so) and np.array_equal(cnt, co)
+
+
+def test_group_kernel_failure_handling():
+    """Status codes from the group kernel: a NaN field node retires the
+    particle that meets it (2), a step cap retires with status 1 after exactly
+    that many attempts, the others finish (0)."""
+    dm = (8, 8, 8)
+    uu = np.zeros(dm + (3,), order="F")
+    bb = np.zeros(dm + (3,), order="F")
+    bb[..., 2] = 1.0
+    bb[2, 4, 6, 0] = np.nan
+    p = W.params_vector(tmax=20.0, dtmax=1.0, atol=1e-8, rtol=1e-8)
+    tg = W.time_grid(2, p[2])
+    st0 = np.zeros((8, len(tg), 3), order="F")
+    st0[:6, 0, 0] = [0.27, 0.52, 0.77, 0.5, 0.0, 0.0]
+    st0[:6, 0, 1] = [0.9, 0.1, 0.1, 0.1, 0.0, 0.0]
+    st0[:6, 0, 2] = [0.9, 0.1, 0.2, 0.1, 0.0, 0.0]
+    try:
+        P.set_option("tail", "group")
+        with P.Fields(uu, bb) as f:
+            say("[group failure] NaN node")
+            _, cnt, status = P.integrate(f, "dop853", p, tg, st0)
+            assert status.tolist() == [2, 0, 0]
+            P.set_option("max_steps", 25)
+            _, cnt, status = P.integrate(f, "rk5", p, tg, st0)
+            assert status[1] == 1 and cnt[0, 1] == 25
+    finally:
+        P.set_option("max_steps", 0)
+        P.set_option("tail", "auto")
