@@ -435,8 +435,8 @@ static int resolve_tail(int tail, const ptc_particles *p)
 {
     if (tail != kTailAuto) return tail;
     const DeviceFields &f = p->fields->d;
-    if (f.prec != PTC_FP64 || f.nsnap > 0 || f.layout != LAYOUT_AOS || p->ctrl != CTRL_PI) return 0;
-    const double bytes = 48.0 * (double)f.nodes;
+    if (f.nsnap > 0 || f.layout != LAYOUT_AOS || p->ctrl != CTRL_PI) return 0;
+    const double bytes = (f.prec == PTC_FP32 ? 32.0 : 48.0) * (double)f.nodes;
     if (p->method == PTC_RK5 || p->d.np <= 32768 || bytes > 2.0 * 1024 * 1024 * 1024)
         return kTailGroupOnly;
     return 32768;

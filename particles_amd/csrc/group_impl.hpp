@@ -17,7 +17,9 @@
 // left (DeviceParticles::park); this kernel then finishes them.  It can also
 // run a whole launch (few particles, where every launch is tail).
 //
-// FP64 fields in node rows, static field, PI controller.
+// Static field in node rows, PI controller; FP64 fields (bit-identical) or
+// FP32 nodes (mixed precision: FP32 weights, force and slopes, FP64 stage
+// sums, as the mixed lane-per-particle kernel).
 #pragma once
 #include "integrator_impl.hpp"
 
@@ -27,7 +29,8 @@ namespace {
 constexpr int kGroup = 8;
 
 // component j of x across the group starting at lane base
-__device__ __forceinline__ double gsh(double x, int base, int j) { return __shfl(x, base + j); }
+template <class T>
+__device__ __forceinline__ T gsh(T x, int base, int j) { return __shfl(x, base + j); }
 
 // Component c of the Lorentz force (accelerations.F90:59-96, the operations
 // of force_from_fields) from the momentum (u0, u1, u2) of the stage point and
@@ -63,6 +66,37 @@ __device__ __forceinline__ double force_component(const StepParams &P, double u0
     return c < 3 ? lin : rot;
 }
 
+// Mixed precision: component c of the force in FP32 (the operations of
+// force_from_fields_f32: FP32 momentum, hardware reciprocal of gamma).
+template <int MODE>
+__device__ __forceinline__ float force_component_f32(const StepParams &P, double u0, double u1,
+                                                     double u2, float f0, float f1, float f2,
+                                                     float f3, float f4, float f5, int c)
+{
+    const float ux = float(u0), uy = float(u1), uz = float(u2);
+    const float rg = __builtin_amdgcn_rcpf(sqrtf(ux * ux + uy * uy + uz * uz + 1.0f));
+    const float v0 = ux * rg, v1 = uy * rg, v2 = uz * rg;
+    const float rvun = float(P.rvun), qom = float(P.qom);
+    const float vc = c == 0 ? v0 : c == 1 ? v1 : v2;
+    const float lin = vc * rvun;
+    const bool is3 = c == 3, is4 = c == 4;
+    const float b = is3 ? f5 : is4 ? f3 : f4;
+    const float e = is3 ? f4 : is4 ? f5 : f3;
+    float rot;
+    if constexpr (MODE == PTC_MODE_EB) {
+        const float a = is3 ? v1 : is4 ? v2 : v0;
+        const float d = is3 ? v2 : is4 ? v0 : v1;
+        const float ec = is3 ? f0 : is4 ? f1 : f2;
+        rot = qom * (ec + (a * b - d * e));
+    } else {
+        const float w0 = v0 - f0, w1 = v1 - f1, w2 = v2 - f2;
+        const float a = is3 ? w1 : is4 ? w2 : w0;
+        const float d = is3 ? w2 : is4 ? w0 : w1;
+        rot = qom * (a * b - d * e);
+    }
+    return c < 3 ? lin : rot;
+}
+
 // true if every component of x in the group is finite
 __device__ __forceinline__ bool group_finite(double x, int base)
 {
@@ -72,14 +106,16 @@ __device__ __forceinline__ bool group_finite(double x, int base)
 
 // The RHS as the group evaluates it: each lane keeps component comp of the
 // eight corner rows of the cell it last gathered from (the lane-per-particle
-// kernel's cell cache, split over the group).
-template <int MODE, int GEOM>
+// kernel's cell cache, split over the group).  PREC = PTC_FP32: FP32 node
+// rows of eight floats, FP32 weights, sums and force; the slope is a float.
+template <int MODE, int GEOM, int PREC>
 struct GroupForce {
+    using K = std::conditional_t<PREC == PTC_FP32, float, double>;
     const DeviceFields &F;
     const StepParams &P;
     const int base, comp;
     uint32_t key0 = 0xffffffffu, key7 = 0xffffffffu;
-    double v[8];
+    K v[8];
 
     __device__ __forceinline__ GroupForce(const DeviceFields &F_, const StepParams &P_, int b, int c)
         : F(F_), P(P_), base(b), comp(c)
@@ -87,35 +123,48 @@ struct GroupForce {
     }
 
     // component comp of f(s), s given by its component comp on each lane
-    __device__ __forceinline__ double operator()(double s)
+    __device__ __forceinline__ K operator()(double s)
     {
         double x[6];
 #pragma unroll
         for (int j = 0; j < 6; j++) x[j] = gsh(s, base, j);
-        Corners c;
-        corners<double, GEOM>(P, x, c);
+        CornersT<K> c;
+        corners<K, GEOM>(P, x, c);
         if (c.o[0] != key0 || c.o[7] != key7) {
-            const double *__restrict__ d = (const double *)F.data + comp;
+            if constexpr (PREC == PTC_FP32) {
+                const float *__restrict__ d = (const float *)F.data + comp;
 #pragma unroll
-            for (int j = 0; j < 8; j++) v[j] = d[(size_t)c.o[j] * 6];
+                for (int j = 0; j < 8; j++) v[j] = d[(size_t)c.o[j] * 8];
+            } else {
+                const double *__restrict__ d = (const double *)F.data + comp;
+#pragma unroll
+                for (int j = 0; j < 8; j++) v[j] = d[(size_t)c.o[j] * 6];
+            }
             key0 = c.o[0];
             key7 = c.o[7];
         }
         // interpolations.F90:212-220, corners summed left to right
-        double f = c.w[0] * v[0];
+        K f = c.w[0] * v[0];
 #pragma unroll
         for (int j = 1; j < 8; j++) f = f + c.w[j] * v[j];
-        return force_component<MODE, GEOM>(P, x[3], x[4], x[5], gsh(f, base, 0), gsh(f, base, 1),
-                                           gsh(f, base, 2), gsh(f, base, 3), gsh(f, base, 4),
-                                           gsh(f, base, 5), comp);
+        if constexpr (PREC == PTC_FP32)
+            return force_component_f32<MODE>(P, x[3], x[4], x[5], gsh(f, base, 0),
+                                             gsh(f, base, 1), gsh(f, base, 2), gsh(f, base, 3),
+                                             gsh(f, base, 4), gsh(f, base, 5), comp);
+        else
+            return force_component<MODE, GEOM>(P, x[3], x[4], x[5], gsh(f, base, 0),
+                                               gsh(f, base, 1), gsh(f, base, 2), gsh(f, base, 3),
+                                               gsh(f, base, 4), gsh(f, base, 5), comp);
     }
 };
 
 // dti_guess (integrations.F90:524-589) with the component sums formed over
 // the group; ki = f(si)
+template <class K>
 __device__ __forceinline__ double group_dti_guess(const StepParams &P, int iord, int base, double si,
-                                                  double ki, const PowTables &T)
+                                                  K kik, const PowTables &T)
 {
+    const double ki = double(kik);
     const double sr = P.atol + P.rtol * fabs(si);
     const double q0 = si / sr, q1 = ki / sr;
     double s = 0.0, q, d0, d1, d2, dti, dtf;
@@ -149,20 +198,21 @@ __device__ __forceinline__ double group_dti_guess(const StepParams &P, int iord,
 // Cash-Karp attempt (integrations.F90:131-148) on one component; returns del
 template <class GF>
 __device__ __forceinline__ double group_attempt_rk5(GF &rhsf, const StepParams &P, int base,
-                                                    double si, double f1, double dt, double &sf,
-                                                    double &k6d)
+                                                    double si, typename GF::K f1, double dt,
+                                                    double &sf, double &k6d)
 {
-    const double k1 = dt * f1;
+    using K = typename GF::K;
+    const double k1 = double(K(dt * double(f1)));
     double s = si + CK_A21 * k1;
-    const double k2 = dt * rhsf(s);
+    const double k2 = double(K(dt * double(rhsf(s))));
     s = si + CK_A31 * k1 + CK_A32 * k2;
-    const double k3 = dt * rhsf(s);
+    const double k3 = double(K(dt * double(rhsf(s))));
     s = si + CK_A41 * k1 + CK_A42 * k2 + CK_A43 * k3;
-    const double k4 = dt * rhsf(s);
+    const double k4 = double(K(dt * double(rhsf(s))));
     s = si + CK_A51 * k1 + CK_A52 * k2 + CK_A53 * k3 + CK_A54 * k4;
-    const double k5 = dt * rhsf(s);
+    const double k5 = double(K(dt * double(rhsf(s))));
     s = si + CK_A61 * k1 + CK_A62 * k2 + CK_A63 * k3 + CK_A64 * k4 + CK_A65 * k5;
-    const double k6 = dt * rhsf(s);
+    const double k6 = double(K(dt * double(rhsf(s))));
 
     const double eps = 2.2250738585072014e-308;  // tiny(1d0), integrations.F90:99
     double sc = 0.0;
@@ -195,33 +245,35 @@ __device__ __forceinline__ double group_attempt_rk5(GF &rhsf, const StepParams &
 // returns err
 template <class GF>
 __device__ __forceinline__ double group_attempt_dp853(GF &rhsf, const StepParams &P, int base,
-                                                      double si, double k1, double dt, double &sf)
+                                                      double si, typename GF::K k1k, double dt,
+                                                      double &sf)
 {
+    const double k1 = double(k1k);
     double ss = si + dt * A0201 * k1;
-    const double k2 = rhsf(ss);
+    const double k2 = double(rhsf(ss));
     ss = si + dt * (A0301 * k1 + A0302 * k2);
-    const double k3 = rhsf(ss);
+    const double k3 = double(rhsf(ss));
     ss = si + dt * (A0401 * k1 + A0403 * k3);
-    const double k4 = rhsf(ss);
+    const double k4 = double(rhsf(ss));
     ss = si + dt * (A0501 * k1 + A0503 * k3 + A0504 * k4);
-    const double k5 = rhsf(ss);
+    const double k5 = double(rhsf(ss));
     ss = si + dt * (A0601 * k1 + A0604 * k4 + A0605 * k5);
-    const double k6 = rhsf(ss);
+    const double k6 = double(rhsf(ss));
     ss = si + dt * (A0701 * k1 + A0704 * k4 + A0705 * k5 + A0706 * k6);
-    const double k7 = rhsf(ss);
+    const double k7 = double(rhsf(ss));
     ss = si + dt * (A0801 * k1 + A0804 * k4 + A0805 * k5 + A0806 * k6 + A0807 * k7);
-    const double k8 = rhsf(ss);
+    const double k8 = double(rhsf(ss));
     ss = si + dt * (A0901 * k1 + A0904 * k4 + A0905 * k5 + A0906 * k6 + A0907 * k7 + A0908 * k8);
-    const double k9 = rhsf(ss);
+    const double k9 = double(rhsf(ss));
     ss = si + dt * (A1001 * k1 + A1004 * k4 + A1005 * k5 + A1006 * k6 + A1007 * k7 + A1008 * k8 +
                     A1009 * k9);
-    const double k10 = rhsf(ss);
+    const double k10 = double(rhsf(ss));
     ss = si + dt * (A1101 * k1 + A1104 * k4 + A1105 * k5 + A1106 * k6 + A1107 * k7 + A1108 * k8 +
                     A1109 * k9 + A1110 * k10);
-    const double k11 = rhsf(ss);
+    const double k11 = double(rhsf(ss));
     ss = si + dt * (A1201 * k1 + A1204 * k4 + A1205 * k5 + A1206 * k6 + A1207 * k7 + A1208 * k8 +
                     A1209 * k9 + A1210 * k10 + A1211 * k11);
-    const double k12 = rhsf(ss);
+    const double k12 = double(rhsf(ss));
 
     const double kb = DB01 * k1 + DB06 * k6 + DB07 * k7 + DB08 * k8 + DB09 * k9 + DB10 * k10 +
                       DB11 * k11 + DB12 * k12;
@@ -269,7 +321,7 @@ __device__ __forceinline__ void group_hermite_out(double *st, int comp, int cl, 
 // taken from the second work-queue head S.qcount[1].  Particles whose
 // status is not running, or that were started and already reached the end
 // of the segment, are skipped.
-template <int METHOD, int MODE, int GEOM>
+template <int METHOD, int MODE, int GEOM, int PREC>
 __global__ void __launch_bounds__(kBlock)
 integrate_group_kernel(const DeviceFields F, const StepParams P, const DeviceParticles S,
                        double t_stop)
@@ -287,7 +339,8 @@ integrate_group_kernel(const DeviceFields F, const StepParams P, const DevicePar
     const int comp = cl < 6 ? cl : cl - 6;
     const double t_end = t_stop < P.tmx ? t_stop : P.tmx;
     const int64_t N = S.np;
-    GroupForce<MODE, GEOM> force(F, P, base, comp);
+    GroupForce<MODE, GEOM, PREC> force(F, P, base, comp);
+    using K = typename decltype(force)::K;
     // this lane's state plane, chosen once (indexing the kernel argument's
     // pointer array by a per-lane value would copy it to scratch)
     double *xc = S.x[0];
@@ -299,7 +352,8 @@ integrate_group_kernel(const DeviceFields F, const StepParams P, const DevicePar
 
     bool active = false, exhausted = false, need_rhs = false, fresh = false;
     int64_t idx = 0;
-    double si = 0.0, f1 = 0.0;
+    double si = 0.0;
+    K f1 = K(0);
     double t = 0.0, dt = 0.0, errold = 1.0e-04, tnext = 0.0;
     int32_t n = 2, flags = 0, status = PTC_STATUS_RUNNING;
     int64_t c_all = 0, c_acc = 0, c_rej = 0, c_rhs = 0;
@@ -402,7 +456,7 @@ integrate_group_kernel(const DeviceFields F, const StepParams P, const DevicePar
                         if (accept) {
                             t = t + dt;
                             if (n <= P.m && t >= tnext) {
-                                const double k1 = dt * f1;
+                                const double k1 = dt * double(f1);
                                 do {
                                     group_hermite_out(
                                         S.out + (size_t)8 * ((n - 1) + (size_t)P.m * idx), comp,
@@ -432,9 +486,9 @@ integrate_group_kernel(const DeviceFields F, const StepParams P, const DevicePar
                         if (err <= 1.0) {
                             t = t + dt;
                             if (n <= P.m && t >= tnext) {
-                                const double fe = force(sf);
+                                const K fe = force(sf);
                                 c_rhs++;
-                                const double ka = dt * f1, kb = dt * fe;
+                                const double ka = dt * double(f1), kb = dt * double(fe);
                                 do {
                                     group_hermite_out(
                                         S.out + (size_t)8 * ((n - 1) + (size_t)P.m * idx), comp,
@@ -493,11 +547,11 @@ integrate_group_kernel(const DeviceFields F, const StepParams P, const DevicePar
     }
 }
 
-template <int METHOD, int MODE, int GEOM>
+template <int METHOD, int MODE, int GEOM, int PREC>
 hipError_t launch_group_t(const DeviceFields &f, const StepParams &sp, const DeviceParticles &p,
                           double t_stop, int block, hipStream_t stream)
 {
-    auto kern = integrate_group_kernel<METHOD, MODE, GEOM>;
+    auto kern = integrate_group_kernel<METHOD, MODE, GEOM, PREC>;
     int dev = 0, cus = 0, per_cu = 0;
     hipError_t e = hipGetDevice(&dev);
     if (e != hipSuccess) return e;

@@ -172,7 +172,7 @@ hipError_t launch_integrate(int method, const DeviceFields &f, const StepParams 
         }
 #define GROUP(M, MO, G)                                                        \
     if (method == M && f.mode == MO && g == G)                                 \
-        return launch_group_t<M, MO, G>(f, sp, q, t_stop, kBlock, stream);
+        return launch_group_t<M, MO, G, PTC_FP64>(f, sp, q, t_stop, kBlock, stream);
         GROUP(PTC_RK5, PTC_MODE_VB, 0) GROUP(PTC_RK5, PTC_MODE_EB, 0)
         GROUP(PTC_DOP853, PTC_MODE_VB, 0) GROUP(PTC_DOP853, PTC_MODE_EB, 0)
         GROUP(PTC_RK5, PTC_MODE_VB, 1) GROUP(PTC_RK5, PTC_MODE_EB, 1)

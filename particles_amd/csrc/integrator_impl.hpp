@@ -1065,7 +1065,7 @@ constexpr int static_split(int method, int ctrl, int cache, int prec, int time)
 
 // TIME = 1 integrates in a snapshot series (fields interpolated in time at
 // each stage; PI controller, node rows, cell cache as described at Force).
-// PARK = 1 (static FP64 node rows, PI controller): once the queue is empty
+// PARK = 1 (static node rows, PI controller): once the queue is empty
 // and at most S.park particles are still in flight (S.qcount[2] counts the
 // finished ones), every wave stores its particles as at the end of a segment
 // and stops; the group kernel (group_impl.hpp) finishes them.  A separate
@@ -1076,9 +1076,8 @@ template <int METHOD, int MODE, int LAYOUT, int CACHE, int PREC, int CTRL, int G
 __global__ void __launch_bounds__(kBlock)
 integrate_kernel(const DeviceFields F, const StepParams P, const DeviceParticles S, double t_stop)
 {
-    static_assert(PARK == 0 || (CTRL == CTRL_PI && TIME == 0 && PREC == PTC_FP64 &&
-                                LAYOUT == LAYOUT_AOS),
-                  "the group kernel takes static FP64 node rows with the PI controller");
+    static_assert(PARK == 0 || (CTRL == CTRL_PI && TIME == 0 && LAYOUT == LAYOUT_AOS),
+                  "the group kernel takes static node rows with the PI controller");
     static_assert(CTRL == CTRL_PI || (CACHE == 1 && LAYOUT == LAYOUT_AOS),
                   "step-size features read the cell cache");
     static_assert(TIME == 0 || (CTRL == CTRL_PI && LAYOUT == LAYOUT_AOS),

@@ -4,7 +4,7 @@
 // compiled with FMA contraction: the mixed path is not bitwise comparable to
 // the reference anyway, and fused multiply-adds in the FP64 stage sums and
 // the FP32 gathers both cut instructions and round less.
-#include "integrator_impl.hpp"
+#include "group_impl.hpp"
 
 namespace ptc {
 
@@ -28,6 +28,43 @@ hipError_t launch_integrate_mixed(int method, const DeviceFields &f, const StepP
         SERIES_MODES(0, 0) SERIES_MODES(0, 1) SERIES_MODES(1, 0) SERIES_MODES(1, 1)
 #undef SERIES_MODES
 #undef SERIES
+        return hipErrorInvalidValue;
+    }
+    // PI controller, static field: the group kernel (eight lanes per
+    // particle) takes the tail of the launch, or all of it ("tail" option)
+    if (lc.tail != 0 && lc.ctrl == CTRL_PI) {
+        DeviceParticles q = p;
+        q.park = lc.tail > 0 ? lc.tail : 0;
+        if (lc.tail != kTailGroupOnly) {
+            hipError_t e = hipErrorInvalidValue;
+#define PARKED(M, MO, C, G)                                                    \
+    if (method == M && f.mode == MO && lc.cell_cache == C && geom == G)        \
+        e = launch_t<M, MO, LAYOUT_AOS, C, PTC_FP32, CTRL_PI, G, 0, 1>(f, sp, q, t_stop, lc, stream); \
+    else
+// without the cell cache the kernels take the general geometry
+#define PARKED0(M, MO)                                                         \
+    if (method == M && f.mode == MO && lc.cell_cache == 0)                     \
+        e = launch_t<M, MO, LAYOUT_AOS, 0, PTC_FP32, CTRL_PI, 0, 0, 1>(f, sp, q, t_stop, lc, stream); \
+    else
+            PARKED(PTC_RK5, PTC_MODE_VB, 1, 0) PARKED(PTC_RK5, PTC_MODE_EB, 1, 0)
+            PARKED(PTC_DOP853, PTC_MODE_VB, 1, 0) PARKED(PTC_DOP853, PTC_MODE_EB, 1, 0)
+            PARKED(PTC_RK5, PTC_MODE_VB, 1, 1) PARKED(PTC_RK5, PTC_MODE_EB, 1, 1)
+            PARKED(PTC_DOP853, PTC_MODE_VB, 1, 1) PARKED(PTC_DOP853, PTC_MODE_EB, 1, 1)
+            PARKED0(PTC_RK5, PTC_MODE_VB) PARKED0(PTC_RK5, PTC_MODE_EB)
+            PARKED0(PTC_DOP853, PTC_MODE_VB) PARKED0(PTC_DOP853, PTC_MODE_EB)
+            e = hipErrorInvalidValue;
+#undef PARKED0
+#undef PARKED
+            if (e != hipSuccess) return e;
+        }
+#define GROUP(M, MO, G)                                                        \
+    if (method == M && f.mode == MO && geom == G)                              \
+        return launch_group_t<M, MO, G, PTC_FP32>(f, sp, q, t_stop, kBlock, stream);
+        GROUP(PTC_RK5, PTC_MODE_VB, 0) GROUP(PTC_RK5, PTC_MODE_EB, 0)
+        GROUP(PTC_DOP853, PTC_MODE_VB, 0) GROUP(PTC_DOP853, PTC_MODE_EB, 0)
+        GROUP(PTC_RK5, PTC_MODE_VB, 1) GROUP(PTC_RK5, PTC_MODE_EB, 1)
+        GROUP(PTC_DOP853, PTC_MODE_VB, 1) GROUP(PTC_DOP853, PTC_MODE_EB, 1)
+#undef GROUP
         return hipErrorInvalidValue;
     }
 #define DISPATCH(M, MO, C, CT, G)                                              \
