@@ -308,7 +308,7 @@ int ptc_particles_use_mlp(ptc_particles *p, const ptc_mlp *net, double shift);
  *   "tail"        static fields with the PI controller: which kernel
  *                 integrates.  "0": one lane per particle (all 64 lanes of
  *                 a wave carry particles, each lane all six components);
- *                 "group": the group kernel, eight lanes per particle (one
+ *                 "group": the group kernel, six lanes per particle (one
  *                 component per lane; fewer instructions per step and two
  *                 waves per SIMD, so single trajectories and gathers from
  *                 large fields finish sooner); N > 0: one lane per particle

@@ -1,6 +1,6 @@
-// Group kernel: eight lanes integrate one particle, lane c of the group
-// holding component c (0..5) of the state and its slopes, lanes 6 and 7
-// mirroring components 0 and 1.  Same arithmetic as integrate_kernel, per
+// Group kernel: six lanes integrate one particle, lane c of the group
+// holding component c (0..5) of the state and its slopes; ten groups per
+// wave, lanes 60..63 idle.  Same arithmetic as integrate_kernel, per
 // component, and every sum over components (error norms, the momentum's
 // norm) is formed by every lane of the group from the six values shuffled
 // across it, in the same order, so every lane of a group holds the same
@@ -11,11 +11,13 @@
 // Why: one lane carrying a whole particle issues every component's
 // arithmetic in sequence, so the slowest trajectory of a launch runs at one
 // wave's issue rate and sets the launch time once the work queue is empty.
-// A group of eight issues about a third of the instructions per attempt, so
-// those last trajectories finish sooner.  The lane-per-particle kernel hands
-// its particles over once the queue is exhausted and a wave has few lanes
-// left (DeviceParticles::park); this kernel then finishes them.  It can also
-// run a whole launch (few particles, where every launch is tail).
+// A group issues about a third of the instructions per attempt on each of
+// its lanes, so those last trajectories finish sooner, and with 180
+// registers per lane two waves share a SIMD, which hides the latency of
+// gathers from fields too large for the caches.  The lane-per-particle
+// kernel hands its particles over once the queue is exhausted and few are
+// left in flight (DeviceParticles::park); this kernel then finishes them.
+// It can also run a whole launch.
 //
 // Static field in node rows, PI controller; FP64 fields (bit-identical) or
 // FP32 nodes (mixed precision: FP32 weights, force and slopes, FP64 stage
@@ -26,7 +28,9 @@
 namespace ptc {
 namespace {
 
-constexpr int kGroup = 8;
+// lanes per particle, and particles per wave (lanes 60..63 stay idle)
+constexpr int kGroup = 6;
+constexpr int kGroupsPerWave = 64 / kGroup;
 
 // component j of x across the group starting at lane base
 template <class T>
@@ -101,7 +105,7 @@ __device__ __forceinline__ float force_component_f32(const StepParams &P, double
 __device__ __forceinline__ bool group_finite(double x, int base)
 {
     const uint64_t bad = __ballot(!isfinite(x));
-    return ((bad >> base) & 0xffull) == 0;
+    return ((bad >> base) & 0x3full) == 0;
 }
 
 // The RHS as the group evaluates it: each lane keeps component comp of the
@@ -309,7 +313,7 @@ __device__ __forceinline__ void group_hermite_out(double *st, int comp, int cl, 
     const double ds = sf - si;
     const double o = fr * si + fl * sf + fl * fr * ((ka - ds) * fr + (ds - kb) * fl);
     const double o3 = gsh(o, base, 3), o4 = gsh(o, base, 4), o5 = gsh(o, base, 5);
-    if (cl < 6) st[comp] = o;
+    st[comp] = o;
     if (cl == 0) {
         st[6] = sqrt(o3 * o3 + o4 * o4 + o5 * o5 + 1.0);
         st[7] = dt;
@@ -317,7 +321,7 @@ __device__ __forceinline__ void group_hermite_out(double *st, int comp, int cl, 
 }
 
 // The same particle loop as integrate_kernel (PI controller, static FP64
-// field in node rows), one particle per group of eight lanes, particles
+// field in node rows), one particle per group of six lanes, particles
 // taken from the second work-queue head S.qcount[1].  Particles whose
 // status is not running, or that were started and already reached the end
 // of the segment, are skipped.
@@ -335,8 +339,8 @@ integrate_group_kernel(const DeviceFields F, const StepParams P, const DevicePar
     const PowTables T{s_logt, s_expt};
 
     const int lane = (int)__lane_id();
-    const int base = lane & ~(kGroup - 1), cl = lane & (kGroup - 1);
-    const int comp = cl < 6 ? cl : cl - 6;
+    const int base = (lane / kGroup) * kGroup, cl = lane - base;
+    const int comp = cl;
     const double t_end = t_stop < P.tmx ? t_stop : P.tmx;
     const int64_t N = S.np;
     GroupForce<MODE, GEOM, PREC> force(F, P, base, comp);
@@ -350,7 +354,9 @@ integrate_group_kernel(const DeviceFields F, const StepParams P, const DevicePar
     xc = comp == 4 ? S.x[4] : xc;
     xc = comp == 5 ? S.x[5] : xc;
 
-    bool active = false, exhausted = false, need_rhs = false, fresh = false;
+    // the lanes past the last whole group never take a particle
+    bool active = false, exhausted = lane >= kGroup * kGroupsPerWave, need_rhs = false,
+         fresh = false;
     int64_t idx = 0;
     double si = 0.0;
     K f1 = K(0);
@@ -528,7 +534,7 @@ integrate_group_kernel(const DeviceFields F, const StepParams P, const DevicePar
         }
 
         if (active && finish) {
-            if (cl < 6) xc[idx] = si;
+            xc[idx] = si;
             if (cl == 0) {
                 S.t[idx] = t;
                 S.dt[idx] = dt;
@@ -560,7 +566,8 @@ hipError_t launch_group_t(const DeviceFields &f, const StepParams &sp, const Dev
     e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, block, 0);
     if (e != hipSuccess) return e;
     int64_t grid = (int64_t)cus * (per_cu > 0 ? per_cu : 1);
-    const int64_t need = (p.np * kGroup + block - 1) / block;
+    const int64_t per_block = (int64_t)kGroupsPerWave * (block / 64);
+    const int64_t need = (p.np + per_block - 1) / per_block;
     if (grid > need) grid = need;
     if (grid < 1) grid = 1;
     hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(block), 0, stream, f, sp, p, t_stop);

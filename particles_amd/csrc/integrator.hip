@@ -161,7 +161,7 @@ hipError_t launch_integrate(int method, const DeviceFields &f, const StepParams 
         return hipErrorInvalidValue;
     }
     // FP64 static field in node rows with the PI controller: the group kernel
-    // (eight lanes per particle) takes the tail of the launch, or all of it
+    // (six lanes per particle) takes the tail of the launch, or all of it
     if (lc.tail != 0 && f.layout == LAYOUT_AOS && lc.ctrl == CTRL_PI) {
         const int g = sp.pow2 ? 1 : 0;
         DeviceParticles q = p;

@@ -30,7 +30,7 @@ hipError_t launch_integrate_mixed(int method, const DeviceFields &f, const StepP
 #undef SERIES
         return hipErrorInvalidValue;
     }
-    // PI controller, static field: the group kernel (eight lanes per
+    // PI controller, static field: the group kernel (six lanes per
     // particle) takes the tail of the launch, or all of it ("tail" option)
     if (lc.tail != 0 && lc.ctrl == CTRL_PI) {
         DeviceParticles q = p;

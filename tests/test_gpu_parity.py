@@ -522,7 +522,7 @@ def test_graph_advances_equal_eager_advances():
 @pytest.fixture(params=["0", "group", "600", "100"])
 def tail(request):
     """Which kernel integrates: "0" one lane per particle, "group" the group
-    kernel (eight lanes per particle) alone, "N" the former handing every
+    kernel (six lanes per particle) alone, "N" the former handing every
     particle still in flight to the latter once the work queue is empty and
     at most N are left."""
     P.set_option("tail", request.param)
