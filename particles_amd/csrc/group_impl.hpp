@@ -41,13 +41,18 @@ __device__ __forceinline__ T gsh(T x, int base, int j) { return __shfl(x, base +
 // the interpolated fields (U or E: f0..f2, B: f3..f5), known on every lane.
 // The operands are scalars selected per lane: selected from an array, they
 // would be indexed per lane and go through scratch.
+// The three divisions v = u / gamma are shared: lane c (and c + 3) of the
+// group divides component c % 3, and the quotients are shuffled.
 template <int MODE, int GEOM>
 __device__ __forceinline__ double force_component(const StepParams &P, double u0, double u1,
                                                   double u2, double f0, double f1, double f2,
-                                                  double f3, double f4, double f5, int c)
+                                                  double f3, double f4, double f5, int c,
+                                                  int base)
 {
     const double g = sqrt(u0 * u0 + u1 * u1 + u2 * u2 + 1.0);
-    const double v0 = u0 / g, v1 = u1 / g, v2 = u2 / g;
+    const int c3 = c < 3 ? c : c - 3;
+    const double q = (c3 == 0 ? u0 : c3 == 1 ? u1 : u2) / g;
+    const double v0 = gsh(q, base, 0), v1 = gsh(q, base, 1), v2 = gsh(q, base, 2);
     const double vc = c == 0 ? v0 : c == 1 ? v1 : v2;
     const double lin = GEOM ? vc * P.rvun : vc / P.vun;
     const bool is3 = c == 3, is4 = c == 4;
@@ -108,6 +113,59 @@ __device__ __forceinline__ bool group_finite(double x, int base)
     return ((bad >> base) & 0x3full) == 0;
 }
 
+// corners() (interpolations.F90:140-149, :207-210) with the per-axis work
+// shared: lane c (and c + 3) of the group computes axis c % 3 from the
+// stage point's position x, the group exchanges the left and right indices
+// and the in-cell fraction, and every lane forms the eight offsets and
+// weights from them in corners()'s order (same operations, same results).
+template <class W, int GEOM>
+__device__ __forceinline__ void group_corners(const StepParams &P, const double *x, int base,
+                                              int c, CornersT<W> &cr)
+{
+    const int a = c < 3 ? c : c - 3;
+    const double pos = a == 0 ? x[0] : a == 1 ? x[1] : x[2];
+    const double xl = a == 0 ? P.xl[0] : a == 1 ? P.xl[1] : P.xl[2];
+    const double inv_dm = a == 0 ? P.inv_dm[0] : a == 1 ? P.inv_dm[1] : P.inv_dm[2];
+    const double dmd = a == 0 ? P.dmd[0] : a == 1 ? P.dmd[1] : P.dmd[2];
+    const int dm = a == 0 ? P.dm[0] : a == 1 ? P.dm[1] : P.dm[2];
+    double xs;
+    if constexpr (GEOM) xs = a == 0 ? P.rxs[0] : a == 1 ? P.rxs[1] : P.rxs[2];
+    else xs = a == 0 ? P.xs[0] : a == 1 ? P.xs[1] : P.xs[2];
+    const double xa = GEOM ? (pos - xl) * xs : (pos - xl) / xs;
+    const double xq = xa + inv_dm;
+    double xp = dmd * (xq - floor(xq));
+    const int r = wrap_idx((int)floor(xp), dm);
+    xp = dmd * (xa - floor(xa));
+    const double pl = floor(xp);
+    const double xdm = xp - pl;
+    const int l = wrap_idx((int)pl, dm);
+    int li[3], ri[3];
+    double xd[3], xu[3];
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        li[j] = gsh(l, base, j);
+        ri[j] = gsh(r, base, j);
+        xd[j] = gsh(xdm, base, j);
+        xu[j] = 1.0 - xd[j];
+    }
+    const uint32_t sy = (uint32_t)P.dm[0], sz = (uint32_t)P.dm[0] * (uint32_t)P.dm[1];
+    const uint32_t yl = (uint32_t)li[1] * sy, yr = (uint32_t)ri[1] * sy;
+    const uint32_t zl = (uint32_t)li[2] * sz, zr = (uint32_t)ri[2] * sz;
+    const uint32_t x0 = (uint32_t)li[0], x1 = (uint32_t)ri[0];
+    cr.o[0] = x0 + yl + zl; cr.o[1] = x0 + yl + zr; cr.o[2] = x0 + yr + zl; cr.o[3] = x0 + yr + zr;
+    cr.o[4] = x1 + yl + zl; cr.o[5] = x1 + yl + zr; cr.o[6] = x1 + yr + zl; cr.o[7] = x1 + yr + zr;
+    const W u0 = W(xu[0]), u1 = W(xu[1]), u2 = W(xu[2]);
+    const W d0 = W(xd[0]), d1 = W(xd[1]), d2 = W(xd[2]);
+    cr.w[0] = u0 * u1 * u2;
+    cr.w[1] = u0 * u1 * d2;
+    cr.w[2] = u0 * d1 * u2;
+    cr.w[3] = u0 * d1 * d2;
+    cr.w[4] = d0 * u1 * u2;
+    cr.w[5] = d0 * u1 * d2;
+    cr.w[6] = d0 * d1 * u2;
+    cr.w[7] = d0 * d1 * d2;
+}
+
 // The RHS as the group evaluates it: each lane keeps component comp of the
 // eight corner rows of the cell it last gathered from (the lane-per-particle
 // kernel's cell cache, split over the group).  PREC = PTC_FP32: FP32 node
@@ -133,7 +191,7 @@ struct GroupForce {
 #pragma unroll
         for (int j = 0; j < 6; j++) x[j] = gsh(s, base, j);
         CornersT<K> c;
-        corners<K, GEOM>(P, x, c);
+        group_corners<K, GEOM>(P, x, base, comp, c);
         if (c.o[0] != key0 || c.o[7] != key7) {
             if constexpr (PREC == PTC_FP32) {
                 const float *__restrict__ d = (const float *)F.data + comp;
@@ -158,7 +216,7 @@ struct GroupForce {
         else
             return force_component<MODE, GEOM>(P, x[3], x[4], x[5], gsh(f, base, 0),
                                                gsh(f, base, 1), gsh(f, base, 2), gsh(f, base, 3),
-                                               gsh(f, base, 4), gsh(f, base, 5), comp);
+                                               gsh(f, base, 4), gsh(f, base, 5), comp, base);
     }
 };
 
