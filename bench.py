@@ -178,9 +178,8 @@ def traffic_from_profile(cfg):
 
 def _norm(cfg, key, defaults):
     """A configuration value for matching; the tail setting only selects
-    kernels for FP64 static fields in node rows, elsewhere it is moot."""
-    if key == "tail" and (cfg.get("field_snapshots", 0) or cfg.get("field_precision") != "fp64"
-                          or cfg.get("layout") != "aos"):
+    kernels for node rows ("aos"), elsewhere it is moot."""
+    if key == "tail" and cfg.get("layout") != "aos":
         return None
     return cfg.get(key, defaults.get(key))
 

@@ -305,7 +305,7 @@ int ptc_particles_use_mlp(ptc_particles *p, const ptc_mlp *net, double shift);
  *                 "spatial": Morton order of the particles' cells
  *                 (neighbouring lanes gather from neighbouring memory).
  *                 Results are identical for every order.
- *   "tail"        static fields with the PI controller: which kernel
+ *   "tail"        node rows with the PI controller: which kernel
  *                 integrates.  "0": one lane per particle (all 64 lanes of
  *                 a wave carry particles, each lane all six components);
  *                 "group": the group kernel, six lanes per particle (one
@@ -315,11 +315,13 @@ int ptc_particles_use_mlp(ptc_particles *p, const ptc_mlp *net, double shift);
  *                 until the queue is empty and at most N particles are in
  *                 flight, then the group kernel finishes them; "auto"
  *                 (default): the group kernel for RK5, for batches of at
- *                 most 32768 particles and for fields over 2 GB, else
- *                 one lane per particle (FP64: handing the last 32768
- *                 over).  Mixed precision (FP32 fields) takes the same
- *                 kernels in FP32.  Results are identical for every
- *                 setting (FP64) or within FP32 rounding of each other.
+ *                 most 32768 particles, for fields over 2 GB and for FP64
+ *                 snapshot series, else one lane per particle (FP64
+ *                 static fields: handing the last 32768 over; a series
+ *                 never hands over).  Mixed precision (FP32 fields) takes
+ *                 the same kernels in FP32.  Results are identical for
+ *                 every setting (FP64) or within FP32 rounding of each
+ *                 other.
  * Takes effect for the next create/advance. */
 int ptc_set_option(const char *key, const char *value);
 /* Reads an option back as text into buf (size len). */

@@ -19,7 +19,7 @@
 // left in flight (DeviceParticles::park); this kernel then finishes them.
 // It can also run a whole launch.
 //
-// Static field in node rows, PI controller; FP64 fields (bit-identical) or
+// Static field or snapshot series in node rows, PI controller; FP64 fields (bit-identical) or
 // FP32 nodes (mixed precision: FP32 weights, force and slopes, FP64 stage
 // sums, as the mixed lane-per-particle kernel).
 #pragma once
@@ -170,7 +170,7 @@ __device__ __forceinline__ void group_corners(const StepParams &P, const double 
 // eight corner rows of the cell it last gathered from (the lane-per-particle
 // kernel's cell cache, split over the group).  PREC = PTC_FP32: FP32 node
 // rows of eight floats, FP32 weights, sums and force; the slope is a float.
-template <int MODE, int GEOM, int PREC>
+template <int MODE, int GEOM, int PREC, int TIME = 0>
 struct GroupForce {
     using K = std::conditional_t<PREC == PTC_FP32, float, double>;
     const DeviceFields &F;
@@ -178,10 +178,64 @@ struct GroupForce {
     const int base, comp;
     uint32_t key0 = 0xffffffffu, key7 = 0xffffffffu;
     K v[8];
+    // TIME = 1 (a snapshot series, Force<..., TIME>): component comp of the
+    // cell's corner rows in snapshot k + 1 as well, the stage time, the
+    // lane's last interval and the cached interval key 2 k + (k + 1 loaded)
+    K v1[TIME ? 8 : 1];
+    double ts = 0.0;
+    int kh = 0, kc = -1;
 
     __device__ __forceinline__ GroupForce(const DeviceFields &F_, const StepParams &P_, int b, int c)
         : F(F_), P(P_), base(b), comp(c)
     {
+    }
+
+    __device__ __forceinline__ void at(double t)
+    {
+        if constexpr (TIME == 1) ts = t;
+    }
+
+    // the snapshot interval of the stage time, as Force::interval
+    __device__ __forceinline__ int interval(double &a, bool &two)
+    {
+        const int ns = F.nsnap;
+        const double *__restrict__ st = F.stime;
+        a = 0.0;
+        two = false;
+        if (ns <= 1 || !(ts > st[0])) return 0;
+        if (ts >= st[ns - 1]) return ns - 1;
+        int k = kh;
+        if (!(st[k] <= ts && ts < st[k + 1])) {
+            if (k + 2 < ns && st[k + 1] <= ts && ts < st[k + 2]) {
+                k = k + 1;
+            } else {
+                int lo = 0, hi = ns - 1;  // st[lo] <= ts < st[hi]
+                while (hi - lo > 1) {
+                    const int mid = (lo + hi) >> 1;
+                    if (st[mid] <= ts) lo = mid;
+                    else hi = mid;
+                }
+                k = lo;
+            }
+        }
+        kh = k;
+        a = (ts - st[k]) / (st[k + 1] - st[k]);
+        two = true;
+        return k;
+    }
+
+    // component comp of the eight corner rows of snapshot k at offsets o
+    __device__ __forceinline__ void load(const uint32_t *o, int k, K *dst) const
+    {
+        if constexpr (PREC == PTC_FP32) {
+            const float *__restrict__ d = (const float *)F.data + (size_t)k * F.sstride + comp;
+#pragma unroll
+            for (int j = 0; j < 8; j++) dst[j] = d[(size_t)o[j] * 8];
+        } else {
+            const double *__restrict__ d = (const double *)F.data + (size_t)k * F.sstride + comp;
+#pragma unroll
+            for (int j = 0; j < 8; j++) dst[j] = d[(size_t)o[j] * 6];
+        }
     }
 
     // component comp of f(s), s given by its component comp on each lane
@@ -192,23 +246,42 @@ struct GroupForce {
         for (int j = 0; j < 6; j++) x[j] = gsh(s, base, j);
         CornersT<K> c;
         group_corners<K, GEOM>(P, x, base, comp, c);
-        if (c.o[0] != key0 || c.o[7] != key7) {
-            if constexpr (PREC == PTC_FP32) {
-                const float *__restrict__ d = (const float *)F.data + comp;
-#pragma unroll
-                for (int j = 0; j < 8; j++) v[j] = d[(size_t)c.o[j] * 8];
-            } else {
-                const double *__restrict__ d = (const double *)F.data + comp;
-#pragma unroll
-                for (int j = 0; j < 8; j++) v[j] = d[(size_t)c.o[j] * 6];
+        K f;
+        if constexpr (TIME == 1) {
+            // the two bracketing snapshots' interpolants blended in time,
+            // as Force::fields_at
+            double a;
+            bool two;
+            const int k = interval(a, two);
+            const int key = 2 * k + (two ? 1 : 0);
+            if (!(c.o[0] == key0 && c.o[7] == key7 && (key == kc || (!two && key + 1 == kc)))) {
+                load(c.o, k, v);
+                if (two) load(c.o, k + 1, v1);
+                key0 = c.o[0];
+                key7 = c.o[7];
+                kc = key;
             }
-            key0 = c.o[0];
-            key7 = c.o[7];
-        }
-        // interpolations.F90:212-220, corners summed left to right
-        K f = c.w[0] * v[0];
+            f = c.w[0] * v[0];
 #pragma unroll
-        for (int j = 1; j < 8; j++) f = f + c.w[j] * v[j];
+            for (int j = 1; j < 8; j++) f = f + c.w[j] * v[j];
+            if (two) {
+                K g = c.w[0] * v1[0];
+#pragma unroll
+                for (int j = 1; j < 8; j++) g = g + c.w[j] * v1[j];
+                const K ak = K(a);
+                f = f + ak * (g - f);
+            }
+        } else {
+            if (c.o[0] != key0 || c.o[7] != key7) {
+                load(c.o, 0, v);
+                key0 = c.o[0];
+                key7 = c.o[7];
+            }
+            // interpolations.F90:212-220, corners summed left to right
+            f = c.w[0] * v[0];
+#pragma unroll
+            for (int j = 1; j < 8; j++) f = f + c.w[j] * v[j];
+        }
         if constexpr (PREC == PTC_FP32)
             return force_component_f32<MODE>(P, x[3], x[4], x[5], gsh(f, base, 0),
                                              gsh(f, base, 1), gsh(f, base, 2), gsh(f, base, 3),
@@ -260,20 +333,25 @@ __device__ __forceinline__ double group_dti_guess(const StepParams &P, int iord,
 // Cash-Karp attempt (integrations.F90:131-148) on one component; returns del
 template <class GF>
 __device__ __forceinline__ double group_attempt_rk5(GF &rhsf, const StepParams &P, int base,
-                                                    double si, typename GF::K f1, double dt,
-                                                    double &sf, double &k6d)
+                                                    double si, typename GF::K f1, double t,
+                                                    double dt, double &sf, double &k6d)
 {
     using K = typename GF::K;
     const double k1 = double(K(dt * double(f1)));
     double s = si + CK_A21 * k1;
+    rhsf.at(t + CK_N2 * dt);
     const double k2 = double(K(dt * double(rhsf(s))));
     s = si + CK_A31 * k1 + CK_A32 * k2;
+    rhsf.at(t + CK_N3 * dt);
     const double k3 = double(K(dt * double(rhsf(s))));
     s = si + CK_A41 * k1 + CK_A42 * k2 + CK_A43 * k3;
+    rhsf.at(t + CK_N4 * dt);
     const double k4 = double(K(dt * double(rhsf(s))));
     s = si + CK_A51 * k1 + CK_A52 * k2 + CK_A53 * k3 + CK_A54 * k4;
+    rhsf.at(t + dt);
     const double k5 = double(K(dt * double(rhsf(s))));
     s = si + CK_A61 * k1 + CK_A62 * k2 + CK_A63 * k3 + CK_A64 * k4 + CK_A65 * k5;
+    rhsf.at(t + CK_N6 * dt);
     const double k6 = double(K(dt * double(rhsf(s))));
 
     const double eps = 2.2250738585072014e-308;  // tiny(1d0), integrations.F90:99
@@ -307,34 +385,45 @@ __device__ __forceinline__ double group_attempt_rk5(GF &rhsf, const StepParams &
 // returns err
 template <class GF>
 __device__ __forceinline__ double group_attempt_dp853(GF &rhsf, const StepParams &P, int base,
-                                                      double si, typename GF::K k1k, double dt,
-                                                      double &sf)
+                                                      double si, typename GF::K k1k, double t,
+                                                      double dt, double &sf)
 {
     const double k1 = double(k1k);
     double ss = si + dt * A0201 * k1;
+    rhsf.at(t + DN02 * dt);
     const double k2 = double(rhsf(ss));
     ss = si + dt * (A0301 * k1 + A0302 * k2);
+    rhsf.at(t + DN03 * dt);
     const double k3 = double(rhsf(ss));
     ss = si + dt * (A0401 * k1 + A0403 * k3);
+    rhsf.at(t + DN04 * dt);
     const double k4 = double(rhsf(ss));
     ss = si + dt * (A0501 * k1 + A0503 * k3 + A0504 * k4);
+    rhsf.at(t + DN05 * dt);
     const double k5 = double(rhsf(ss));
     ss = si + dt * (A0601 * k1 + A0604 * k4 + A0605 * k5);
+    rhsf.at(t + DN06 * dt);
     const double k6 = double(rhsf(ss));
     ss = si + dt * (A0701 * k1 + A0704 * k4 + A0705 * k5 + A0706 * k6);
+    rhsf.at(t + DN07 * dt);
     const double k7 = double(rhsf(ss));
     ss = si + dt * (A0801 * k1 + A0804 * k4 + A0805 * k5 + A0806 * k6 + A0807 * k7);
+    rhsf.at(t + DN08 * dt);
     const double k8 = double(rhsf(ss));
     ss = si + dt * (A0901 * k1 + A0904 * k4 + A0905 * k5 + A0906 * k6 + A0907 * k7 + A0908 * k8);
+    rhsf.at(t + DN09 * dt);
     const double k9 = double(rhsf(ss));
     ss = si + dt * (A1001 * k1 + A1004 * k4 + A1005 * k5 + A1006 * k6 + A1007 * k7 + A1008 * k8 +
                     A1009 * k9);
+    rhsf.at(t + DN10 * dt);
     const double k10 = double(rhsf(ss));
     ss = si + dt * (A1101 * k1 + A1104 * k4 + A1105 * k5 + A1106 * k6 + A1107 * k7 + A1108 * k8 +
                     A1109 * k9 + A1110 * k10);
+    rhsf.at(t + DN11 * dt);
     const double k11 = double(rhsf(ss));
     ss = si + dt * (A1201 * k1 + A1204 * k4 + A1205 * k5 + A1206 * k6 + A1207 * k7 + A1208 * k8 +
                     A1209 * k9 + A1210 * k10 + A1211 * k11);
+    rhsf.at(t + dt);
     const double k12 = double(rhsf(ss));
 
     const double kb = DB01 * k1 + DB06 * k6 + DB07 * k7 + DB08 * k8 + DB09 * k9 + DB10 * k10 +
@@ -383,7 +472,7 @@ __device__ __forceinline__ void group_hermite_out(double *st, int comp, int cl, 
 // taken from the second work-queue head S.qcount[1].  Particles whose
 // status is not running, or that were started and already reached the end
 // of the segment, are skipped.
-template <int METHOD, int MODE, int GEOM, int PREC>
+template <int METHOD, int MODE, int GEOM, int PREC, int TIME>
 __global__ void __launch_bounds__(kBlock)
 integrate_group_kernel(const DeviceFields F, const StepParams P, const DeviceParticles S,
                        double t_stop)
@@ -401,7 +490,7 @@ integrate_group_kernel(const DeviceFields F, const StepParams P, const DevicePar
     const int comp = cl;
     const double t_end = t_stop < P.tmx ? t_stop : P.tmx;
     const int64_t N = S.np;
-    GroupForce<MODE, GEOM, PREC> force(F, P, base, comp);
+    GroupForce<MODE, GEOM, PREC, TIME> force(F, P, base, comp);
     using K = typename decltype(force)::K;
     // this lane's state plane, chosen once (indexing the kernel argument's
     // pointer array by a per-lane value would copy it to scratch)
@@ -463,6 +552,7 @@ integrate_group_kernel(const DeviceFields F, const StepParams P, const DevicePar
 
         // ---- f(si) for new particles and after accepted steps ----
         if (need_rhs) {
+            force.at(t);
             f1 = force(si);
             c_rhs++;
             need_rhs = false;
@@ -507,7 +597,7 @@ integrate_group_kernel(const DeviceFields F, const StepParams P, const DevicePar
                 double sf = 0.0, dtn = 0.0;
                 if constexpr (METHOD == PTC_RK5) {
                     double k6 = 0.0;
-                    const double del = group_attempt_rk5(force, P, base, si, f1, dt, sf, k6);
+                    const double del = group_attempt_rk5(force, P, base, si, f1, t, dt, sf, k6);
                     c_rhs += 5;
                     if (!isfinite(del) || !group_finite(sf, base)) {
                         status = PTC_STATUS_NONFINITE;
@@ -539,7 +629,7 @@ integrate_group_kernel(const DeviceFields F, const StepParams P, const DevicePar
                         }
                     }
                 } else {
-                    const double err = group_attempt_dp853(force, P, base, si, f1, dt, sf);
+                    const double err = group_attempt_dp853(force, P, base, si, f1, t, dt, sf);
                     c_rhs += 11;
                     if (!isfinite(err) || !group_finite(sf, base)) {
                         status = PTC_STATUS_NONFINITE;
@@ -550,6 +640,7 @@ integrate_group_kernel(const DeviceFields F, const StepParams P, const DevicePar
                         if (err <= 1.0) {
                             t = t + dt;
                             if (n <= P.m && t >= tnext) {
+                                force.at(t);
                                 const K fe = force(sf);
                                 c_rhs++;
                                 const double ka = dt * double(f1), kb = dt * double(fe);
@@ -611,11 +702,11 @@ integrate_group_kernel(const DeviceFields F, const StepParams P, const DevicePar
     }
 }
 
-template <int METHOD, int MODE, int GEOM, int PREC>
+template <int METHOD, int MODE, int GEOM, int PREC, int TIME = 0>
 hipError_t launch_group_t(const DeviceFields &f, const StepParams &sp, const DeviceParticles &p,
                           double t_stop, int block, hipStream_t stream)
 {
-    auto kern = integrate_group_kernel<METHOD, MODE, GEOM, PREC>;
+    auto kern = integrate_group_kernel<METHOD, MODE, GEOM, PREC, TIME>;
     int dev = 0, cus = 0, per_cu = 0;
     hipError_t e = hipGetDevice(&dev);
     if (e != hipSuccess) return e;

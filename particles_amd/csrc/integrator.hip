@@ -149,6 +149,18 @@ hipError_t launch_integrate(int method, const DeviceFields &f, const StepParams 
         // snapshot series: node rows, PI controller; the cell cache keeps
         // the second snapshot in LDS where the device has room for it
         if (f.layout != LAYOUT_AOS || lc.ctrl != CTRL_PI) return hipErrorInvalidValue;
+        if (lc.tail == kTailGroupOnly) {
+            const int g = sp.pow2 ? 1 : 0;
+#define GROUP_T(M, MO, G)                                                      \
+    if (method == M && f.mode == MO && g == G)                                 \
+        return launch_group_t<M, MO, G, PTC_FP64, 1>(f, sp, p, t_stop, kBlock, stream);
+            GROUP_T(PTC_RK5, PTC_MODE_VB, 0) GROUP_T(PTC_RK5, PTC_MODE_EB, 0)
+            GROUP_T(PTC_DOP853, PTC_MODE_VB, 0) GROUP_T(PTC_DOP853, PTC_MODE_EB, 0)
+            GROUP_T(PTC_RK5, PTC_MODE_VB, 1) GROUP_T(PTC_RK5, PTC_MODE_EB, 1)
+            GROUP_T(PTC_DOP853, PTC_MODE_VB, 1) GROUP_T(PTC_DOP853, PTC_MODE_EB, 1)
+#undef GROUP_T
+            return hipErrorInvalidValue;
+        }
         const int c = lc.cell_cache == 1 && series_cache_fits(PTC_FP64, lc.block) ? 1 : 0;
 #define SERIES(M, MO, C)                                                       \
     if (method == M && f.mode == MO && c == C)                                 \

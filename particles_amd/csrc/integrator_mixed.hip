@@ -18,6 +18,17 @@ hipError_t launch_integrate_mixed(int method, const DeviceFields &f, const StepP
         // snapshot series: PI controller; the cell cache keeps both
         // snapshots' corner values in registers
         if (lc.ctrl != CTRL_PI) return hipErrorInvalidValue;
+        if (lc.tail == kTailGroupOnly) {
+#define GROUP_T(M, MO, G)                                                      \
+    if (method == M && f.mode == MO && geom == G)                              \
+        return launch_group_t<M, MO, G, PTC_FP32, 1>(f, sp, p, t_stop, kBlock, stream);
+            GROUP_T(PTC_RK5, PTC_MODE_VB, 0) GROUP_T(PTC_RK5, PTC_MODE_EB, 0)
+            GROUP_T(PTC_DOP853, PTC_MODE_VB, 0) GROUP_T(PTC_DOP853, PTC_MODE_EB, 0)
+            GROUP_T(PTC_RK5, PTC_MODE_VB, 1) GROUP_T(PTC_RK5, PTC_MODE_EB, 1)
+            GROUP_T(PTC_DOP853, PTC_MODE_VB, 1) GROUP_T(PTC_DOP853, PTC_MODE_EB, 1)
+#undef GROUP_T
+            return hipErrorInvalidValue;
+        }
         const int c = lc.cell_cache == 1 ? 1 : 0;
 #define SERIES(M, MO, C, G)                                                    \
     if (method == M && f.mode == MO && c == C && geom == G)                    \

@@ -234,6 +234,53 @@ def test_device_series_matches_oracle(oracle, method, mode, cache):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("method", ["rk5", "dop853"])
+@pytest.mark.parametrize("mode", ["vb", "eb"])
+def test_group_kernel_series_matches_oracle(oracle, method, mode):
+    """The group kernel (six lanes per particle, "tail group") in a snapshot
+    series: FP64 bit-identical to the oracle's restatement (three
+    propagating snapshots, past the last snapshot time; and a shifted box of
+    non-power-of-two size with velocity unit 0.9), FP32 within the static
+    mixed-precision bound (median relative deviation 1e-6, every particle
+    1e-3 over 0.1 gyro-periods)."""
+    import particles_amd as P
+    tt = TIMES[:3]
+    uu, bb = W.turbulent_vb_series(24, seed=11, times=tt, speed=0.02, nmodes=128)
+    if mode == "eb":
+        uu = np.asfortranarray(0.05 * uu)
+    p = W.params_vector(tmax=8 * W.T0, dtmax=1.0, atol=1e-8, rtol=1e-8)
+    tg = W.time_grid(8, p[2])
+    st0 = W.initial_state(256, len(tg), seed=12, gamma_range=(1.0, 1e3))
+    xl, xs = (-0.5, 0.2, 0.1), (1.5, 2.0, 0.75)
+    g = W.params_vector(qom=1.0, vun=0.9, tmax=2 * W.T0, dtmax=1.0, atol=1e-8, rtol=1e-8,
+                        xl=xl, xs=xs)
+    tgg = W.time_grid(3, g[2])
+    sg = W.initial_state(256, len(tgg), seed=26, gamma_range=(1.0, 1e3), xl=xl, xs=xs)
+    q = W.params_vector(tmax=0.1 * W.T0, dtmax=1.0, atol=1e-8, rtol=1e-8)
+    tgq = W.time_grid(2, q[2])
+    say(f"[group series] {method} {mode}")
+    so, co, _ = oracle.integrate_series(method, uu, bb, tt, p, tg, st0, mode=mode)
+    sgo, cgo, _ = oracle.integrate_series(method, uu, bb, tt, g, tgg, sg, mode=mode)
+    sqo, _, _ = oracle.integrate_series(method, uu, bb, tt, q, tgq, st0[:, :len(tgq), :], mode=mode)
+    try:
+        P.set_option("tail", "group")
+        with device_series(P, uu, bb, tt, mode=mode) as f:
+            st, cnt, status = P.integrate(f, method, p, tg, st0)
+            stg, cntg, statusg = P.integrate(f, method, g, tgg, sg)
+        with device_series(P, uu, bb, tt, mode=mode, prec="fp32") as f:
+            s32, _, status32 = P.integrate(f, method, q, tgq, st0[:, :len(tgq), :])
+    finally:
+        P.set_option("tail", "auto")
+    assert (status == 0).all() and np.array_equal(st, so) and np.array_equal(cnt, co)
+    assert (statusg == 0).all() and np.array_equal(stg, sgo) and np.array_equal(cntg, cgo)
+    d = s32[:6] - sqo[:6]
+    dev = np.maximum(np.sqrt((d[:3] ** 2).sum(0)) / np.maximum(np.sqrt((sqo[:3] ** 2).sum(0)), 1),
+                     np.sqrt((d[3:6] ** 2).sum(0)) / np.maximum(np.sqrt((sqo[3:6] ** 2).sum(0)), 1)).max(0)
+    say(f"  fp32 deviation median {np.median(dev):.2e} max {dev.max():.2e}")
+    assert (status32 == 0).all() and np.median(dev) < 1e-6 and dev.max() < 1e-3
+
+
+@pytest.mark.gpu
 def test_device_series_many_snapshots_matches_oracle(oracle):
     """33 snapshots over the run: lanes that pick up a new particle start
     from an interval far from their last one (the lookup's bisection)."""
