@@ -201,8 +201,9 @@ def device_series(P, uu, bb, times, mode="vb", prec="fp64"):
 @pytest.mark.parametrize("cache", [0, 1])
 def test_device_series_matches_oracle(oracle, method, mode, cache):
     """Three propagating turbulent snapshots; the run goes past the last
-    snapshot time (held constant there).  With the cell cache (second
-    snapshot in LDS) also at 64 threads per block."""
+    snapshot time (held constant there).  One lane per particle ("tail 0");
+    with the cell cache (second snapshot in LDS) also at 64 threads per
+    block."""
     import particles_amd as P
     tt = TIMES[:3]
     uu, bb = W.turbulent_vb_series(24, seed=11, times=tt, speed=0.02, nmodes=128)
@@ -215,6 +216,7 @@ def test_device_series_matches_oracle(oracle, method, mode, cache):
     so, co, sto = oracle.integrate_series(method, uu, bb, tt, p, tg, st0, mode=mode)
     old = P.get_option("cell_cache"), P.get_option("block")
     try:
+        P.set_option("tail", 0)
         P.set_option("cell_cache", cache)
         runs = []
         with device_series(P, uu, bb, tt, mode=mode) as f:
@@ -224,6 +226,7 @@ def test_device_series_matches_oracle(oracle, method, mode, cache):
     finally:
         P.set_option("cell_cache", old[0])
         P.set_option("block", old[1])
+        P.set_option("tail", "auto")
     for block, (st, cnt, status) in runs:
         assert (status == sto).all() and (status == 0).all()
         same = np.array([np.array_equal(st[..., i], so[..., i]) and
