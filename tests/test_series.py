@@ -369,11 +369,21 @@ def test_device_series_general_geometry(oracle, prec):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("method", ["rk5", "dop853"])
-def test_mixed_series_close_to_oracle(oracle, method):
-    """FP32 series: the bound of the static mixed-precision test (median
-    relative deviation 1e-6, every particle 1e-3 over 0.1 gyro-periods),
-    and the ramp closed form to FP32 accuracy."""
+@pytest.mark.parametrize("tail", ["0", "group"])
+def test_mixed_series_close_to_oracle(oracle, method, tail):
+    """FP32 series, one lane per particle and the group kernel: the bound of
+    the static mixed-precision test (median relative deviation 1e-6, every
+    particle 1e-3 over 0.1 gyro-periods), and the ramp closed form to FP32
+    accuracy."""
     import particles_amd as P
+    P.set_option("tail", tail)
+    try:
+        _mixed_series_checks(P, oracle, method)
+    finally:
+        P.set_option("tail", "auto")
+
+
+def _mixed_series_checks(P, oracle, method):
     tt = np.array([0.0, 0.04, 0.1]) * W.T0
     uu, bb = W.turbulent_vb_series(24, seed=17, times=tt, speed=1.0, nmodes=128)
     p = W.params_vector(tmax=0.1 * W.T0, dtmax=1.0, atol=1e-8, rtol=1e-8)
