@@ -296,7 +296,9 @@ int ptc_particles_use_mlp(ptc_particles *p, const ptc_mlp *net, double shift);
  *   "cell_cache"  1 (default) keeps, per particle, the corner values of the
  *                 last cell it sampled (aos layout) in registers, so stage
  *                 points in the same cell skip the reload; 0 reloads every
- *                 time.  Results are identical either way.
+ *                 time.  Results are identical either way.  Applies to
+ *                 the one-lane-per-particle kernel; the group kernel
+ *                 ("tail") always keeps its cell.
  *   "sort"        N > 0: before every N-th advance (default 1), re-order
  *                 the work queue ("order"); 0: index order
  *   "order"       "cost" (default): most RHS evaluations in the previous
