@@ -11,17 +11,18 @@
 // Why: one lane carrying a whole particle issues every component's
 // arithmetic in sequence, so the slowest trajectory of a launch runs at one
 // wave's issue rate and sets the launch time once the work queue is empty.
-// A group issues about a third of the instructions per attempt on each of
-// its lanes, so those last trajectories finish sooner, and with 180
-// registers per lane two waves share a SIMD, which hides the latency of
-// gathers from fields too large for the caches.  The lane-per-particle
-// kernel hands its particles over once the queue is exhausted and few are
-// left in flight (DeviceParticles::park); this kernel then finishes them.
-// It can also run a whole launch.
+// A group issues about a quarter of the instructions per attempt on each of
+// its lanes (per-particle work such as the cell lookup and the divisions by
+// gamma is split over the group, not repeated), so those last trajectories
+// finish sooner, and with under 200 registers per lane two waves share a
+// SIMD, which hides the latency of gathers from fields too large for the
+// caches.  The lane-per-particle kernel hands its particles over once the
+// queue is exhausted and few are left in flight (DeviceParticles::park);
+// this kernel then finishes them.  It can also run a whole launch.
 //
-// Static field or snapshot series in node rows, PI controller; FP64 fields (bit-identical) or
-// FP32 nodes (mixed precision: FP32 weights, force and slopes, FP64 stage
-// sums, as the mixed lane-per-particle kernel).
+// Static field or snapshot series in node rows, PI controller; FP64 fields
+// (bit-identical) or FP32 nodes (mixed precision: FP32 weights, force and
+// slopes, FP64 stage sums, as the mixed lane-per-particle kernel).
 #pragma once
 #include "integrator_impl.hpp"
 
