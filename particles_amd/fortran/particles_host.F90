@@ -18,6 +18,8 @@
 !!   * counter.fits, time.fits (TUNIT) and state.fits (TEMP) as
 !!     particles.F90:365-371.
 !! Extra command-line switches: --mixed (FP32 fields, mixed precision),
+!! --kernels auto|lanes|group|N (which device kernels integrate: the
+!! library's "tail" option, "lanes" = one lane per particle; default auto),
 !! --dry-run (stop before the integration; needs no GPU).
 !! Extra keys (the reference's field is static): field_snapshots = N > 1
 !! reads N field snapshots, snapshot s from <name>_<ssss>.fits[.gz]
@@ -178,6 +180,7 @@ program particles_host
 
   character(len=512) :: pfile = './params.in', arg, path
   logical  :: ok, dry_run = .false., mixed = .false., exists
+  character(len=32) :: kernels = 'auto'
   integer  :: i, k, iret, nargs, naxis, bitpix, imethod, nbad, s
   integer  :: nfs = 1
   real(8)  :: fdt = 0.0d0
@@ -206,6 +209,14 @@ program particles_host
       dry_run = .true.
     case ('--mixed')
       mixed = .true.
+    case ('--kernels')
+      i = i + 1
+      call get_command_argument(i, kernels)
+      if (trim(kernels) == 'lanes') kernels = '0'
+      if (ptc_set_option('tail' // c_null_char, trim(kernels) // c_null_char) /= PTC_OK) then
+        write(*, '(a)') ' --kernels: ' // ptc_last_error()
+        stop 1
+      end if
     case default
       write(*, '(a)') ' unknown option ' // trim(arg)
       stop 1
@@ -325,6 +336,7 @@ program particles_host
   write(*, "(3x,'facmin facmax = ',2es13.5)") facmin, facmax
   write(*, "(3x,'safe beta     = ',2es13.5)") safe, beta
   write(*, "(3x,'precision     = ',a)") merge('mixed (FP32 fields)', 'FP64               ', mixed)
+  write(*, "(3x,'kernels       = ',a)") trim(kernels)
   if (nfs > 1) write(*, "(3x,'field snaps   = ',i0,' every ',es13.5)") nfs, fdt
   write(*, '(a)') ' Output:'
   write(*, "(3x,'time scale    = ',a)") trim(tscale)

@@ -112,6 +112,34 @@ def test_dry_run_errors(tmp_path):
     assert r.returncode != 0 and "missing field file" in r.stdout
 
 
+def test_dry_run_kernel_choice(tmp_path):
+    tmp = str(tmp_path)
+    _setup(tmp, PARAMS)
+    r = _run(tmp, "--dry-run")
+    assert r.returncode == 0 and re.search(r"kernels\s*=\s*auto", r.stdout)
+    r = _run(tmp, "--kernels", "group", "--dry-run")
+    assert r.returncode == 0 and re.search(r"kernels\s*=\s*group", r.stdout)
+    r = _run(tmp, "--kernels", "fastest", "--dry-run")
+    assert r.returncode != 0 and "--kernels" in r.stdout and "tail" in r.stdout
+
+
+@pytest.mark.gpu
+def test_full_run_kernel_choices_agree(tmp_path):
+    """--kernels lanes (one lane per particle) and --kernels group (six
+    lanes per particle) write bit-identical state and counter files."""
+    out = {}
+    for k in ("lanes", "group"):
+        tmp = os.path.join(str(tmp_path), k)
+        os.makedirs(tmp)
+        _setup(tmp, PARAMS.replace("nparticles  = 9", "nparticles  = 40"))
+        r = _run(tmp, "--kernels", k)
+        assert r.returncode == 0, r.stdout + r.stderr
+        out[k] = (fits.read(os.path.join(tmp, "state.fits")),
+                  fits.read(os.path.join(tmp, "counter.fits")))
+    assert np.array_equal(out["lanes"][0], out["group"][0])
+    assert np.array_equal(out["lanes"][1], out["group"][1])
+
+
 @pytest.mark.gpu
 def test_full_run_writes_reference_files(tmp_path):
     import particles_amd as P
