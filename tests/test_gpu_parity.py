@@ -493,6 +493,7 @@ def test_autotune_keeps_results():
             s, c, status = b.download()
     say(f"[autotune] chose {chosen}")
     assert chosen["block"] in (64, 128, 256)
+    assert chosen["tail"] in ("0", "group", "32768")
     assert np.array_equal(s, s_ref) and np.array_equal(c, c_ref) and (status == 0).all()
 
 
