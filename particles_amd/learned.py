@@ -2,7 +2,9 @@
 
 The reference sizes every step with its PI controller
 (sources/integrations.F90:177,186 for RK5, :466-490 for DOP853).  Here the
-PI controller stays the fallback and the judge:
+PI controller stays the fallback (after every rejection) and the floor of
+every proposal, and the embedded error estimate still accepts or rejects
+each step:
 
   1. ``record``: the PI-controlled integrator runs a training particle set
      and writes (features, next accepted step, error ratio) records on the
