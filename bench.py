@@ -331,10 +331,40 @@ def main():
                     help="time-dependent field: snapshot spacing in gyro-periods of B0")
     ap.add_argument("--shift", type=float, default=None,
                     help="added to the network's log step factor (default: calibrated)")
+    ap.add_argument("--copy-bandwidth", action="store_true",
+                    help="only time device-to-device copies and full reductions of a 4 GiB buffer "
+                         "(bytes per second: practical HBM figures next to the nominal 8 TB/s)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only copy a 1 GiB float64 device tensor three times (the known-size "
                          "copy that scales the rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
+    if args.copy_bandwidth:
+        import torch
+        a = torch.ones(1 << 29, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(20):
+            b.copy_(a)
+        e1.record()
+        torch.cuda.synchronize()
+        sec = e0.elapsed_time(e1) / 1e3
+        gbps = 20 * 2 * a.numel() * 8 / sec / 1e9
+        # read-only: a full reduction reads every byte once
+        r = a.sum()
+        e0.record()
+        for _ in range(20):
+            r = a.sum()
+        e1.record()
+        torch.cuda.synchronize()
+        rsec = e0.elapsed_time(e1) / 1e3
+        print(json.dumps({"bytes": a.numel() * 8, "passes": 20,
+                          "copy_read_plus_write_GBps": gbps,
+                          "sum_read_GBps": 20 * a.numel() * 8 / rsec / 1e9,
+                          "check": float(r)}), flush=True)
+        return
     if args.calibrate_copy:
         import torch
         a = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
