@@ -285,12 +285,18 @@ def variants(args, fields64, make_fields, p, tg, st0, method, plain):
 
 
 def summary(r, npart, steps, time_per_s_plain):
+    """One variant (FP32 node rows: 192 algorithmic bytes per RHS; the
+    learned controller's features come from the cell already gathered)."""
     tps = npart * steps / r["wall"]
+    bpr = BYTES_PER_RHS_FP64 // 2
+    achieved = bpr * r["rhs"] / (r["kernel_ms"] * 1e-3) / 1e9
     return {"value": r["acc"] / r["wall"], "unit": "accepted particle-steps/s",
             "particle_T0_per_s": tps, "time_throughput_vs_plain": tps / time_per_s_plain,
             "accepted": r["acc"], "rejected": r["rej"],
             "rejected_per_accepted": r["rej"] / max(r["acc"], 1),
-            "rhs_per_accepted": r["rhs"] / max(r["acc"], 1), "kernel_ms": r["kernel_ms"]}
+            "rhs_per_accepted": r["rhs"] / max(r["acc"], 1), "kernel_ms": r["kernel_ms"],
+            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "bytes_per_rhs": bpr}}
 
 
 def main():
