@@ -249,6 +249,11 @@ int ptc_mlp_get_params(const ptc_mlp *net, float *params);
 int ptc_mlp_set_params(ptc_mlp *net, const float *params);
 /* out[i] = network(features[i][0:16]), host arrays */
 int ptc_mlp_forward(const ptc_mlp *net, int64_t n, const float *features, float *out);
+/* The same outputs, evaluated as the six-lanes-per-particle integrator
+ * evaluates the learned controller (ten samples per wavefront in one
+ * 32-sample matrix tile); identical to ptc_mlp_forward.  No counterpart in
+ * the reference (its step size is the PI rule only). */
+int ptc_mlp_forward_groups(const ptc_mlp *net, int64_t n, const float *features, float *out);
 /* gradient (PTC_MLP_TRAINABLE floats, packed order) and value of the loss
  * mean((out - target)^2) over n host records */
 int ptc_mlp_gradient(const ptc_mlp *net, int64_t n, const float *records, float alpha,
@@ -323,7 +328,11 @@ int ptc_particles_use_mlp(ptc_particles *p, const ptc_mlp *net, double shift);
  *                 never hands over).  Mixed precision (FP32 fields) takes
  *                 the same kernels in FP32.  Results are identical for
  *                 every setting (FP64) or within FP32 rounding of each
- *                 other.
+ *                 other.  The learned controller (ptc_particles_use_mlp)
+ *                 on FP32 fields follows "group" and "auto" but never
+ *                 hands over (N > 0 runs one lane per particle); on FP64
+ *                 fields, and in record mode, it runs one lane per
+ *                 particle.
  * Takes effect for the next create/advance. */
 int ptc_set_option(const char *key, const char *value);
 /* Reads an option back as text into buf (size len). */

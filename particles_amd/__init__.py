@@ -87,6 +87,7 @@ def _load():
         "ptc_mlp_get_params": ([vp, vp], i32),
         "ptc_mlp_set_params": ([vp, vp], i32),
         "ptc_mlp_forward": ([vp, i64, vp, vp], i32),
+        "ptc_mlp_forward_groups": ([vp, i64, vp, vp], i32),
         "ptc_mlp_gradient": ([vp, i64, vp, C.c_float, vp, C.POINTER(dbl)], i32),
         "ptc_mlp_normalize": ([vp, i64, vp, i32], i32),
         "ptc_mlp_train": ([vp, i64, vp, i32, i32, i64, C.c_float, C.c_float, C.c_uint64, vp], i32),
@@ -452,10 +453,13 @@ class Mlp:
             raise ValueError(f"need {MLP_PARAMS} parameters, got {p.size}")
         _check(_lib.ptc_mlp_set_params(self._h, _ptr(p)))
 
-    def forward(self, features) -> np.ndarray:
+    def forward(self, features, groups: bool = False) -> np.ndarray:
+        """Network outputs for rows of 16 features; groups=True evaluates
+        them as the six-lanes-per-particle integrator does (same values)."""
         f = np.ascontiguousarray(features, dtype=np.float32).reshape(-1, 16)
         out = np.zeros(f.shape[0], dtype=np.float32)
-        _check(_lib.ptc_mlp_forward(self._h, f.shape[0], _ptr(f), _ptr(out)))
+        fn = _lib.ptc_mlp_forward_groups if groups else _lib.ptc_mlp_forward
+        _check(fn(self._h, f.shape[0], _ptr(f), _ptr(out)))
         return out
 
     def gradient(self, records, alpha: float = 0.0):

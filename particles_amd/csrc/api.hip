@@ -438,15 +438,20 @@ int ptc_fields_destroy(ptc_fields *f)
 // particles: DOP853 1.15x, RK5 2.3x the lane kernel; FP32 DOP853 0.62x).
 static int resolve_tail(int tail, const ptc_particles *p)
 {
-    if (tail != kTailAuto) return tail;
     const DeviceFields &f = p->fields->d;
-    if (f.layout != LAYOUT_AOS || p->ctrl != CTRL_PI) return 0;
+    // the learned controller runs in the group kernel on FP32 node rows
+    // only, and never hands over; training records come from one lane per
+    // particle
+    const bool learned = p->ctrl == CTRL_LEARNED;
+    if (learned && f.prec != PTC_FP32) return 0;
+    if (tail != kTailAuto) return learned && tail > 0 ? 0 : tail;
+    if (f.layout != LAYOUT_AOS || (p->ctrl != CTRL_PI && !learned)) return 0;
     const double snap = (f.prec == PTC_FP32 ? 32.0 : 48.0) * (double)f.nodes;
     const double bytes = f.nsnap > 0 ? 2.0 * snap : snap;
     if (p->method == PTC_RK5 || p->d.np <= 32768 || bytes > 2.0 * 1024 * 1024 * 1024 ||
         (f.nsnap > 0 && f.prec == PTC_FP64))
         return kTailGroupOnly;
-    if (f.nsnap > 0) return 0;
+    if (f.nsnap > 0 || learned) return 0;
     return f.prec == PTC_FP32 ? 0 : 32768;
 }
 
@@ -1178,22 +1183,34 @@ int ptc_mlp_set_params(ptc_mlp *net, const float *params)
     return PTC_OK;
 }
 
-int ptc_mlp_forward(const ptc_mlp *net, int64_t n, const float *features, float *out)
+static int mlp_forward_host(const ptc_mlp *net, int64_t n, const float *features, float *out,
+                            int groups, const char *who)
 {
-    if (!net) return fail(PTC_EARG, "ptc_mlp_forward: null handle");
-    if (n < 0) return fail(PTC_EARG, "ptc_mlp_forward: negative count");
+    if (!net) return fail(PTC_EARG, "%s: null handle", who);
+    if (n < 0) return fail(PTC_EARG, "%s: negative count", who);
     if (n == 0) { ok(); return PTC_OK; }
-    if (!features || !out) return fail(PTC_EARG, "ptc_mlp_forward: null array");
+    if (!features || !out) return fail(PTC_EARG, "%s: null array", who);
     float *d = nullptr;
     HIP_TRY(hipMalloc(&d, sizeof(float) * (kMlpIn + 1) * (size_t)n));
     hipError_t e = hipMemcpy(d, features, sizeof(float) * kMlpIn * n, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = mlp_forward(net->packed, n, d, kMlpIn, d + kMlpIn * n, net->stream);
+    if (e == hipSuccess)
+        e = mlp_forward(net->packed, n, d, kMlpIn, d + kMlpIn * n, net->stream, groups);
     if (e == hipSuccess) e = hipStreamSynchronize(net->stream);
     if (e == hipSuccess) e = hipMemcpy(out, d + kMlpIn * n, sizeof(float) * n, hipMemcpyDeviceToHost);
     (void)hipFree(d);
-    if (e != hipSuccess) return hip_fail(e, "ptc_mlp_forward");
+    if (e != hipSuccess) return hip_fail(e, who);
     ok();
     return PTC_OK;
+}
+
+int ptc_mlp_forward(const ptc_mlp *net, int64_t n, const float *features, float *out)
+{
+    return mlp_forward_host(net, n, features, out, 0, "ptc_mlp_forward");
+}
+
+int ptc_mlp_forward_groups(const ptc_mlp *net, int64_t n, const float *features, float *out)
+{
+    return mlp_forward_host(net, n, features, out, 1, "ptc_mlp_forward_groups");
 }
 
 int ptc_mlp_gradient(const ptc_mlp *net, int64_t n, const float *records, float alpha,

@@ -22,7 +22,8 @@
 //
 // Static field or snapshot series in node rows, PI controller; FP64 fields
 // (bit-identical) or FP32 nodes (mixed precision: FP32 weights, force and
-// slopes, FP64 stage sums, as the mixed lane-per-particle kernel).
+// slopes, FP64 stage sums, as the mixed lane-per-particle kernel).  A static
+// field of FP32 nodes can also take the learned step-size controller.
 #pragma once
 #include "integrator_impl.hpp"
 
@@ -119,9 +120,10 @@ __device__ __forceinline__ bool group_finite(double x, int base)
 // stage point's position x, the group exchanges the left and right indices
 // and the in-cell fraction, and every lane forms the eight offsets and
 // weights from them in corners()'s order (same operations, same results).
+// xdo (if given): the in-cell fraction of the three axes.
 template <class W, int GEOM>
 __device__ __forceinline__ void group_corners(const StepParams &P, const double *x, int base,
-                                              int c, CornersT<W> &cr)
+                                              int c, CornersT<W> &cr, float *xdo = nullptr)
 {
     const int a = c < 3 ? c : c - 3;
     const double pos = a == 0 ? x[0] : a == 1 ? x[1] : x[2];
@@ -148,6 +150,7 @@ __device__ __forceinline__ void group_corners(const StepParams &P, const double 
         ri[j] = gsh(r, base, j);
         xd[j] = gsh(xdm, base, j);
         xu[j] = 1.0 - xd[j];
+        if (xdo) xdo[j] = float(xd[j]);
     }
     const uint32_t sy = (uint32_t)P.dm[0], sz = (uint32_t)P.dm[0] * (uint32_t)P.dm[1];
     const uint32_t yl = (uint32_t)li[1] * sy, yr = (uint32_t)ri[1] * sy;
@@ -171,7 +174,9 @@ __device__ __forceinline__ void group_corners(const StepParams &P, const double 
 // eight corner rows of the cell it last gathered from (the lane-per-particle
 // kernel's cell cache, split over the group).  PREC = PTC_FP32: FP32 node
 // rows of eight floats, FP32 weights, sums and force; the slope is a float.
-template <int MODE, int GEOM, int PREC, int TIME = 0>
+// FEAT = 1 also keeps, from the last evaluation, component comp of the
+// interpolated fields and the in-cell fraction (step-size features).
+template <int MODE, int GEOM, int PREC, int TIME = 0, int FEAT = 0>
 struct GroupForce {
     using K = std::conditional_t<PREC == PTC_FP32, float, double>;
     const DeviceFields &F;
@@ -185,6 +190,8 @@ struct GroupForce {
     K v1[TIME ? 8 : 1];
     double ts = 0.0;
     int kh = 0, kc = -1;
+    K fint = K(0);
+    float xdl[3] = {0.0f, 0.0f, 0.0f};
 
     __device__ __forceinline__ GroupForce(const DeviceFields &F_, const StepParams &P_, int b, int c)
         : F(F_), P(P_), base(b), comp(c)
@@ -246,7 +253,7 @@ struct GroupForce {
 #pragma unroll
         for (int j = 0; j < 6; j++) x[j] = gsh(s, base, j);
         CornersT<K> c;
-        group_corners<K, GEOM>(P, x, base, comp, c);
+        group_corners<K, GEOM>(P, x, base, comp, c, FEAT ? xdl : nullptr);
         K f;
         if constexpr (TIME == 1) {
             // the two bracketing snapshots' interpolants blended in time,
@@ -283,6 +290,7 @@ struct GroupForce {
 #pragma unroll
             for (int j = 1; j < 8; j++) f = f + c.w[j] * v[j];
         }
+        if constexpr (FEAT == 1) fint = f;
         if constexpr (PREC == PTC_FP32)
             return force_component_f32<MODE>(P, x[3], x[4], x[5], gsh(f, base, 0),
                                              gsh(f, base, 1), gsh(f, base, 2), gsh(f, base, 3),
@@ -468,21 +476,53 @@ __device__ __forceinline__ void group_hermite_out(double *st, int comp, int cl, 
     }
 }
 
-// The same particle loop as integrate_kernel (PI controller, static FP64
-// field in node rows), one particle per group of six lanes, particles
+// The step-size features (features_bc) of the group's accepted state si,
+// on every lane of the group, from the force's last evaluation, which was
+// at si: the interpolated fields and the in-cell fraction it kept, and |B|
+// at the eight corners from the B components held by lanes 3..5.
+template <int MODE, class GF>
+__device__ __forceinline__ void group_features(const GF &force, const StepParams &P, double si,
+                                               int base, double h, float err, float pi_ratio,
+                                               bool rej, float *f)
+{
+    const double s[6] = {0.0, 0.0, 0.0, gsh(si, base, 3), gsh(si, base, 4), gsh(si, base, 5)};
+    float ub[6], bc[8];
+#pragma unroll
+    for (int j = 0; j < 6; j++) ub[j] = float(gsh(force.fint, base, j));
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+        const float b = float(force.v[j]);
+        bc[j] = corner_bmag(gsh(b, base, 3), gsh(b, base, 4), gsh(b, base, 5));
+    }
+    features_bc<MODE>(P, s, ub, bc, force.xdl, h, err, pi_ratio, rej, f);
+}
+
+// The same particle loop as integrate_kernel (static field or snapshot
+// series in node rows), one particle per group of six lanes, particles
 // taken from the second work-queue head S.qcount[1].  Particles whose
 // status is not running, or that were started and already reached the end
-// of the segment, are skipped.
-template <int METHOD, int MODE, int GEOM, int PREC, int TIME>
+// of the segment, are skipped.  CTRL_PI: the reference's controller;
+// CTRL_LEARNED (static FP32 nodes): after an accepted step the network
+// proposes the next step, as in integrate_kernel, evaluated once per wave
+// for its ten particles (mlp_forward_groups; the network and a 64x72 BF16
+// buffer per wave in dynamic LDS).
+template <int METHOD, int MODE, int GEOM, int PREC, int TIME, int CTRL = CTRL_PI>
 __global__ void __launch_bounds__(kBlock)
 integrate_group_kernel(const DeviceFields F, const StepParams P, const DeviceParticles S,
                        double t_stop)
 {
+    static_assert(CTRL == CTRL_PI || (CTRL == CTRL_LEARNED && TIME == 0 && PREC == PTC_FP32),
+                  "the group kernel's learned controller takes static FP32 node rows");
     __shared__ double s_logt[128][3];
     __shared__ unsigned long long s_expt[256];
     for (int i = threadIdx.x; i < 128 * 3; i += blockDim.x)
         (&s_logt[0][0])[i] = (&ptc_pow_log_tab[0][0])[i];
     for (int i = threadIdx.x; i < 256; i += blockDim.x) s_expt[i] = ptc_pow_exp_tab[i];
+    extern __shared__ __attribute__((aligned(16))) char s_dyn[];
+    if constexpr (CTRL == CTRL_LEARNED) {
+        for (int i = threadIdx.x; i < (int)(sizeof(MlpInfer) / 4); i += blockDim.x)
+            ((uint32_t *)s_dyn)[i] = ((const uint32_t *)&S.net->f)[i];
+    }
     __syncthreads();
     const PowTables T{s_logt, s_expt};
 
@@ -491,7 +531,7 @@ integrate_group_kernel(const DeviceFields F, const StepParams P, const DevicePar
     const int comp = cl;
     const double t_end = t_stop < P.tmx ? t_stop : P.tmx;
     const int64_t N = S.np;
-    GroupForce<MODE, GEOM, PREC, TIME> force(F, P, base, comp);
+    GroupForce<MODE, GEOM, PREC, TIME, CTRL == CTRL_LEARNED ? 1 : 0> force(F, P, base, comp);
     using K = typename decltype(force)::K;
     // this lane's state plane, chosen once (indexing the kernel argument's
     // pointer array by a per-lane value would copy it to scratch)
@@ -511,6 +551,11 @@ integrate_group_kernel(const DeviceFields F, const StepParams P, const DevicePar
     double t = 0.0, dt = 0.0, errold = 1.0e-04, tnext = 0.0;
     int32_t n = 2, flags = 0, status = PTC_STATUS_RUNNING;
     int64_t c_all = 0, c_acc = 0, c_rej = 0, c_rhs = 0;
+    // learned controller: features are taken at the top of the iteration
+    // after an accepted step of size h_acc with error ratio err_acc
+    bool want_feat = false, rej_since = false, rej_before = false;
+    double h_acc = 0.0;
+    float err_acc = 0.0f, pi_ratio = 0.0f;
 
     for (;;) {
         // ---- refill groups without a particle (one atomic per wave) ----
@@ -534,6 +579,7 @@ integrate_group_kernel(const DeviceFields F, const StepParams P, const DevicePar
                         active = true;
                         fresh = true;
                         need_rhs = true;
+                        want_feat = rej_since = false;
                         si = xc[idx];
                         dt = S.dt[idx];
                         errold = S.errold[idx];
@@ -557,6 +603,36 @@ integrate_group_kernel(const DeviceFields F, const StepParams P, const DevicePar
             f1 = force(si);
             c_rhs++;
             need_rhs = false;
+        }
+
+        if constexpr (CTRL == CTRL_LEARNED) {
+            // (the force's last evaluation was at si: the one above, or the
+            // end slope of the accepted step, which became si)
+            if (__any(want_feat)) {
+                float feat[kFeatures];
+                if (want_feat) {
+                    group_features<MODE>(force, P, si, base, h_acc, err_acc, pi_ratio, rej_before,
+                                         feat);
+                } else {
+#pragma unroll
+                    for (int i = 0; i < kFeatures; i++) feat[i] = 0.0f;
+                }
+                const MlpInfer &W = *(const MlpInfer *)s_dyn;
+                MlpWaveBuf &wb = ((MlpWaveBuf *)(s_dyn + kNetLds))[threadIdx.x >> 6];
+                const float y = mlp_forward_groups(W, wb, feat);
+                if (want_feat && isfinite(y)) {
+                    // as integrate_kernel: growth at most facmax, never below
+                    // facmin times the PI proposal
+                    float lg = y + S.ctrl_shift;
+                    const float lfmn = logf(float(P.fcmn)), hi = logf(float(P.fcmx));
+                    const float lo = pi_ratio + lfmn > lfmn ? pi_ratio + lfmn : lfmn;
+                    lg = lg < lo ? lo : lg;
+                    lg = lg > hi ? hi : lg;
+                    const double dtl = h_acc * double(__expf(lg));
+                    dt = dtl < P.dtm ? dtl : P.dtm;
+                }
+            }
+            want_feat = false;
         }
 
         bool finish = false;
@@ -621,11 +697,15 @@ integrate_group_kernel(const DeviceFields F, const StepParams P, const DevicePar
                                 } while (n <= P.m && t >= tnext);
                             }
                             si = sf;
+                            if constexpr (CTRL != CTRL_PI)
+                                on_accept<CTRL>(S, idx, flags, c_rej, dt, del, 5.0f, rej_since,
+                                                want_feat, h_acc, err_acc, rej_before);
                             c_acc++;
                             dtn = dt * (P.fcmx < q ? P.fcmx : q);
                             need_rhs = true;
                         } else {
                             c_rej++;
+                            rej_since = true;
                             dtn = dt * (P.fcmn > q ? P.fcmn : q);
                         }
                     }
@@ -657,6 +737,10 @@ integrate_group_kernel(const DeviceFields F, const StepParams P, const DevicePar
                                 need_rhs = true;
                             }
                             si = sf;
+                            if constexpr (CTRL != CTRL_PI)
+                                on_accept<CTRL>(S, idx, flags, c_rej, dt, err,
+                                                float(-1.0 / P.expo), rej_since, want_feat, h_acc,
+                                                err_acc, rej_before);
                             c_acc++;
                             double q = qe;
                             if (P.beta != 0.0) q = q * ptc_pow_with(errold, P.beta, T.log, T.exp);
@@ -667,6 +751,7 @@ integrate_group_kernel(const DeviceFields F, const StepParams P, const DevicePar
                             flags &= ~FLAG_REJECTED;
                         } else {
                             c_rej++;
+                            rej_since = true;
                             dtn = dt * (P.fcmn > qe ? P.fcmn : qe);
                             flags |= FLAG_REJECTED;
                         }
@@ -675,6 +760,9 @@ integrate_group_kernel(const DeviceFields F, const StepParams P, const DevicePar
                 if (!finish) {
                     c_all++;
                     dt = dtn < P.dtm ? dtn : P.dtm;
+                    if constexpr (CTRL != CTRL_PI) {
+                        if (want_feat) pi_ratio = float(log(dt / h_acc));
+                    }
                     if (!(t < t_end)) {
                         if (!(t < P.tmx)) status = PTC_STATUS_DONE;
                         finish = true;
@@ -699,28 +787,37 @@ integrate_group_kernel(const DeviceFields F, const StepParams P, const DevicePar
             }
             active = false;
             need_rhs = false;
+            want_feat = false;
         }
     }
 }
 
-template <int METHOD, int MODE, int GEOM, int PREC, int TIME = 0>
+template <int METHOD, int MODE, int GEOM, int PREC, int TIME = 0, int CTRL = CTRL_PI>
 hipError_t launch_group_t(const DeviceFields &f, const StepParams &sp, const DeviceParticles &p,
                           double t_stop, int block, hipStream_t stream)
 {
-    auto kern = integrate_group_kernel<METHOD, MODE, GEOM, PREC, TIME>;
+    auto kern = integrate_group_kernel<METHOD, MODE, GEOM, PREC, TIME, CTRL>;
+    const size_t lds =
+        CTRL == CTRL_LEARNED ? kNetLds + (size_t)(block / 64) * sizeof(MlpWaveBuf) : 0;
+    if (CTRL == CTRL_LEARNED && !p.net) return hipErrorInvalidValue;
+    if (lds > 65536) {
+        hipError_t e = hipFuncSetAttribute((const void *)kern,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
     int dev = 0, cus = 0, per_cu = 0;
     hipError_t e = hipGetDevice(&dev);
     if (e != hipSuccess) return e;
     e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     if (e != hipSuccess) return e;
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, block, 0);
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, block, lds);
     if (e != hipSuccess) return e;
     int64_t grid = (int64_t)cus * (per_cu > 0 ? per_cu : 1);
     const int64_t per_block = (int64_t)kGroupsPerWave * (block / 64);
     const int64_t need = (p.np + per_block - 1) / per_block;
     if (grid > need) grid = need;
     if (grid < 1) grid = 1;
-    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(block), 0, stream, f, sp, p, t_stop);
+    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(block), lds, stream, f, sp, p, t_stop);
     return hipGetLastError();
 }
 

@@ -559,20 +559,21 @@ struct Force {
 // ---------------------------------------------------------------------------
 constexpr int kFeatures = 16;
 
-// From the interpolated fields ub = (U or E, B) at s, the cached corner rows
-// cv (48 values) of its cell and the position xd inside the cell.
-template <int MODE, class V>
-__device__ __forceinline__ void features_core(const StepParams &P, const double *s, const float *ub,
-                                              const V *cv, const float *xd, double h, float err,
-                                              float pi_ratio, bool rej, float *f)
+// |B| at corner j from its three components
+__device__ __forceinline__ float corner_bmag(float b0, float b1, float b2)
+{
+    return __builtin_amdgcn_sqrtf(b0 * b0 + b1 * b1 + b2 * b2);
+}
+
+// From the interpolated fields ub = (U or E, B) at s (only the momentum
+// s[3..5] is read), |B| at the eight corners of its cell (bc, corner order
+// of the gather) and the position xd inside the cell.
+template <int MODE>
+__device__ __forceinline__ void features_bc(const StepParams &P, const double *s, const float *ub,
+                                            const float *bc, const float *xd, double h, float err,
+                                            float pi_ratio, bool rej, float *f)
 {
     const float U[3] = {ub[0], ub[1], ub[2]}, B[3] = {ub[3], ub[4], ub[5]};
-    float bc[8];
-#pragma unroll
-    for (int j = 0; j < 8; j++) {
-        const float b0 = float(cv[6 * j + 3]), b1 = float(cv[6 * j + 4]), b2 = float(cv[6 * j + 5]);
-        bc[j] = __builtin_amdgcn_sqrtf(b0 * b0 + b1 * b1 + b2 * b2);
-    }
     float E[3];
     if constexpr (MODE == PTC_MODE_EB) {
         E[0] = U[0]; E[1] = U[1]; E[2] = U[2];
@@ -626,6 +627,19 @@ __device__ __forceinline__ void features_core(const StepParams &P, const double 
         x = x < -30.0f ? -30.0f : x;
         f[i] = x == x ? x : 0.0f;
     }
+}
+
+// The same from the cached corner rows cv (48 values) of the cell.
+template <int MODE, class V>
+__device__ __forceinline__ void features_core(const StepParams &P, const double *s, const float *ub,
+                                              const V *cv, const float *xd, double h, float err,
+                                              float pi_ratio, bool rej, float *f)
+{
+    float bc[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++)
+        bc[j] = corner_bmag(float(cv[6 * j + 3]), float(cv[6 * j + 4]), float(cv[6 * j + 5]));
+    features_bc<MODE>(P, s, ub, bc, xd, h, err, pi_ratio, rej, f);
 }
 
 // The features at s from the cell cache (loading the cell if needed).

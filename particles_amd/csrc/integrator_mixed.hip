@@ -41,6 +41,19 @@ hipError_t launch_integrate_mixed(int method, const DeviceFields &f, const StepP
 #undef SERIES
         return hipErrorInvalidValue;
     }
+    // learned controller: the group kernel runs the whole launch ("tail"
+    // group) or one lane per particle does (no hand-over)
+    if (lc.ctrl == CTRL_LEARNED && lc.tail == kTailGroupOnly) {
+#define GROUP_L(M, MO, G)                                                      \
+    if (method == M && f.mode == MO && geom == G)                              \
+        return launch_group_t<M, MO, G, PTC_FP32, 0, CTRL_LEARNED>(f, sp, p, t_stop, kBlock, stream);
+        GROUP_L(PTC_RK5, PTC_MODE_VB, 0) GROUP_L(PTC_RK5, PTC_MODE_EB, 0)
+        GROUP_L(PTC_DOP853, PTC_MODE_VB, 0) GROUP_L(PTC_DOP853, PTC_MODE_EB, 0)
+        GROUP_L(PTC_RK5, PTC_MODE_VB, 1) GROUP_L(PTC_RK5, PTC_MODE_EB, 1)
+        GROUP_L(PTC_DOP853, PTC_MODE_VB, 1) GROUP_L(PTC_DOP853, PTC_MODE_EB, 1)
+#undef GROUP_L
+        return hipErrorInvalidValue;
+    }
     // PI controller, static field: the group kernel (six lanes per
     // particle) takes the tail of the launch, or all of it ("tail" option)
     if (lc.tail != 0 && lc.ctrl == CTRL_PI) {

@@ -154,8 +154,9 @@ hipError_t launch_moments(const DeviceParticles &S, int m, const double *shift, 
 // step-size network (mlp.hip; layouts in mlp_device.hpp)
 constexpr int kMlpMaxGradBlocks = 512;
 hipError_t mlp_pack(const float *params, MlpPacked *packed, hipStream_t stream);
+// groups = 1: evaluated as the group kernel evaluates it (mlp_forward_groups)
 hipError_t mlp_forward(const MlpPacked *packed, int64_t n, const float *feat, int ld, float *out,
-                       hipStream_t stream);
+                       hipStream_t stream, int groups = 0);
 int mlp_grad_blocks(int64_t batch);
 hipError_t mlp_gradient(const MlpPacked *packed, const float *rec, int64_t n, int64_t batch,
                         uint64_t seed, uint64_t step, float alpha, float lo, float hi,

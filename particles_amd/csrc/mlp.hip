@@ -89,6 +89,29 @@ mlp_forward_kernel(const MlpPacked *__restrict__ Wg, int64_t n, const float *__r
     }
 }
 
+// the same through mlp_forward_groups: ten samples per wave and pass
+__global__ void __launch_bounds__(256)
+mlp_forward_groups_kernel(const MlpPacked *__restrict__ Wg, int64_t n,
+                          const float *__restrict__ feat, int ld, float *__restrict__ out)
+{
+    __shared__ MlpInfer W;
+    __shared__ MlpWaveBuf buf[4];
+    for (int i = threadIdx.x; i < (int)(sizeof(MlpInfer) / 4); i += blockDim.x)
+        ((uint32_t *)&W)[i] = ((const uint32_t *)&Wg->f)[i];
+    __syncthreads();
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane / 6;
+    const int64_t chunks = (n + 9) / 10;
+    for (int64_t c = (int64_t)blockIdx.x * 4 + wave; c < chunks; c += (int64_t)gridDim.x * 4) {
+        const int64_t j = c * 10 + g;
+        const bool own = lane < 60 && j < n;
+        float f[kMlpIn];
+#pragma unroll
+        for (int i = 0; i < kMlpIn; i++) f[i] = own ? feat[j * ld + i] : W.mu[i];
+        const float y = mlp_forward_groups(W, buf[wave], f);
+        if (own && lane == 6 * g) out[j] = y;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // forward + backward, one wave per block
 // ---------------------------------------------------------------------------
@@ -401,13 +424,18 @@ hipError_t mlp_pack(const float *params, MlpPacked *packed, hipStream_t stream)
 }
 
 hipError_t mlp_forward(const MlpPacked *packed, int64_t n, const float *feat, int ld, float *out,
-                       hipStream_t stream)
+                       hipStream_t stream, int groups)
 {
     if (n <= 0) return hipSuccess;
-    int64_t blocks = (n + 255) / 256;
+    const int64_t per_block = groups ? 40 : 256;
+    int64_t blocks = (n + per_block - 1) / per_block;
     if (blocks > 2048) blocks = 2048;
-    hipLaunchKernelGGL(mlp_forward_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, packed, n,
-                       feat, ld, out);
+    if (groups)
+        hipLaunchKernelGGL(mlp_forward_groups_kernel, dim3((unsigned)blocks), dim3(256), 0, stream,
+                           packed, n, feat, ld, out);
+    else
+        hipLaunchKernelGGL(mlp_forward_kernel, dim3((unsigned)blocks), dim3(256), 0, stream,
+                           packed, n, feat, ld, out);
     return hipGetLastError();
 }
 

@@ -203,4 +203,76 @@ __device__ __forceinline__ float mlp_forward_wave(const MlpInfer &W, MlpWaveBuf 
     return W.b3 + (lane < 32 ? y[0] : y[1]);
 }
 
+// Forward pass for the ten six-lane groups of a wave (lanes 6 g .. 6 g + 5
+// hold the same features, those of sample g; lanes 60..63 belong to no
+// group): the ten samples fill one 32-sample column tile, so the layers take
+// half the matrix products of mlp_forward_wave and a quarter of its
+// accumulators, with the same operations per sample (identical outputs).
+// Every lane of group g receives sample g's output.  Every lane of the wave
+// must call it, with the wave converged.  Buffer rows 32..63 hold the
+// inputs of samples 0..31 (samples 10..31 zero), rows 0..31 the layer-1
+// activations.
+__device__ __forceinline__ float mlp_forward_groups(const MlpInfer &W, MlpWaveBuf &B,
+                                                    const float *feat)
+{
+    const int lane = (int)__lane_id();
+    const int g = lane / 6;
+    if (lane < 60 && lane == 6 * g) {
+        bf16x8 x0, x1;
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            x0[i] = __bf16((feat[i] - W.mu[i]) * W.rsig[i]);
+            x1[i] = __bf16((feat[8 + i] - W.mu[8 + i]) * W.rsig[8 + i]);
+        }
+        *(bf16x8 *)&B.h[32 + g][0] = x0;
+        *(bf16x8 *)&B.h[32 + g][8] = x1;
+    }
+    if (lane >= 42) {
+        bf16x8 z;
+#pragma unroll
+        for (int i = 0; i < 8; i++) z[i] = __bf16(0.0f);
+        *(bf16x8 *)&B.h[lane][0] = z;
+        *(bf16x8 *)&B.h[lane][8] = z;
+    }
+    lds_sync_wave();
+
+    const int r = lane & 31, kh = (lane >> 5) * 8, hi4 = 4 * (lane >> 5);
+    const bf16x8 xb = *(const bf16x8 *)&B.h[32 + r][kh];
+#pragma unroll
+    for (int m = 0; m < 2; m++) {
+        f32x16 acc;
+#pragma unroll
+        for (int v = 0; v < 16; v++) acc[v] = W.b1[32 * m + 8 * (v >> 2) + hi4 + (v & 3)];
+        const bf16x8 a = *(const bf16x8 *)(&W.w1t[0][0] + (32 * m + r) * kLdIn + kh);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, xb, acc, 0, 0, 0);
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+            store_bf16x4(&B.h[r][32 * m + 8 * q + hi4], gelu(acc[4 * q + 0]),
+                         gelu(acc[4 * q + 1]), gelu(acc[4 * q + 2]), gelu(acc[4 * q + 3]));
+    }
+    lds_sync_wave();
+
+    float y = 0.0f;
+#pragma unroll
+    for (int m = 0; m < 2; m++) {
+        f32x16 acc;
+#pragma unroll
+        for (int v = 0; v < 16; v++) acc[v] = W.b2[32 * m + 8 * (v >> 2) + hi4 + (v & 3)];
+#pragma unroll
+        for (int k0 = 0; k0 < kMlpHidden; k0 += 16) {
+            const bf16x8 a = *(const bf16x8 *)(&W.w2t[0][0] + (32 * m + r) * kLdH + k0 + kh);
+            const bf16x8 b = *(const bf16x8 *)&B.h[r][k0 + kh];
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+        }
+#pragma unroll
+        for (int v = 0; v < 16; v++) {
+            const int o = 32 * m + 8 * (v >> 2) + hi4 + (v & 3);
+            y = __builtin_fmaf(float(__bf16(gelu(acc[v]))), W.w3[o], y);
+        }
+    }
+    lds_sync_wave();  // the buffer may be reused right after
+    y += __shfl_xor(y, 32);
+    return __shfl(W.b3 + y, g);
+}
+
 }  // namespace ptc

@@ -177,14 +177,25 @@ def test_learned_mode_needs_a_trained_network_and_the_cell_cache():
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("kernel", ["group", "0"])
 @pytest.mark.parametrize("method", ["dop853", "rk5"])
-def test_trained_network_gains_over_pi(method):
+def test_trained_network_gains_over_pi(method, kernel):
     """Record with the PI controller on one particle set, train the network
     on the GPU, integrate a different set: the learned controller rejects
     far fewer steps and needs fewer RHS evaluations, at an error against a
     tight-tolerance FP64 run no larger than the PI controller's (mixed
-    precision, tol 1e-8, turbulent 32^3 field, gamma in [1, 1e3])."""
+    precision, tol 1e-8, turbulent 32^3 field, gamma in [1, 1e3]), in the
+    six-lanes-per-particle kernel ("group") and in one lane per particle
+    ("0"), the PI runs in the same kernel."""
     import particles_amd as P
+    P.set_option("tail", kernel)
+    try:
+        _gains_over_pi(P, method)
+    finally:
+        P.set_option("tail", "auto")
+
+
+def _gains_over_pi(P, method):
     uu, bb = W.turbulent_vb(32, seed=1, nmodes=128)
     p = W.params_vector(tmax=0.5 * W.T0, dtmax=1.0, atol=1e-8, rtol=1e-8)
     tg = W.time_grid(1, p[2])

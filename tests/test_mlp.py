@@ -119,6 +119,23 @@ def test_forward_matches_oracle():
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("n", [1, 10, 4096 + 17])
+def test_group_forward_matches_wave_forward(n):
+    """The forward pass of the six-lanes-per-particle integrator (ten
+    samples per wavefront in one 32-sample tile) gives the per-wavefront
+    forward's outputs exactly, for whole and ragged last waves."""
+    import particles_amd as P
+    p = M.init_params(7)
+    f = _records(n, 8)[:, :16]
+    with P.Mlp(p.astype(np.float32)) as net:
+        y = net.forward(f)
+        yg = net.forward(f, groups=True)
+    want = M.forward(p.astype(np.float32).astype(np.float64), f.astype(np.float32), emulate=True)
+    assert np.array_equal(yg, y)
+    assert np.max(np.abs(yg - want)) < 1e-2
+
+
+@pytest.mark.gpu
 def test_gradient_matches_oracle():
     import particles_amd as P
     p = M.init_params(11).astype(np.float32)
