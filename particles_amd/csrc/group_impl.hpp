@@ -497,6 +497,16 @@ __device__ __forceinline__ void group_features(const GF &force, const StepParams
     features_bc<MODE>(P, s, ub, bc, force.xdl, h, err, pi_ratio, rej, f);
 }
 
+// Waves per SIMD the kernel is compiled for.  FP32 static fields with the
+// PI controller: three (168 registers, 0-20 B of spills), 1.04-1.24x the
+// accepted steps per second of two (1024^3 DOP853 +15 %, 512^3 RK5 +22 %);
+// the other forms need 180-230 registers and run two (forced to three they
+// spill 60-250 B per lane).
+constexpr int group_waves(int prec, int time, int ctrl)
+{
+    return prec == PTC_FP32 && time == 0 && ctrl == CTRL_PI ? 3 : 1;
+}
+
 // The same particle loop as integrate_kernel (static field or snapshot
 // series in node rows), one particle per group of six lanes, particles
 // taken from the second work-queue head S.qcount[1].  Particles whose
@@ -507,7 +517,7 @@ __device__ __forceinline__ void group_features(const GF &force, const StepParams
 // for its ten particles (mlp_forward_groups; the network and a 64x72 BF16
 // buffer per wave in dynamic LDS).
 template <int METHOD, int MODE, int GEOM, int PREC, int TIME, int CTRL = CTRL_PI>
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kBlock, group_waves(PREC, TIME, CTRL))
 integrate_group_kernel(const DeviceFields F, const StepParams P, const DeviceParticles S,
                        double t_stop)
 {
