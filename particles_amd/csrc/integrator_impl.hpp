@@ -1070,6 +1070,8 @@ __device__ __forceinline__ bool record_sampled(int64_t idx, int64_t c, uint64_t 
 // last 16 in LDS it spills 36 B and runs 1.09x faster on the 256^3
 // turbulent workload (8 to 20 measured alike, 32 or more far slower: the
 // sums then wait on too many LDS reads).  RK5 spills 52 B and gains nothing.
+// The FP32 kernel does not spill (its overflow goes to AGPRs); moving 16 or
+// 32 of its values to LDS made it 4 % and 40 % slower.
 constexpr int static_split(int method, int ctrl, int cache, int prec, int time)
 {
     return method == PTC_DOP853 && ctrl == CTRL_PI && cache == 1 && prec == PTC_FP64 && time == 0
