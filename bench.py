@@ -114,7 +114,8 @@ def build_workload(args):
            "grid": dm, "particles": npart, "method": method, "tol": tol,
            "snapshots_every_T0": tmax / W.T0 / nsnap, "layout": args.layout,
            "cell_cache": args.cell_cache, "field_precision": args.prec, "sort": args.sort,
-           "order": args.order, "field_snapshots": args.snapshots, "tail": args.tail}
+           "order": args.order, "field_snapshots": args.snapshots, "tail": args.tail,
+           "f32_layout": args.f32_layout}
     if args.snapshots > 0:
         cfg["snapshot_every_T0"] = args.snapshot_every
     return make_fields, uu, bb, p, tg, st0, method, cfg
@@ -162,8 +163,8 @@ def traffic_from_profile(cfg):
     configuration."""
     import glob
     keys = ("grid", "particles", "method", "tol", "layout", "cell_cache", "field_precision",
-            "sort", "order", "field_snapshots", "snapshot_every_T0", "tail")
-    defaults = {"field_snapshots": 0, "tail": "0"}
+            "sort", "order", "field_snapshots", "snapshot_every_T0", "tail", "f32_layout")
+    defaults = {"field_snapshots": 0, "tail": "0", "f32_layout": "nodes"}
     for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.json"))):
         try:
             with open(path) as fh:
@@ -181,6 +182,8 @@ def _norm(cfg, key, defaults):
     kernels for node rows ("aos"), elsewhere it is moot."""
     if key == "tail" and cfg.get("layout") != "aos":
         return None
+    if key == "f32_layout" and cfg.get("field_precision") != "fp32":
+        return None  # FP64 fields ignore it
     return cfg.get(key, defaults.get(key))
 
 
@@ -309,6 +312,10 @@ def main():
     ap.add_argument("--grid", type=int, default=0)
     ap.add_argument("--method", default="", choices=["", "rk5", "dop853"])
     ap.add_argument("--layout", default="aos", choices=["aos", "soa", "brick"])
+    ap.add_argument("--f32-layout", default="nodes", choices=["nodes", "pairs"],
+                    help="FP32 fields (--prec fp32 and the mixed / learned-h variants): "
+                         "32-byte node rows or 64-byte rows of a node and its +x neighbour "
+                         "(read by the group kernel only)")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--cell-cache", type=int, default=1, choices=[0, 1])
     ap.add_argument("--tail", default="auto",
@@ -392,6 +399,7 @@ def main():
     t_setup = time.perf_counter()
     make_fields, uu, bb, p, tg, st0, method, cfg = build_workload(args)
     P.set_option("layout", args.layout)
+    P.set_option("f32_layout", args.f32_layout)
     P.set_option("cell_cache", args.cell_cache)
     P.set_option("grid_blocks", args.grid_blocks)
     P.set_option("sort", args.sort)

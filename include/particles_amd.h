@@ -296,7 +296,14 @@ int ptc_particles_use_mlp(ptc_particles *p, const ptc_mlp *net, double shift);
  *                 "soa" (the reference's six planes), "aos" (six
  *                 components interleaved per node) or "brick" (aos plus
  *                 each cell's eight corners in one 384-byte block; 9x the
- *                 memory of "soa")
+ *                 memory of "soa"); FP64 fields only
+ *   "f32_layout"  FP32 fields: "nodes" (default; 32-byte node rows) or
+ *                 "pairs" (64-byte rows holding a node and its +x
+ *                 neighbour, so a gather reads four aligned 64-byte rows;
+ *                 twice the memory).  Pair rows are read by the group
+ *                 kernel only: every "tail" setting then runs it, and
+ *                 probes, record mode and snapshot series refuse them.
+ *                 Results are identical to "nodes" with the group kernel.
  *   "max_steps"   attempted-step cap per particle (0 = none)
  *   "cell_cache"  1 (default) keeps, per particle, the corner values of the
  *                 last cell it sampled (aos layout) in registers, so stage

@@ -90,6 +90,7 @@ struct Options {
     int sort = 1;   // re-order the work queue every `sort` advances (0 = never)
     int order = 2;  // the order then: 1 spatial (Morton), 2 cost (most work first)
     int tail = kTailAuto;  // LaunchConfig::tail, or chosen per batch
+    int f32_layout = LAYOUT_AOS;  // FP32 fields: node rows or LAYOUT_PAIRS
 };
 
 std::mutex g_opt_mu;
@@ -132,6 +133,10 @@ int ptc_set_option(const char *key, const char *value)
         else if (!strcmp(value, "aos")) g_opt.layout = LAYOUT_AOS;
         else if (!strcmp(value, "brick")) g_opt.layout = LAYOUT_BRICK;
         else return fail(PTC_EARG, "layout must be soa, aos or brick (got %s)", value);
+    } else if (!strcmp(key, "f32_layout")) {
+        if (!strcmp(value, "nodes")) g_opt.f32_layout = LAYOUT_AOS;
+        else if (!strcmp(value, "pairs")) g_opt.f32_layout = LAYOUT_PAIRS;
+        else return fail(PTC_EARG, "f32_layout must be nodes or pairs (got %s)", value);
     } else if (!strcmp(key, "max_steps")) {
         long long v = strtoll(value, &end, 10);
         if (*end || v < 0) return fail(PTC_EARG, "bad max_steps %s", value);
@@ -175,6 +180,8 @@ int ptc_get_option(const char *key, char *buf, int len)
     else if (!strcmp(key, "grid_blocks")) snprintf(buf, len, "%d", o.grid_blocks);
     else if (!strcmp(key, "layout"))
         snprintf(buf, len, "%s", o.layout == LAYOUT_SOA ? "soa" : o.layout == LAYOUT_AOS ? "aos" : "brick");
+    else if (!strcmp(key, "f32_layout"))
+        snprintf(buf, len, "%s", o.f32_layout == LAYOUT_PAIRS ? "pairs" : "nodes");
     else if (!strcmp(key, "max_steps")) snprintf(buf, len, "%lld", (long long)o.max_steps);
     else if (!strcmp(key, "cell_cache")) snprintf(buf, len, "%d", o.cell_cache);
     else if (!strcmp(key, "sort")) snprintf(buf, len, "%d", o.sort);
@@ -198,9 +205,21 @@ static int fields_from_device_soa(const int dm[3], double *soa, int mode, int pr
 {
     const int64_t nodes = (int64_t)dm[0] * dm[1] * dm[2];
     Options o = options();
-    if (prec == PTC_FP32) o.layout = LAYOUT_AOS;  // FP32 fields are always node rows
+    if (prec == PTC_FP32) o.layout = o.f32_layout;  // FP32: node rows or pair rows
     void *data = soa, *bricks = nullptr;
     hipError_t e = hipSuccess;
+    if (o.layout == LAYOUT_PAIRS) {
+        void *rows = nullptr;
+        e = hipMalloc(&rows, sizeof(float) * 16 * nodes);
+        if (e == hipSuccess) e = launch_to_pairs_f32(soa, dm, (float *)rows, nullptr);
+        if (e == hipSuccess) e = hipDeviceSynchronize();
+        (void)hipFree(soa);
+        if (e != hipSuccess) {
+            if (rows) (void)hipFree(rows);
+            return hip_fail(e, "ptc_fields: pair rows");
+        }
+        data = rows;
+    }
     if (o.layout == LAYOUT_AOS || o.layout == LAYOUT_BRICK) {
         void *aos = nullptr;
         e = hipMalloc(&aos, prec == PTC_FP32 ? sizeof(float) * 8 * nodes : sizeof(double) * 6 * nodes);
@@ -443,6 +462,7 @@ static int resolve_tail(int tail, const ptc_particles *p)
     // only, and never hands over; training records come from one lane per
     // particle
     const bool learned = p->ctrl == CTRL_LEARNED;
+    if (f.layout == LAYOUT_PAIRS) return kTailGroupOnly;  // read by the group kernel only
     if (learned && f.prec != PTC_FP32) return 0;
     if (tail != kTailAuto) return learned && tail > 0 ? 0 : tail;
     if (f.layout != LAYOUT_AOS || (p->ctrl != CTRL_PI && !learned)) return 0;
@@ -609,7 +629,9 @@ int ptc_particles_advance(ptc_particles *p, double t_stop)
     }
     p->sp.max_steps = o.max_steps;
     LaunchConfig lc{o.block, o.grid_blocks,
-                    p->fields->d.layout == LAYOUT_AOS ? o.cell_cache : 0,
+                    p->fields->d.layout == LAYOUT_AOS || p->fields->d.layout == LAYOUT_PAIRS
+                        ? o.cell_cache
+                        : 0,
                     p->ctrl, resolve_tail(o.tail, p)};
     if (p->ctrl != CTRL_PI && lc.cell_cache != 1)
         return fail(PTC_ESTATE, "record and learned step-size modes need the aos layout with "
@@ -754,7 +776,9 @@ int ptc_particles_advance_graph(ptc_particles *p, int n, const double *t_stop)
     }
     p->sp.max_steps = o.max_steps;
     LaunchConfig lc{o.block, o.grid_blocks,
-                    p->fields->d.layout == LAYOUT_AOS ? o.cell_cache : 0,
+                    p->fields->d.layout == LAYOUT_AOS || p->fields->d.layout == LAYOUT_PAIRS
+                        ? o.cell_cache
+                        : 0,
                     p->ctrl, resolve_tail(o.tail, p)};
     if (p->ctrl != CTRL_PI && lc.cell_cache != 1)
         return fail(PTC_ESTATE, "record and learned step-size modes need the aos layout with "
@@ -926,6 +950,8 @@ int ptc_probe(const ptc_fields *f, int op, int64_t n, const double *params, int 
     if (rc) return rc;
     if (op >= PTC_PROBE_FEATURES && f->d.layout != LAYOUT_AOS)
         return fail(PTC_EARG, "ptc_probe: op %d needs the aos layout", op);
+    if (f->d.layout == LAYOUT_PAIRS)
+        return fail(PTC_EARG, "ptc_probe: the probes read node rows (f32_layout nodes)");
     const int nin = op == PTC_PROBE_GATHER     ? 3
                     : op == PTC_PROBE_POW      ? 2
                     : op == PTC_PROBE_FEATURES ? 10
@@ -1030,8 +1056,10 @@ int ptc_particles_use_mlp(ptc_particles *p, const ptc_mlp *net, double shift)
         ok();
         return PTC_OK;
     }
-    if (p->fields->d.layout != LAYOUT_AOS || p->fields->d.nsnap > 0)
-        return fail(PTC_ESTATE, "ptc_particles_use_mlp: needs a static field in the aos layout");
+    if ((p->fields->d.layout != LAYOUT_AOS && p->fields->d.layout != LAYOUT_PAIRS) ||
+        p->fields->d.nsnap > 0)
+        return fail(PTC_ESTATE, "ptc_particles_use_mlp: needs a static field in the aos layout "
+                                "(or FP32 pair rows)");
     if (!net->trained)
         return fail(PTC_ESTATE, "ptc_particles_use_mlp: the network still has its random "
                                 "initialisation (train it or set its parameters)");

@@ -176,8 +176,13 @@ __device__ __forceinline__ void group_corners(const StepParams &P, const double 
 // rows of eight floats, FP32 weights, sums and force; the slope is a float.
 // FEAT = 1 also keeps, from the last evaluation, component comp of the
 // interpolated fields and the in-cell fraction (step-size features).
-template <int MODE, int GEOM, int PREC, int TIME = 0, int FEAT = 0>
+// PAIRS = 1 (static FP32 field in LAYOUT_PAIRS): the four x-left corner rows
+// also hold the x-right corners, read from them when the cell's right x
+// index is the left one's periodic successor (else from their own rows).
+template <int MODE, int GEOM, int PREC, int TIME = 0, int FEAT = 0, int PAIRS = 0>
 struct GroupForce {
+    static_assert(PAIRS == 0 || (PREC == PTC_FP32 && TIME == 0),
+                  "pair rows hold a static FP32 field");
     using K = std::conditional_t<PREC == PTC_FP32, float, double>;
     const DeviceFields &F;
     const StepParams &P;
@@ -246,6 +251,24 @@ struct GroupForce {
         }
     }
 
+    // the same from pair rows (PAIRS = 1)
+    __device__ __forceinline__ void load_pairs(const uint32_t *o, K *dst) const
+    {
+        const float *__restrict__ d = (const float *)F.data + comp;
+        // x1 - x0 (mod 2^32) from the x-right and x-left offsets
+        const uint32_t dx = o[4] - o[0];
+        const bool succ = dx == 1u || dx == (uint32_t)(1 - P.dm[0]);
+#pragma unroll
+        for (int j = 0; j < 4; j++) dst[j] = d[(size_t)o[j] * 16];
+        if (succ) {
+#pragma unroll
+            for (int j = 0; j < 4; j++) dst[4 + j] = d[(size_t)o[j] * 16 + 6];
+        } else {
+#pragma unroll
+            for (int j = 0; j < 4; j++) dst[4 + j] = d[(size_t)o[4 + j] * 16];
+        }
+    }
+
     // component comp of f(s), s given by its component comp on each lane
     __device__ __forceinline__ K operator()(double s)
     {
@@ -281,7 +304,8 @@ struct GroupForce {
             }
         } else {
             if (c.o[0] != key0 || c.o[7] != key7) {
-                load(c.o, 0, v);
+                if constexpr (PAIRS == 1) load_pairs(c.o, v);
+                else load(c.o, 0, v);
                 key0 = c.o[0];
                 key7 = c.o[7];
             }
@@ -516,7 +540,7 @@ constexpr int group_waves(int prec, int time, int ctrl)
 // proposes the next step, as in integrate_kernel, evaluated once per wave
 // for its ten particles (mlp_forward_groups; the network and a 64x72 BF16
 // buffer per wave in dynamic LDS).
-template <int METHOD, int MODE, int GEOM, int PREC, int TIME, int CTRL = CTRL_PI>
+template <int METHOD, int MODE, int GEOM, int PREC, int TIME, int CTRL = CTRL_PI, int PAIRS = 0>
 __global__ void __launch_bounds__(kBlock, group_waves(PREC, TIME, CTRL))
 integrate_group_kernel(const DeviceFields F, const StepParams P, const DeviceParticles S,
                        double t_stop)
@@ -541,7 +565,8 @@ integrate_group_kernel(const DeviceFields F, const StepParams P, const DevicePar
     const int comp = cl;
     const double t_end = t_stop < P.tmx ? t_stop : P.tmx;
     const int64_t N = S.np;
-    GroupForce<MODE, GEOM, PREC, TIME, CTRL == CTRL_LEARNED ? 1 : 0> force(F, P, base, comp);
+    GroupForce<MODE, GEOM, PREC, TIME, CTRL == CTRL_LEARNED ? 1 : 0, PAIRS> force(F, P, base,
+                                                                                comp);
     using K = typename decltype(force)::K;
     // this lane's state plane, chosen once (indexing the kernel argument's
     // pointer array by a per-lane value would copy it to scratch)
@@ -802,11 +827,12 @@ integrate_group_kernel(const DeviceFields F, const StepParams P, const DevicePar
     }
 }
 
-template <int METHOD, int MODE, int GEOM, int PREC, int TIME = 0, int CTRL = CTRL_PI>
+template <int METHOD, int MODE, int GEOM, int PREC, int TIME = 0, int CTRL = CTRL_PI,
+          int PAIRS = 0>
 hipError_t launch_group_t(const DeviceFields &f, const StepParams &sp, const DeviceParticles &p,
                           double t_stop, int block, hipStream_t stream)
 {
-    auto kern = integrate_group_kernel<METHOD, MODE, GEOM, PREC, TIME, CTRL>;
+    auto kern = integrate_group_kernel<METHOD, MODE, GEOM, PREC, TIME, CTRL, PAIRS>;
     const size_t lds =
         CTRL == CTRL_LEARNED ? kNetLds + (size_t)(block / 64) * sizeof(MlpWaveBuf) : 0;
     if (CTRL == CTRL_LEARNED && !p.net) return hipErrorInvalidValue;

@@ -131,6 +131,28 @@ __global__ void to_nodes_f32_kernel(const double *__restrict__ soa, int64_t node
     }
 }
 
+// SOA planes (FP64) -> FP32 pair rows: node n's six values, then those of
+// its +x neighbour (periodic in x), then four zeros
+__global__ void to_pairs_f32_kernel(const double *__restrict__ soa, int dm0, int64_t nodes,
+                                    float *__restrict__ out)
+{
+    for (int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; n < nodes;
+         n += (int64_t)gridDim.x * blockDim.x) {
+        const int i = (int)(n % dm0);
+        const int64_t nx = n - i + (i + 1 == dm0 ? 0 : i + 1);
+        float4 r[4];
+        float *v = (float *)r;
+#pragma unroll
+        for (int c = 0; c < 6; c++) {
+            v[c] = float(soa[c * nodes + n]);
+            v[6 + c] = float(soa[c * nodes + nx]);
+        }
+        v[12] = v[13] = v[14] = v[15] = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 4; k++) ((float4 *)out)[4 * n + k] = r[k];
+    }
+}
+
 }  // namespace
 
 static hipError_t launch_integrate_lanes(int method, const DeviceFields &f, const StepParams &sp,
@@ -257,6 +279,7 @@ hipError_t launch_probe(const DeviceFields &f, const StepParams &sp, int op, int
     if (n <= 0) return hipSuccess;
     if (op < 0 || op > 7) return hipErrorInvalidValue;
     if (op >= 5 && f.layout != LAYOUT_AOS) return hipErrorInvalidValue;
+    if (f.layout == LAYOUT_PAIRS) return hipErrorInvalidValue;  // the probes read node rows
     if (f.prec == PTC_FP32) return launch_probe_mixed(f, sp, op, n, in, out, stream);
     if (f.mode == PTC_MODE_VB && f.layout == LAYOUT_SOA)
         return launch_probe_t<PTC_MODE_VB, LAYOUT_SOA>(f, sp, op, n, in, out, stream);
@@ -315,6 +338,16 @@ hipError_t launch_zero_i64(int64_t *ptr, int64_t n, hipStream_t stream)
     if (blocks > 4096) blocks = 4096;
     if (blocks < 1) blocks = 1;
     hipLaunchKernelGGL(zero_i64_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, ptr, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_to_pairs_f32(const double *soa, const int dm[3], float *out, hipStream_t stream)
+{
+    const int64_t nodes = (int64_t)dm[0] * dm[1] * dm[2];
+    int64_t blocks = (nodes + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(to_pairs_f32_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, soa,
+                       dm[0], nodes, out);
     return hipGetLastError();
 }
 

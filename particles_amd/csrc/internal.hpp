@@ -9,13 +9,16 @@ namespace ptc {
 
 struct MlpPacked;
 
-enum Layout : int { LAYOUT_SOA = 0, LAYOUT_AOS = 1, LAYOUT_BRICK = 2 };
+enum Layout : int { LAYOUT_SOA = 0, LAYOUT_AOS = 1, LAYOUT_BRICK = 2, LAYOUT_PAIRS = 3 };
 
 // Device copy of one field snapshot.
 //   SOA: six planes of dm1*dm2*dm3 values (U/E x,y,z then B x,y,z), each x
 //        fastest -- the reference's uu(:,:,:,c) / bb(:,:,:,c) planes.
 //   AOS: one array of nodes, six values per node (U/E x,y,z, B x,y,z), so
 //        the eight corners of a cell are eight 48-byte rows.
+//   PAIRS (FP32 only): per node a 64-byte row of 16 floats, the node's six
+//        values, then those of its +x neighbour (periodic), then 4 zeros;
+//        a cell's eight corners are four aligned rows (the group kernel).
 //   BRICK: AOS nodes plus, per cell, the eight corner rows of that cell in
 //        the reference's corner order (48 values, 384 bytes, three 128-byte
 //        lines) -- eight copies of every node, so a gather reads one
@@ -121,6 +124,7 @@ hipError_t launch_zero_i64(int64_t *ptr, int64_t n, hipStream_t stream);
 hipError_t launch_clear_flag(int32_t *flags, int64_t n, int32_t bit, hipStream_t stream);
 // SOA planes -> AOS nodes (and FP64 -> FP32 when prec is PTC_FP32).
 hipError_t launch_to_aos(const double *soa, int64_t nodes, void *aos, int prec, hipStream_t stream);
+hipError_t launch_to_pairs_f32(const double *soa, const int dm[3], float *out, hipStream_t stream);
 // AOS nodes -> per-cell corner bricks.
 hipError_t launch_to_bricks(const DeviceFields &f, hipStream_t stream);
 

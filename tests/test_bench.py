@@ -18,7 +18,7 @@ def _cfg(argv):
     import argparse
     ap = argparse.ArgumentParser()
     for a, d in (("--workload", "turbulent256"), ("--layout", "aos"), ("--order", "cost"), ("--tail", "auto"),
-                 ("--prec", "fp64"), ("--method", "")):
+                 ("--prec", "fp64"), ("--method", ""), ("--f32-layout", "nodes")):
         ap.add_argument(a, default=d)
     for a, d in (("--particles", 0), ("--grid", 0), ("--seed", 1), ("--cell-cache", 1),
                  ("--sort", 1), ("--snapshots", 0), ("--steps", 10), ("--warmup", 2)):

@@ -600,3 +600,92 @@ def test_group_kernel_failure_handling():
     finally:
         P.set_option("max_steps", 0)
         P.set_option("tail", "auto")
+
+
+def _fields_fp32(uu, bb, layout, **kw):
+    P.set_option("f32_layout", layout)
+    try:
+        return P.Fields(uu, bb, prec="fp32", **kw)
+    finally:
+        P.set_option("f32_layout", "nodes")
+
+
+@pytest.mark.parametrize("method", ["rk5", "dop853"])
+def test_pair_rows_match_node_rows(method):
+    """FP32 pair rows (a node and its +x neighbour in one 64-byte row) hold
+    the same values as node rows, and the group kernel reads the x-right
+    corners from the pair row only where the right index is the left one's
+    periodic successor: snapshots and counters identical to node rows with
+    the group kernel, in the unit box (power-of-two geometry), on a 12 x 10
+    x 8 grid in a shifted box, in EB mode, with particles started exactly on
+    cell faces (where the separately rounded right index can differ) and
+    across the periodic x boundary; every "tail" setting runs the group
+    kernel on pair rows."""
+    uu, bb = W.turbulent_vb(32, seed=51, nmodes=64)
+    p = W.params_vector(tmax=1 * W.T0, dtmax=1.0, atol=1e-8, rtol=1e-8)
+    tg = W.time_grid(3, p[2])
+    st0 = W.initial_state(1200, len(tg), seed=52, gamma_range=(1.0, 1000.0))
+    rng = np.random.default_rng(53)
+    st0[0, 0, :200] = rng.integers(0, 32, 200) / 32.0  # on x faces
+    st0[0, 0, 200:260] = 1.0 - 1e-12  # last cell in x
+    cases = [("unit box", uu, bb, {}, p, st0)]
+    ug, bg = W.turbulent_vb(32, seed=54, nmodes=64)
+    ug, bg = np.asfortranarray(ug[:12, :10, :8]), np.asfortranarray(bg[:12, :10, :8])
+    g = W.params_vector(vun=0.7, tmax=1 * W.T0, dtmax=1.0, atol=1e-8, rtol=1e-8,
+                        xl=(-1.0, 0.5, 2.0), xs=(2.5, 3.0, 1.75))
+    sg = W.initial_state(800, len(tg), seed=55, gamma_range=(1.0, 100.0),
+                         xl=(-1.0, 0.5, 2.0), xs=(2.5, 3.0, 1.75))
+    sg[0, 0, :200] = -1.0 + 2.5 * rng.integers(0, 12, 200) / 12.0
+    cases.append(("12x10x8 shifted box", ug, bg, {}, g, sg))
+    ee, be = W.reconnection_eb(32, seed=56, nmodes=64)
+    cases.append(("EB", ee, be, {"mode": "eb"}, p, st0))
+    try:
+        for name, a, b, kw, q, s0 in cases:
+            say(f"[pairs] {method} {name}")
+            P.set_option("tail", "group")
+            with _fields_fp32(a, b, "nodes", **kw) as f:
+                ref = P.integrate(f, method, q, tg, s0)
+            for tail in ("group", "0", "auto"):
+                P.set_option("tail", tail)
+                with _fields_fp32(a, b, "pairs", **kw) as f:
+                    got = P.integrate(f, method, q, tg, s0)
+                assert (got[2] == 0).all()
+                assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1]), (name, tail)
+    finally:
+        P.set_option("tail", "auto")
+
+
+def test_pair_rows_refused_where_node_rows_are_read():
+    """Probes, record mode and snapshot series read node rows: they refuse
+    pair rows with an error; the learned controller runs on them (group
+    kernel) with the results it gives on node rows."""
+    from oracle import mlp_oracle as M
+    uu, bb = W.turbulent_vb(16, seed=57, nmodes=32)
+    p = W.params_vector(tmax=0.5 * W.T0, dtmax=1.0, atol=1e-8, rtol=1e-8)
+    tg = W.time_grid(1, p[2])
+    st0 = W.initial_state(300, len(tg), seed=58, gamma_range=(1.0, 100.0))
+    zero = np.zeros(P.MLP_PARAMS, dtype=np.float32)
+    zero[M.SIG:] = 1.0  # output 0: keep h, within the PI floor
+    with _fields_fp32(uu, bb, "pairs") as f, P.Mlp(zero) as net:
+        with pytest.raises(P.PtcError):
+            P.probe(f, "gather", np.full((4, 3), 0.3), p)
+        with P.Particles(f, "dop853", p, tg, st0) as b:
+            with pytest.raises(P.PtcError):
+                b.record(1.0, 1000)
+        with pytest.raises(P.PtcError):
+            P.Fields.series([f, f], [0.0, 1.0])
+        P.set_option("tail", "group")
+        try:
+            with P.Particles(f, "dop853", p, tg, st0) as b:
+                b.use_mlp(net)
+                b.advance()
+                got = b.download()
+            with _fields_fp32(uu, bb, "nodes") as g:
+                with P.Particles(g, "dop853", p, tg, st0) as b:
+                    b.use_mlp(net)
+                    b.advance()
+                    ref = b.download()
+        finally:
+            P.set_option("tail", "auto")
+    assert (got[2] == 0).all()
+    assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1])
