@@ -57,9 +57,17 @@ def test_options_roundtrip_and_validation():
             with pytest.raises(P.PtcError):
                 P.set_option("tail", bad)
         assert P.get_option("tail") == "auto"
+        assert P.get_option("f32_layout") == "nodes"
+        P.set_option("f32_layout", "pairs")
+        assert P.get_option("f32_layout") == "pairs"
+        for bad in ("pair", "aos", ""):
+            with pytest.raises(P.PtcError):
+                P.set_option("f32_layout", bad)
+        assert P.get_option("f32_layout") == "pairs"
     finally:
         P.set_option("layout", old)
         P.set_option("tail", "auto")
+        P.set_option("f32_layout", "nodes")
 
 
 def test_fields_argument_validation_before_device():
