@@ -14,9 +14,9 @@
 // A group issues about a quarter of the instructions per attempt on each of
 // its lanes (per-particle work such as the cell lookup and the divisions by
 // gamma is split over the group, not repeated), so those last trajectories
-// finish sooner, and with under 200 registers per lane two waves share a
-// SIMD, which hides the latency of gathers from fields too large for the
-// caches.  The lane-per-particle kernel hands its particles over once the
+// finish sooner, and with under 256 registers per lane two waves share a
+// SIMD (three for FP32 static fields, group_waves), which hides the latency
+// of gathers from fields too large for the caches.  The lane-per-particle kernel hands its particles over once the
 // queue is exhausted and few are left in flight (DeviceParticles::park);
 // this kernel then finishes them.  It can also run a whole launch.
 //
