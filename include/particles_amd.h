@@ -324,7 +324,7 @@ int ptc_particles_use_mlp(ptc_particles *p, const ptc_mlp *net, double shift);
  *                 a wave carry particles, each lane all six components);
  *                 "group": the group kernel, six lanes per particle (one
  *                 component per lane; fewer instructions per step and two
- *                 waves per SIMD, so single trajectories and gathers from
+ *                 or three waves per SIMD, so single trajectories and gathers from
  *                 large fields finish sooner); N > 0: one lane per particle
  *                 until the queue is empty and at most N particles are in
  *                 flight, then the group kernel finishes them; "auto"
