@@ -1,37 +1,51 @@
 """Benchmark: accepted particle-steps/s of the device integrator on one MI355X.
 
-Workload (default): the DOP853 turbulent-box configuration -- 256^3 periodic
-grid of side 1, B = z + dB and V built from 256 random solenoidal Fourier
-modes (|k| in [2 pi, 16 pi], amplitude ~ |k|^(-5/3), rms 0.5 and 0.1), force
-q/m (v - V) x B, 100,000 protons (q/m = 1, c = 1) with gamma log-uniform in
-[1, 1e3] and isotropic directions, atol = rtol = 1e-8, snapshots every
-100 gyro-periods of B0 up to 1000.  One benchmark step advances every
-particle by one gyro-period of B0 (T0 = 2 pi); the integration continues
-across steps exactly as one long run would.  Inputs are synthetic (seeded)
-and resident in HBM before the timed region.
+Default workload (the 1024^3 mixed-precision / learned step-size
+configuration): DOP853 at atol = rtol = 1e-8 on a 1024^3 periodic EB
+reconnection field built on the device (B_x = a double current sheet of
+width 0.02, guide field B_z = 0.1, 256 random solenoidal Fourier modes at
+rms 0.05; E = -V x B + 1e-3 curl B by central differences, V_y = -0.1
+sheet(y)), 4,000,000 protons (q/m = 1, c = 1) with gamma log-uniform in
+[1, 1e4], isotropic directions, uniform positions.  One benchmark step
+advances every particle by one gyro-period of B0 (T0 = 2 pi); the
+integration continues across steps exactly as one long run would.  Inputs
+are synthetic (seeded) and resident in HBM before the timed region.
 
-After the timed plain run, the same workload runs (untimed for `value`) in
-mixed precision (FP32 fields, weights and RHS; FP64 state, sums and error
-norm) and with the learned step-size proposal in mixed precision (network
-trained on the GPU from records of a PI-controlled run of a separate particle
-set); the `variants` object reports their rates, rejected-step fractions and
-RHS counts, their physical-time throughput relative to the plain run, and
-their deviation from the plain FP64 result.
+`value` is the plain FP64 integrator (bit-identical to the oracle, which
+restates the reference's algorithm).  After it, untimed for `value`, the
+same particles run in mixed precision (FP32 fields, weights and slopes;
+FP64 position, time, stage sums and error norm) and with the learned
+step-size proposal (mixed precision; network trained on the GPU from
+records of a separate particle set run by the PI controller, its log-step
+offset calibrated on that set).  `variants` reports their accepted steps
+per second, their physical-time throughput (particle gyro-periods per
+second) against plain FP64 and mixed PI, rejected-step fractions, RHS
+counts, and their error against an FP64 run at atol = rtol = 1e-12 next to
+plain FP64's.  `learned_h` is the calibration's choice: the network where it
+ran faster than PI at no larger error, else the PI controller (the reason is
+recorded); `learned_h_network` is the network's own run either way.
 
-Prints one JSON line.  `roofline` is the integrator kernel's algorithmic
-gather traffic (8 corners x 6 components x 8 B = 384 B per RHS evaluation,
-times the RHS evaluations the kernel counted) over its duration measured with
-HIP events on the batch's stream, against the MI355X HBM peak.
-With `--snapshots N` the field is time-dependent: N device-built snapshots
-of the same turbulent recipe, its modes travelling in time, spread over the
-benchmarked time and blended linearly in time at every stage; each RHS then
-reads two snapshots (768 B per evaluation in FP64), the second kept in LDS
-by the cell cache.  The reference has no
-time-dependent field, so no CPU baseline or variants are run for it.
+`roofline` (HBM-bound gather): `achieved` = cell loads x field bytes per
+load (384 B in FP64, 192 B in FP32: eight corners x six components) over
+the integrator's kernel time (HIP events on the batch's stream).  A cell
+load is a fetch of a cell's eight corner rows that the per-particle cell
+cache could not serve; the kernels count them on the device.  Next to it:
+the same bytes on the RHS basis (every evaluation a full gather, which the
+cell cache undercuts), the HBM traffic measured with rocprofv3 for this
+configuration (profiles/*.json), and an FP64 operation estimate.
 
-`cpu_baseline` times the reference's own FP64 OpenMP integrator (compiled
-from its sources into oracle/_ref) -- or the C oracle where that library is
-absent -- on the host cores, on the first particles of the same workload.
+`cpu_baseline`: the reference's own FP64 OpenMP integrator (compiled from
+its sources into oracle/_ref) on the host cores, first particles of the
+workload, horizon scaled to a bounded run.  The reference has no E-field
+form, so on the EB workloads it runs in its VB form with the same B and
+plasma velocity V (E = -V x B at the particle, the 1e-3 curl B term
+dropped), and `cpu_baseline_oracle_eb` adds the C oracle on the exact EB
+workload (the fields downloaded from the device).
+
+Other workloads: --workload turbulent256 (DOP853, 256^3 turbulent VB field,
+100,000 protons, gamma in [1, 1e3]), reconnection512 (Cash-Karp, 512^3,
+1,000,000 protons), turbulent64 (the reference's CPU-runnable case);
+--snapshots N turns the turbulent field into N travelling snapshots.
 """
 import argparse
 import json
@@ -46,12 +60,18 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-BYTES_PER_RHS_FP64 = 8 * 6 * 8
+FP64_PEAK_TFLOPS = 78.6
+BYTES_PER_LOAD = {"fp64": 8 * 6 * 8, "fp32": 8 * 6 * 4}
+# FP64 operations per RHS evaluation (cell lookup ~30, weights 16, corner
+# sums 96, Lorentz force ~30) and per attempted step beyond the RHS (stage
+# sums, solution, error estimate and control): Cash-Karp 6 stages, DOP853 12
+FLOP_PER_RHS = 172
+FLOP_PER_ATTEMPT = {"rk5": 160, "dop853": 620}
 
 
 def build_workload(args):
-    """(make_fields(prec) -> Fields, host fields (uu, bb) or None, params, time
-    grid, initial state, method, config)."""
+    """(make_fields(prec) -> Fields, host fields (f1, bb) or None, params,
+    time grid, initial state, method, config)."""
     from particles_amd import workloads as W
     gr = None
     device = False
@@ -66,7 +86,7 @@ def build_workload(args):
         tmax, nsnap, recipe, device = 10 * W.T0, 10, "reconnection", True
     elif args.workload == "reconnection1024":
         dm, npart, method, tol, gr = 1024, 4_000_000, "dop853", 1e-8, (1.0, 1.0e4)
-        tmax, nsnap, recipe, device = 10 * W.T0, 10, "reconnection", True
+        tmax, nsnap, recipe, device = 100 * W.T0, 10, "reconnection", True
     else:
         raise SystemExit(f"unknown workload {args.workload}")
     if args.particles:
@@ -115,45 +135,81 @@ def build_workload(args):
            "snapshots_every_T0": tmax / W.T0 / nsnap, "layout": args.layout,
            "cell_cache": args.cell_cache, "field_precision": args.prec, "sort": args.sort,
            "order": args.order, "field_snapshots": args.snapshots, "tail": args.tail,
-           "f32_layout": args.f32_layout}
+           "f32_layout": args.f32_layout, "recipe": recipe}
     if args.snapshots > 0:
         cfg["snapshot_every_T0"] = args.snapshot_every
     return make_fields, uu, bb, p, tg, st0, method, cfg
 
 
-def cpu_baseline(uu, bb, p, tg, st0, method, target_s):
-    """Reference FP64 OpenMP integrator on the host, first particles of the
-    workload, horizon scaled so the timed run takes about target_s."""
+def _timed_cpu(run, target_s, start=0.05):
+    """Runs run(horizon) with growing horizons (in T0) until one takes at
+    least a third of target_s; returns (horizon, counters, seconds)."""
     from particles_amd import workloads as W
-    try:
-        from oracle.oracle import REF_SO, Oracle, Reference
-    except ImportError:
-        return None
-    if os.path.exists(REF_SO):
-        impl, kind = Reference(), "reference"
-        threads = impl.num_threads()
-        run = lambda q, s: impl.integrate(method, uu, bb, q, np.array([0.0, 1e300]), s)
-    else:
-        impl, kind = Oracle(), "port"
-        threads = impl.max_threads()
-        run = lambda q, s: impl.integrate(method, uu, bb, q, np.array([0.0, 1e300]), s)[:2]
-    n = min(st0.shape[2], 64 * threads)
-    s0 = np.asfortranarray(st0[:, :1, :n].repeat(2, axis=1))
-    s0[:, 1, :] = 0.0
-    horizon = 0.05 * W.T0
-    for _ in range(6):
-        q = p.copy(); q[2] = horizon
+    horizon = start * W.T0
+    for _ in range(8):
         t0 = time.perf_counter()
-        _, cnt = run(q, s0)
+        cnt = run(horizon)
         el = time.perf_counter() - t0
         if el > 0.3 * target_s:
             break
         horizon *= min(8.0, max(1.5, target_s / max(el, 1e-3)))
-    acc = int(cnt[1].sum())
-    return {"value": acc / el, "unit": "accepted particle-steps/s", "cores": int(threads),
-            "kind": kind,
-            "sample": f"first {n} particles of the workload, t = 0 .. {horizon / W.T0:.3g} T0 "
-                      f"({acc} accepted steps in {el:.2f} s, {threads} OpenMP threads)"}
+    return horizon, cnt, el
+
+
+def cpu_baselines(f1, bb, v_plasma, p, st0, method, target_s, eb):
+    """The reference's FP64 OpenMP integrator (VB form) and, for EB fields,
+    the oracle on the exact workload, on the first particles.  f1 is U (VB
+    workloads) or E (EB workloads); v_plasma the plasma velocity the
+    reference's VB form takes on EB workloads."""
+    from particles_amd import workloads as W
+    try:
+        from oracle.oracle import REF_SO, Oracle, Reference
+    except ImportError:
+        return None, None
+    grid = np.array([0.0, 1e300])
+    ref_line = orc_line = None
+    vb_u = v_plasma if eb else f1
+    if os.path.exists(REF_SO):
+        ref = Reference()
+        threads = ref.num_threads()
+        n = min(st0.shape[2], 64 * threads)
+        s0 = np.asfortranarray(st0[:, :1, :n].repeat(2, axis=1))
+        s0[:, 1, :] = 0.0
+
+        def run(h):
+            q = p.copy(); q[2] = h
+            return ref.integrate(method, vb_u, bb, q, grid, s0)[1]
+        h, cnt, el = _timed_cpu(run, target_s)
+        acc = int(cnt[1].sum())
+        form = ("VB form: the same B and plasma velocity V, E = -V x B at the particle; the "
+                "1e-3 curl B term of the EB field dropped" if eb else "the workload's VB field")
+        ref_line = {"value": acc / el, "unit": "accepted particle-steps/s", "cores": int(threads),
+                    "kind": "reference",
+                    "sample": f"first {n} particles of the workload, t = 0 .. {h / W.T0:.3g} T0, "
+                              f"{form} ({acc} accepted steps in {el:.2f} s, {threads} OpenMP "
+                              "threads)"}
+        print(f"cpu baseline (reference): {ref_line['value']:.3e} accepted/s", file=sys.stderr,
+              flush=True)
+    if eb or ref_line is None:
+        orc = Oracle()
+        threads = orc.max_threads()
+        n = min(st0.shape[2], 64 * threads)
+        s0 = np.asfortranarray(st0[:, :1, :n].repeat(2, axis=1))
+        s0[:, 1, :] = 0.0
+
+        def run(h):
+            q = p.copy(); q[2] = h
+            return orc.integrate(method, f1, bb, q, grid, s0, mode="eb" if eb else "vb")[1]
+        h, cnt, el = _timed_cpu(run, target_s)
+        acc = int(cnt[1].sum())
+        orc_line = {"value": acc / el, "unit": "accepted particle-steps/s", "cores": int(threads),
+                    "kind": "port",
+                    "sample": f"first {n} particles of the workload, t = 0 .. {h / W.T0:.3g} T0, "
+                              f"{'exact EB workload (fields downloaded from the device)' if eb else 'VB field'}"
+                              f" ({acc} accepted steps in {el:.2f} s, {threads} OpenMP threads)"}
+        print(f"cpu baseline (oracle): {orc_line['value']:.3e} accepted/s", file=sys.stderr,
+              flush=True)
+    return ref_line, orc_line
 
 
 def traffic_from_profile(cfg):
@@ -187,67 +243,120 @@ def _norm(cfg, key, defaults):
     return cfg.get(key, defaults.get(key))
 
 
-def run_variant(fields, method, p, tg, st, steps, warmup, net=None, shift=0.0):
+def roofline(tot, kernel_ms, launches, prec, method, snapshots=0, traffic=None):
+    """The gather roofline of `launches` integrator launches from their
+    summed counters and kernel time (see the module docstring); `traffic`
+    is the measured HBM bytes of one launch, if a profile has it."""
+    sec = kernel_ms * 1e-3
+    bpl = BYTES_PER_LOAD[prec]
+    loads = tot["cell_loads"]
+    achieved = bpl * loads / sec / 1e9
+    bpr = bpl * (2 if snapshots > 1 else 1)
+    flop = FLOP_PER_RHS * tot["rhs"] + FLOP_PER_ATTEMPT[method] * tot["all"]
+    r = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+         "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+         "basis": f"cell loads x {bpl} B (eight corner rows of six {prec.upper()} values)",
+         "bytes_per_load": bpl, "cell_loads_per_launch": loads / launches,
+         "rhs_per_launch": tot["rhs"] / launches, "launch_ms": kernel_ms / launches,
+         "cell_loads_per_rhs": loads / max(tot["rhs"], 1),
+         "rhs_basis_GBps": bpr * tot["rhs"] / sec / 1e9,
+         "fp64_flop_estimate_TFLOPs": flop / sec / 1e12 if prec == "fp64" else None,
+         "fp64_peak_TFLOPs": FP64_PEAK_TFLOPS}
+    if traffic:
+        r["traffic_GBps"] = traffic / (sec / launches) / 1e9
+        r["traffic_per_algorithmic_byte"] = traffic / (bpl * loads / launches)
+    return r
+
+
+def _delta(a, b):
+    return {k: b[k] - a[k] for k in a}
+
+
+def run_batch(fields, method, p, tg, st, steps, warmup, net=None, shift=0.0, label=""):
     """Advances a fresh batch warmup + steps gyro-periods; returns the timed
-    part's counters and wall time, and the snapshots."""
+    part's counters, wall and kernel time, and the final snapshots."""
     from particles_amd import workloads as W
     import particles_amd as P
     with P.Particles(fields, method, p, tg, st) as b:
         if net is not None:
             b.use_mlp(net, shift)
         t = 0.0
-        for _ in range(warmup):
+        for i in range(warmup):
             t += W.T0
-            b.advance(t)
+            ms = b.advance(t)
+            print(f"{label} warmup {i + 1}/{warmup}: kernel {ms:.1f} ms", file=sys.stderr,
+                  flush=True)
         tot0 = b.totals()
         kms = 0.0
         t0 = time.perf_counter()
-        for _ in range(steps):
+        for i in range(steps):
             t += W.T0
-            kms += b.advance(t)
+            ms = b.advance(t)
+            kms += ms
+            print(f"{label} step {i + 1}/{steps}: kernel {ms:.1f} ms", file=sys.stderr,
+                  flush=True)
         wall = time.perf_counter() - t0
         tot1 = b.totals()
-        state = b.download()[0]
-    acc = tot1["accepted"] - tot0["accepted"]
-    rej = tot1["rejected"] - tot0["rejected"]
-    rhs = tot1["rhs"] - tot0["rhs"]
-    return dict(acc=acc, rej=rej, rhs=rhs, wall=wall, kernel_ms=kms, t_end=t), state
+    return dict(tot=_delta(tot0, tot1), wall=wall, kernel_ms=kms)
+
+
+def summary(r, npart, steps, prec, method, plain_tps=None, mixed_tps=None):
+    tot = r["tot"]
+    tps = npart * steps / r["wall"]
+    out = {"value": tot["accepted"] / r["wall"], "unit": "accepted particle-steps/s",
+           "particle_T0_per_s": tps, "accepted": tot["accepted"], "rejected": tot["rejected"],
+           "rejected_per_accepted": tot["rejected"] / max(tot["accepted"], 1),
+           "rhs_per_accepted": tot["rhs"] / max(tot["accepted"], 1),
+           "kernel_ms": r["kernel_ms"], "wall_s": r["wall"],
+           "roofline": roofline(tot, r["kernel_ms"], steps, prec, method)}
+    if plain_tps:
+        out["time_throughput_vs_plain"] = tps / plain_tps
+    if mixed_tps:
+        out["time_throughput_vs_mixed_pi"] = tps / mixed_tps
+    return out
 
 
 def variants(args, fields64, make_fields, p, tg, st0, method, plain):
-    """Mixed precision and learned-h (mixed) on the same workload, then their
-    accuracy next to the plain run's on the first particles over one T0."""
+    """Mixed precision and learned-h (mixed) on the same workload, then the
+    accuracy of each against a tight FP64 run on the first particles."""
     import particles_amd as P
     from particles_amd import workloads as W
+    from particles_amd.learned import train_controller
 
+    npart = st0.shape[2]
     out = {}
-    time_per_s_plain = st0.shape[2] * args.steps / plain["wall"]
+    plain_tps = npart * args.steps / plain["wall"]
+    P.set_option("f32_layout", args.f32_layout)
     f32 = make_fields("fp32")
-    r, _ = run_variant(f32, method, p, tg, st0, args.steps, args.warmup)
-    out["mixed"] = summary(r, st0.shape[2], args.steps, time_per_s_plain)
+    P.set_option("f32_layout", "nodes")
+    r = run_batch(f32, method, p, tg, st0, args.steps, args.warmup, label="mixed")
+    out["mixed"] = summary(r, npart, args.steps, "fp32", method, plain_tps)
+    out["mixed"]["f32_layout"] = args.f32_layout
+    mixed_tps = out["mixed"]["particle_T0_per_s"]
     print(f"mixed: {out['mixed']['value']:.3e} accepted/s", file=sys.stderr, flush=True)
+
     # network trained on records of a separate particle set under the PI
     # controller, its log-step offset calibrated on that set against PI
-    from particles_amd.learned import train_controller
     st_train = W.initial_state(args.train_particles, 2, seed=args.seed + 101,
                                gamma_range=tuple(args.train_gamma))
-    ctl = train_controller(f32, method, p, tg, st_train, fields_fp64=fields64, horizon=W.T0,
-                           steps=args.train_steps, seed=args.seed)
+    ctl = train_controller(f32, method, p, tg, st_train, fields_fp64=fields64,
+                           horizon=args.train_horizon * W.T0, steps=args.train_steps,
+                           seed=args.seed, fallback=False)
     if args.shift is not None:
         ctl.shift = args.shift
-    net = ctl.net
-    r, _ = run_variant(f32, method, p, tg, st0, args.steps, args.warmup, net, ctl.shift)
-    out["learned_h"] = summary(r, st0.shape[2], args.steps, time_per_s_plain)
-    out["learned_h"]["controller"] = dict(
+    r = run_batch(f32, method, p, tg, st0, args.steps, args.warmup, ctl.net, ctl.shift,
+                  label="learned")
+    net_line = summary(r, npart, args.steps, "fp32", method, plain_tps, mixed_tps)
+    net_line["controller"] = dict(
         network="16-64-64-1 GELU, BF16 MFMA, FP32 accumulation", shift=ctl.shift,
-        record_run=f"{args.train_particles} particles (separate seed), 1 T0, PI controller",
-        **ctl.info)
-    print(f"learned-h: {out['learned_h']['value']:.3e} accepted/s (shift {ctl.shift})",
+        record_run=f"{args.train_particles} particles (separate seed), "
+                   f"{args.train_horizon:g} T0, PI controller", **ctl.info)
+    print(f"learned-h network: {net_line['value']:.3e} accepted/s (shift {ctl.shift})",
           file=sys.stderr, flush=True)
 
     # accuracy: one T0 from the initial state, snapshot at the end, against
     # the FP64 integrator at atol = rtol = 1e-12
-    n = min(args.accuracy_particles, st0.shape[2])
+    n = min(args.accuracy_particles, npart)
     q = p.copy()
     q[2] = W.T0
     tga = np.array([0.0, W.T0])
@@ -263,13 +372,11 @@ def variants(args, fields64, make_fields, p, tg, st0, method, plain):
     s_plain = P.integrate(fields64, method, q, tga, sa)[0]
     s_mixed = P.integrate(f32, method, q, tga, sa)[0]
     with P.Particles(f32, method, q, tga, sa) as b:
-        b.use_mlp(net, ctl.shift)
+        b.use_mlp(ctl.net, ctl.shift)
         b.advance()
         s_learned = b.download()[0]
 
     def dev_plain(state):
-        # the same-seed FP64 run at the same tolerance: position/momentum and
-        # relative energy (gamma) deviation
         d = np.abs(state[:6, 1, :] - s_plain[:6, 1, :]).max(0)
         dg = np.abs(state[6, 1, :] - s_plain[6, 1, :]) / s_plain[6, 1, :]
         return {"median": float(np.median(d)), "p90": float(np.percentile(d, 90)),
@@ -277,29 +384,23 @@ def variants(args, fields64, make_fields, p, tg, st0, method, plain):
 
     out["plain_error_vs_fp64_tol1e-12"] = err(s_plain)
     out["mixed"]["error_vs_fp64_tol1e-12"] = err(s_mixed)
-    out["learned_h"]["error_vs_fp64_tol1e-12"] = err(s_learned)
     out["mixed"]["deviation_vs_plain_fp64"] = dev_plain(s_mixed)
-    out["learned_h"]["deviation_vs_plain_fp64"] = dev_plain(s_learned)
+    net_line["error_vs_fp64_tol1e-12"] = err(s_learned)
+    net_line["deviation_vs_plain_fp64"] = dev_plain(s_learned)
+    out["learned_h_network"] = net_line
+    if ctl.beats_pi:
+        out["learned_h"] = dict(net_line, selected="network (ran faster than PI at no larger "
+                                                    "error on the calibration set)")
+    else:
+        why = ctl.info.get("not_beating_pi", "")
+        out["learned_h"] = dict(out["mixed"], selected="PI controller (calibration fallback: "
+                                                       f"{why})",
+                                time_throughput_vs_mixed_pi=1.0)
     out["accuracy_sample"] = (f"first {n} particles, t = 0 .. 1 T0, max over x, u of "
                               "|state - state(FP64, atol = rtol = 1e-12)|")
-    net.close()
+    ctl.close()
     f32.close()
     return out
-
-
-def summary(r, npart, steps, time_per_s_plain):
-    """One variant (FP32 node rows: 192 algorithmic bytes per RHS; the
-    learned controller's features come from the cell already gathered)."""
-    tps = npart * steps / r["wall"]
-    bpr = BYTES_PER_RHS_FP64 // 2
-    achieved = bpr * r["rhs"] / (r["kernel_ms"] * 1e-3) / 1e9
-    return {"value": r["acc"] / r["wall"], "unit": "accepted particle-steps/s",
-            "particle_T0_per_s": tps, "time_throughput_vs_plain": tps / time_per_s_plain,
-            "accepted": r["acc"], "rejected": r["rej"],
-            "rejected_per_accepted": r["rej"] / max(r["acc"], 1),
-            "rhs_per_accepted": r["rhs"] / max(r["acc"], 1), "kernel_ms": r["kernel_ms"],
-            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                         "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "bytes_per_rhs": bpr}}
 
 
 def main():
@@ -307,7 +408,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="turbulent256")
+    ap.add_argument("--workload", default="reconnection1024")
     ap.add_argument("--particles", type=int, default=0)
     ap.add_argument("--grid", type=int, default=0)
     ap.add_argument("--method", default="", choices=["", "rk5", "dop853"])
@@ -330,13 +431,16 @@ def main():
                     help="cost: most work in the previous step first; spatial: Morton order")
     ap.add_argument("--prec", default="fp64", choices=["fp64", "fp32"],
                     help="fp32 = mixed precision (FP32 fields, weights and RHS; FP64 state)")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-variants", action="store_true",
                     help="skip the mixed-precision and learned-h runs")
-    ap.add_argument("--train-particles", type=int, default=20000)
+    ap.add_argument("--train-particles", type=int, default=100_000)
+    ap.add_argument("--train-horizon", type=float, default=0.5,
+                    help="record and calibration horizon in T0")
     ap.add_argument("--accuracy-particles", type=int, default=20000)
-    ap.add_argument("--train-gamma", type=float, nargs=2, default=(1.0, 1.0e3))
+    ap.add_argument("--train-gamma", type=float, nargs=2, default=None,
+                    help="gamma range of the training particles (default: the workload's)")
     ap.add_argument("--train-steps", type=int, default=3000)
     ap.add_argument("--snapshots", type=int, default=0,
                     help="time-dependent field: this many snapshots from t = 0 (0 = static)")
@@ -398,6 +502,8 @@ def main():
 
     t_setup = time.perf_counter()
     make_fields, uu, bb, p, tg, st0, method, cfg = build_workload(args)
+    if args.train_gamma is None:
+        args.train_gamma = (1.0, 1.0e4) if cfg["recipe"] == "reconnection" else (1.0, 1.0e3)
     P.set_option("layout", args.layout)
     P.set_option("f32_layout", args.f32_layout)
     P.set_option("cell_cache", args.cell_cache)
@@ -406,51 +512,59 @@ def main():
     P.set_option("order", args.order)
     P.set_option("tail", args.tail)
     fields = make_fields(args.prec)
-    batch = P.Particles(fields, method, p, tg, st0)
+    P.set_option("f32_layout", "nodes")
+    npart = st0.shape[2]
     print(f"setup {time.perf_counter() - t_setup:.1f}s: {cfg['workload']}", file=sys.stderr,
           flush=True)
 
-    dt_step = W.T0
-    t = 0.0
     tuned = None
-    for i in range(args.warmup):
-        t += dt_step
-        if args.autotune and i == args.warmup - 1:
-            tuned = batch.autotune(t)
-            print(f"warmup {i + 1}/{args.warmup}: autotuned {tuned}", file=sys.stderr, flush=True)
-            continue
-        ms = batch.advance(t)
-        print(f"warmup {i + 1}/{args.warmup}: kernel {ms:.1f} ms", file=sys.stderr, flush=True)
-    tot0 = batch.totals()
-    kernel_ms = 0.0
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        t += dt_step
-        ms = batch.advance(t)
-        kernel_ms += ms
-        print(f"step {i + 1}/{args.steps}: kernel {ms:.1f} ms", file=sys.stderr, flush=True)
-    wall = time.perf_counter() - t0
-    tot1 = batch.totals()
-    acc = tot1["accepted"] - tot0["accepted"]
-    rej = tot1["rejected"] - tot0["rejected"]
-    rhs = tot1["rhs"] - tot0["rhs"]
-    # algorithmic gather bytes per RHS: 8 corners x 6 components x 8 B (FP64)
-    # or x 4 B (FP32 fields)
-    bytes_per_rhs = BYTES_PER_RHS_FP64 if args.prec == "fp64" else BYTES_PER_RHS_FP64 // 2
-    if args.snapshots > 1:
-        bytes_per_rhs *= 2  # the two bracketing snapshots
-    achieved = bytes_per_rhs * rhs / (kernel_ms * 1e-3) / 1e9
-    batch.close()
+    with P.Particles(fields, method, p, tg, st0) as batch:
+        t = 0.0
+        for i in range(args.warmup):
+            t += W.T0
+            if args.autotune and i == args.warmup - 1:
+                tuned = batch.autotune(t)
+                print(f"warmup {i + 1}/{args.warmup}: autotuned {tuned}", file=sys.stderr,
+                      flush=True)
+                continue
+            ms = batch.advance(t)
+            print(f"warmup {i + 1}/{args.warmup}: kernel {ms:.1f} ms", file=sys.stderr, flush=True)
+        tot0 = batch.totals()
+        kernel_ms = 0.0
+        t0 = time.perf_counter()
+        for i in range(args.steps):
+            t += W.T0
+            ms = batch.advance(t)
+            kernel_ms += ms
+            print(f"step {i + 1}/{args.steps}: kernel {ms:.1f} ms", file=sys.stderr, flush=True)
+        wall = time.perf_counter() - t0
+        tot = _delta(tot0, batch.totals())
+    acc = tot["accepted"]
+    # one integrator launch per step
+    rl = roofline(tot, kernel_ms, args.steps, args.prec, method, args.snapshots,
+                  traffic_from_profile(cfg))
 
     var = None
     if not args.no_variants and args.prec == "fp64" and args.snapshots == 0:
         print("variants ...", file=sys.stderr, flush=True)
         var = variants(args, fields, make_fields, p, tg, st0, method, dict(acc=acc, wall=wall))
 
-    cpu = None
-    if not args.no_cpu and uu is not None:
+    cpu = cpu_oracle = None
+    if not args.no_cpu and args.snapshots == 0:
         print("cpu baseline ...", file=sys.stderr, flush=True)
-        cpu = cpu_baseline(uu, bb, p, tg, st0, method, args.cpu_seconds)
+        eb = cfg["recipe"] == "reconnection"
+        if uu is None:
+            f1h, bbh = fields.download()
+        else:
+            f1h, bbh = uu, bb
+        vpl = None
+        if eb:
+            # the recipe's plasma velocity (0, -0.1 sheet(y), 0) at the nodes
+            dm = cfg["grid"]
+            vpl = np.zeros((dm, dm, dm, 3), order="F")
+            vpl[:, :, :, 1] = -0.1 * W.sheet_profile(np.arange(dm) / dm, 0.02)[None, :, None]
+        cpu, cpu_oracle = cpu_baselines(f1h, bbh, vpl, p, st0, method, args.cpu_seconds, eb)
+        del f1h, bbh, vpl
 
     out = {
         "metric": "accepted particle-steps/s on 1 MI355X at tol=1e-8, plain vs learned-h",
@@ -466,14 +580,19 @@ def main():
                     "reference's field is static)" if args.snapshots > 0 else
                     "plain FP64 (EB fields: bit-identical to the oracle; the reference has "
                     "no E-field form)"),
-        "accepted": acc, "rejected": rej, "rejected_per_accepted": rej / max(acc, 1),
-        "rhs_per_accepted": rhs / max(acc, 1), "kernel_ms": kernel_ms,
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profile(cfg),
-                     "bytes_per_rhs": bytes_per_rhs, "rhs_per_launch": rhs / args.steps},
+        "accepted": acc, "rejected": tot["rejected"],
+        "rejected_per_accepted": tot["rejected"] / max(acc, 1),
+        "rhs_per_accepted": tot["rhs"] / max(acc, 1), "kernel_ms": kernel_ms,
+        "particle_T0_per_s": npart * args.steps / wall,
+        "roofline": rl,
         "cpu_baseline": cpu,
+        "cpu_baseline_oracle_eb": cpu_oracle,
         "variants": var,
         "autotuned": tuned,
+        "energy_note": ("FP64 is bit-identical to the oracle (the reference's algorithm); with "
+                        "E = 0 its relative gamma drift over 10 T0 at tol 1e-8 has median "
+                        "1.3e-10 and maximum 1.3e-6 (tests/test_ensemble.py), so the 1e-10 "
+                        "energy criterion holds for the median particle, not every one"),
     }
     if args.prec == "fp32":
         out["dtype"] = "f64 state, f32 fields and RHS"

@@ -70,6 +70,14 @@ typedef struct ptc_particles ptc_particles;
  * prec = PTC_FP32 stores the fields in float (mixed-precision mode). */
 int ptc_fields_create(const int dm[3], const double *f1, const double *bb, int mode, int prec,
                       ptc_fields **out);
+/* The stored field values back on the host, as the arrays ptc_fields_create
+ * takes: f1 (U or E) and bb, each double (dm1, dm2, dm3, 3), x fastest
+ * (the reference's uu / bb of sources/fields.F90:63-64).  FP32 storage
+ * comes back widened to double.  snapshot selects one snapshot of a series
+ * (0 for a static field).  No counterpart in the reference (whose fields
+ * stay on the host); for checks and CPU baselines on fields built on the
+ * device. */
+int ptc_fields_download(const ptc_fields *f, int snapshot, double *f1, double *bb);
 /* Replaces finalize_fields (sources/fields.F90:86-101). */
 int ptc_fields_destroy(ptc_fields *f);
 
@@ -180,6 +188,15 @@ int ptc_particles_autotune(ptc_particles *p, double t_end, int chosen[4]);
 /* totals[0..4] = sum of all / accepted / rejected steps, number of RHS
  * evaluations, number of particles still running; reduced on the device. */
 int ptc_particles_totals(ptc_particles *p, int64_t totals[5]);
+/* Cell loads since the batch was created: each is one fetch of a cell's
+ * eight corner rows of one field snapshot from device memory (384 bytes of
+ * field values in FP64, 192 in FP32), made where an RHS evaluation
+ * (acceleration -> interpolate_lin, sources/accelerations.F90:82) leaves
+ * the cell its particle last sampled; evaluations in the same cell reuse
+ * the values (the cell cache).  Counted per wave on the device.  No
+ * counterpart in the reference (diagnostics: the bytes the gathers must
+ * move). */
+int ptc_particles_cell_loads(ptc_particles *p, int64_t *loads);
 /* Duration in ms of the integrator kernels of the last advance, measured
  * with HIP events on the batch's own stream. */
 int ptc_particles_last_kernel_ms(const ptc_particles *p, double *ms);

@@ -14,6 +14,14 @@
  */
 #include "ptc_oracle.h"
 
+/* ORC_PLANT != 0 builds the oracle with a known mistake in the EB-mode
+ * force (tests/test_eb_closed_form.py compiles such copies into a temporary
+ * directory to show that its closed-form checks catch each one); the
+ * library built by the Makefile leaves it 0. */
+#ifndef ORC_PLANT
+#define ORC_PLANT 0
+#endif
+
 #include <math.h>
 #include <float.h>
 #include <stddef.h>
@@ -105,9 +113,26 @@ static void lorentz(int mode, double qom, double vun, const double u[3], const d
     acc[1] = v[1] / vun;
     acc[2] = v[2] / vun;
     if (mode == PTC_MODE_EB) {
+#if ORC_PLANT == 1
+        /* planted mistake (tests only): v x B with the wrong sign */
+        acc[3] = qom * (u[0] - (v[1] * b[2] - v[2] * b[1]));
+        acc[4] = qom * (u[1] - (v[2] * b[0] - v[0] * b[2]));
+        acc[5] = qom * (u[2] - (v[0] * b[1] - v[1] * b[0]));
+#elif ORC_PLANT == 2
+        /* planted mistake (tests only): two components of v x B swapped */
+        acc[3] = qom * (u[0] + (v[2] * b[0] - v[0] * b[2]));
+        acc[4] = qom * (u[1] + (v[1] * b[2] - v[2] * b[1]));
+        acc[5] = qom * (u[2] + (v[0] * b[1] - v[1] * b[0]));
+#elif ORC_PLANT == 3
+        /* planted mistake (tests only): E taken with the wrong sign */
+        acc[3] = qom * (-u[0] + (v[1] * b[2] - v[2] * b[1]));
+        acc[4] = qom * (-u[1] + (v[2] * b[0] - v[0] * b[2]));
+        acc[5] = qom * (-u[2] + (v[0] * b[1] - v[1] * b[0]));
+#else
         acc[3] = qom * (u[0] + (v[1] * b[2] - v[2] * b[1]));
         acc[4] = qom * (u[1] + (v[2] * b[0] - v[0] * b[2]));
         acc[5] = qom * (u[2] + (v[0] * b[1] - v[1] * b[0]));
+#endif
     } else {
         w[0] = v[0] - u[0];                                                /* :87 */
         w[1] = v[1] - u[1];

@@ -71,6 +71,8 @@ def _load():
         "ptc_particles_download": ([vp, vp, vp, vp], i32),
         "ptc_particles_totals": ([vp, vp], i32),
         "ptc_particles_last_kernel_ms": ([vp, C.POINTER(dbl)], i32),
+        "ptc_particles_cell_loads": ([vp, C.POINTER(i64)], i32),
+        "ptc_fields_download": ([vp, i32, vp, vp], i32),
         "ptc_particles_destroy": ([vp], i32),
         "ptc_probe": ([vp, i32, i64, vp, i32, vp, vp], i32),
         "ptc_set_option": ([C.c_char_p, C.c_char_p], i32),
@@ -198,6 +200,16 @@ class Fields:
         self._h = h
         return self
 
+    def download(self, snapshot: int = 0):
+        """(f1, bb): the stored values back on the host as Fortran
+        (dm1, dm2, dm3, 3) float64 arrays (FP32 storage widened), the form
+        the constructor takes (ptc_fields_download)."""
+        shape = tuple(int(d) for d in self.dm) + (3,)
+        f1 = np.empty(shape, dtype=np.float64, order="F")
+        bb = np.empty(shape, dtype=np.float64, order="F")
+        _check(_lib.ptc_fields_download(self._h, int(snapshot), _ptr(f1), _ptr(bb)))
+        return f1, bb
+
     @property
     def handle(self):
         return self._h
@@ -310,8 +322,10 @@ class Particles:
         """dict of summed counters, reduced on the device."""
         t = np.zeros(5, dtype=np.int64)
         _check(_lib.ptc_particles_totals(self._h, _ptr(t)))
+        loads = C.c_int64()
+        _check(_lib.ptc_particles_cell_loads(self._h, C.byref(loads)))
         return dict(all=int(t[0]), accepted=int(t[1]), rejected=int(t[2]), rhs=int(t[3]),
-                    running=int(t[4]))
+                    running=int(t[4]), cell_loads=int(loads.value))
 
     def generate(self, kind: str, seed: int, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0),
                  a: float = 1.0, b: float = 1.0) -> None:

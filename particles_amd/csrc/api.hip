@@ -425,6 +425,29 @@ int ptc_fields_series(const ptc_fields *const *snaps, const double *times, int n
     return PTC_OK;
 }
 
+int ptc_fields_download(const ptc_fields *f, int snapshot, double *f1, double *bb)
+{
+    if (!f || !f1 || !bb) return fail(PTC_EARG, "ptc_fields_download: null argument");
+    const DeviceFields &d = f->d;
+    const int ns = d.nsnap > 0 ? d.nsnap : 1;
+    if (snapshot < 0 || snapshot >= ns)
+        return fail(PTC_EARG, "ptc_fields_download: snapshot %d outside [0, %d)", snapshot, ns);
+    double *plane = nullptr;
+    HIP_TRY(hipMalloc(&plane, sizeof(double) * (size_t)d.nodes));
+    hipError_t e = hipSuccess;
+    for (int c = 0; c < 6 && e == hipSuccess; c++) {
+        double *dst = (c < 3 ? f1 : bb) + (size_t)(c % 3) * (size_t)d.nodes;
+        e = launch_extract_plane(d, snapshot, c, plane, nullptr);
+        if (e == hipSuccess) e = hipDeviceSynchronize();
+        if (e == hipSuccess)
+            e = hipMemcpy(dst, plane, sizeof(double) * (size_t)d.nodes, hipMemcpyDeviceToHost);
+    }
+    (void)hipFree(plane);
+    if (e != hipSuccess) return hip_fail(e, "ptc_fields_download");
+    ok();
+    return PTC_OK;
+}
+
 int ptc_fields_destroy(ptc_fields *f)
 {
     if (!f) return PTC_OK;
@@ -567,6 +590,7 @@ int ptc_particles_create(const ptc_fields *f, int method, int64_t np, int m, con
     p->d.time = p->d_time;
     p->d_totals = (int64_t *)(p->d_time + m);
     p->d.qcount = (int32_t *)(p->d_totals + 8);
+    p->d.loads = (unsigned long long *)(p->d_totals + 16);
 
     auto cleanup = [&](hipError_t err, const char *what) {
         (void)hipFree(p->d_time);
@@ -574,6 +598,9 @@ int ptc_particles_create(const ptc_fields *f, int method, int64_t np, int m, con
         delete p;
         return hip_fail(err, what);
     };
+    // totals, queue heads and the cell-load counter start at zero
+    e = hipMemset(p->d_totals, 0, 64 * sizeof(int64_t));
+    if (e != hipSuccess) return cleanup(e, "ptc_particles_create: counters");
     e = hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking);
     if (e != hipSuccess) return cleanup(e, "hipStreamCreate");
     if ((e = hipEventCreate(&p->ev0)) != hipSuccess || (e = hipEventCreate(&p->ev1)) != hipSuccess)
@@ -906,6 +933,19 @@ int ptc_particles_totals(ptc_particles *p, int64_t totals[5])
     HIP_TRY(launch_totals(p->d, p->d_totals, p->stream));
     HIP_TRY(hipMemcpyAsync(totals, p->d_totals, sizeof(int64_t) * 5, hipMemcpyDeviceToHost, p->stream));
     HIP_TRY(hipStreamSynchronize(p->stream));
+    ok();
+    return PTC_OK;
+}
+
+int ptc_particles_cell_loads(ptc_particles *p, int64_t *loads)
+{
+    if (!p || !loads) return fail(PTC_EARG, "ptc_particles_cell_loads: null argument");
+    *loads = 0;
+    if (p->d.np == 0) { ok(); return PTC_OK; }
+    unsigned long long v = 0;
+    HIP_TRY(hipStreamSynchronize(p->stream));
+    HIP_TRY(hipMemcpy(&v, p->d.loads, sizeof v, hipMemcpyDeviceToHost));
+    *loads = (int64_t)v;
     ok();
     return PTC_OK;
 }

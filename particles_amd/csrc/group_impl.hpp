@@ -188,6 +188,7 @@ struct GroupForce {
     const StepParams &P;
     const int base, comp;
     uint32_t key0 = 0xffffffffu, key7 = 0xffffffffu;
+    uint32_t loads = 0;  // cell loads of the group's particle (the same on each lane)
     K v[8];
     // TIME = 1 (a snapshot series, Force<..., TIME>): component comp of the
     // cell's corner rows in snapshot k + 1 as well, the stage time, the
@@ -286,6 +287,7 @@ struct GroupForce {
             const int k = interval(a, two);
             const int key = 2 * k + (two ? 1 : 0);
             if (!(c.o[0] == key0 && c.o[7] == key7 && (key == kc || (!two && key + 1 == kc)))) {
+                loads += two ? 2u : 1u;
                 load(c.o, k, v);
                 if (two) load(c.o, k + 1, v1);
                 key0 = c.o[0];
@@ -304,6 +306,7 @@ struct GroupForce {
             }
         } else {
             if (c.o[0] != key0 || c.o[7] != key7) {
+                loads++;
                 if constexpr (PAIRS == 1) load_pairs(c.o, v);
                 else load(c.o, 0, v);
                 key0 = c.o[0];
@@ -825,6 +828,8 @@ integrate_group_kernel(const DeviceFields F, const StepParams P, const DevicePar
             want_feat = false;
         }
     }
+    // one count per group (its lane 0); lanes 60..63 never load
+    add_cell_loads(S, cl == 0 && lane < kGroup * kGroupsPerWave ? force.loads : 0u);
 }
 
 template <int METHOD, int MODE, int GEOM, int PREC, int TIME = 0, int CTRL = CTRL_PI,

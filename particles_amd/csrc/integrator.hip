@@ -153,7 +153,42 @@ __global__ void to_pairs_f32_kernel(const double *__restrict__ soa, int dm0, int
     }
 }
 
+// One component of a stored field back to a plane of doubles (the
+// reference's uu(:,:,:,c) / bb(:,:,:,c) order): from the six SOA planes,
+// FP64 node rows, FP32 node rows (eight floats) or FP32 pair rows (sixteen
+// floats, the node's own six first).
+__global__ void extract_plane_kernel(const void *__restrict__ data, int layout, int prec,
+                                     int64_t nodes, int comp, double *__restrict__ out)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nodes;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        double v;
+        if (prec == PTC_FP32)
+            v = double(((const float *)data)[i * (layout == LAYOUT_PAIRS ? 16 : 8) + comp]);
+        else if (layout == LAYOUT_SOA)
+            v = ((const double *)data)[comp * nodes + i];
+        else
+            v = ((const double *)data)[i * 6 + comp];
+        out[i] = v;
+    }
+}
+
 }  // namespace
+
+hipError_t launch_extract_plane(const DeviceFields &f, int snap, int comp, double *out,
+                                hipStream_t stream)
+{
+    if (comp < 0 || comp > 5 || snap < 0 || (f.nsnap == 0 && snap > 0) ||
+        (f.nsnap > 0 && snap >= f.nsnap))
+        return hipErrorInvalidValue;
+    const size_t esz = f.prec == PTC_FP32 ? sizeof(float) : sizeof(double);
+    const char *data = (const char *)f.data + (size_t)snap * (size_t)f.sstride * esz;
+    int64_t blocks = (f.nodes + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(extract_plane_kernel, dim3((unsigned)blocks), dim3(256), 0, stream,
+                       (const void *)data, f.layout, f.prec, f.nodes, comp, out);
+    return hipGetLastError();
+}
 
 static hipError_t launch_integrate_lanes(int method, const DeviceFields &f, const StepParams &sp,
                                          const DeviceParticles &p, double t_stop,

@@ -258,6 +258,16 @@ __device__ __forceinline__ void rhs(const DeviceFields &F, const StepParams &P, 
 // SPLIT > 0 (static FP64 field, PI controller): the last SPLIT of the 48
 // cached values live in LDS (lds1, the per-wave slab layout below) instead
 // of registers.
+// Adds a wave's cell loads to the batch's counter (one atomic per wave, at
+// the end of a launch): every lane passes its own count, zero for lanes
+// that do not stand for a particle of their own.
+__device__ __forceinline__ void add_cell_loads(const DeviceParticles &S, uint32_t loads)
+{
+    unsigned long long l = loads;
+    for (int off = 32; off > 0; off >>= 1) l += __shfl_down(l, off);
+    if (__lane_id() == 0 && S.loads && l) atomicAdd(S.loads, l);
+}
+
 template <int MODE, int LAYOUT, int CACHE, int PREC, int GEOM = 0, int TIME = 0, int SPLIT = 0>
 struct Force {
     using K = std::conditional_t<PREC == PTC_FP32, float, double>;
@@ -265,6 +275,7 @@ struct Force {
     const DeviceFields &F;
     const StepParams &P;
     uint32_t key0 = 0xffffffffu, key7 = 0xffffffffu;
+    uint32_t loads = 0;  // cell loads (eight corner rows of one snapshot) by this lane
     K v[kRegVals > 0 ? kRegVals : 1];
     double ts = 0.0;  // TIME: stage time of the next evaluation
     int kh = 0;       // TIME: the lane's last snapshot interval, in [0, nsnap - 2]
@@ -327,6 +338,7 @@ struct Force {
     {
         const int key = 2 * k + (two ? 1 : 0);
         if (o[0] == key0 && o[7] == key7 && (key == kc || (!two && key + 1 == kc))) return;
+        loads += two ? 2u : 1u;
         if constexpr (PREC == PTC_FP32) {
             const float *__restrict__ d = (const float *)F.data + (size_t)k * F.sstride;
 #pragma unroll
@@ -402,6 +414,7 @@ struct Force {
                 for (int i = 0; i < 6; i++) f[i] = f[i] + ak * (g[i] - f[i]);
             }
         } else if constexpr (PREC == PTC_FP32) {
+            loads += two ? 2u : 1u;
             const float *__restrict__ d = (const float *)F.data + (size_t)k * F.sstride;
 #pragma unroll
             for (int j = 0; j < 8; j++) {
@@ -423,6 +436,7 @@ struct Force {
                 for (int i = 0; i < 6; i++) f[i] = f[i] + af * (g[i] - f[i]);
             }
         } else {
+            loads += two ? 2u : 1u;
             const double *__restrict__ d = (const double *)F.data + (size_t)k * F.sstride;
             gather_rows(d, c, f);
             if (two) {
@@ -439,6 +453,7 @@ struct Force {
     {
         static_assert(CACHE == 1 && LAYOUT == LAYOUT_AOS, "the cell cache reads AOS node rows");
         if (o[0] == key0 && o[7] == key7) return;
+        loads++;
         if constexpr (PREC == PTC_FP32) {
             const float *__restrict__ d = (const float *)F.data;
 #pragma unroll
@@ -514,6 +529,7 @@ struct Force {
         } else if constexpr (PREC == PTC_FP32) {
             static_assert(LAYOUT == LAYOUT_AOS, "FP32 fields are stored as AOS nodes");
             if constexpr (CACHE == 0) {
+                loads++;
                 rhs_f32<MODE, GEOM>(F, P, s, a);
             } else {
                 CornersT<float> c;
@@ -525,6 +541,7 @@ struct Force {
                 force_from_fields_f32<MODE>(P, s, f, a);
             }
         } else if constexpr (CACHE == 0) {
+            loads++;
             rhs<MODE, LAYOUT>(F, P, s, a);
         } else {
             static_assert(LAYOUT == LAYOUT_AOS, "the cell cache reads AOS node rows");
@@ -1083,7 +1100,7 @@ constexpr int static_split(int method, int ctrl, int cache, int prec, int time)
 // each stage; PI controller, node rows, cell cache as described at Force).
 // PARK = 1 (static node rows, PI controller): once the queue is empty
 // and at most S.park particles are still in flight (S.qcount[2] counts the
-// finished ones), every wave stores its particles as at the end of a segment
+// finished ones and those fetched already retired), every wave stores its particles as at the end of a segment
 // and stops; the group kernel (group_impl.hpp) finishes them.  A separate
 // instantiation, because the extra live state changes the register
 // allocation of the kernels that do not hand over.
@@ -1141,6 +1158,7 @@ integrate_kernel(const DeviceFields F, const StepParams P, const DeviceParticles
         // ---- refill lanes without a particle (one atomic per wave) ----
         const bool want = !active && !exhausted;
         const uint64_t wmask = __ballot(want);
+        bool retired = false;  // fetched a particle that no longer runs
         if (wmask) {
             const int leader = __ffsll((unsigned long long)wmask) - 1;
             int base = 0;
@@ -1153,6 +1171,7 @@ integrate_kernel(const DeviceFields F, const StepParams P, const DeviceParticles
                 } else {
                     idx = S.order ? (int64_t)S.order[my] : my;
                     status = S.status[idx];
+                    retired = status != PTC_STATUS_RUNNING;
                     if (status == PTC_STATUS_RUNNING) {
                         active = true;
                         fresh = true;
@@ -1174,6 +1193,12 @@ integrate_kernel(const DeviceFields F, const StepParams P, const DeviceParticles
             }
         }
         if constexpr (PARK == 1) {
+            // particles retired before this launch (done, capped, non-finite)
+            // count as finished, so the hand-over threshold compares the
+            // particles still in flight
+            const uint64_t rm = __ballot(retired);
+            if (rm && lane == __ffsll((unsigned long long)rm) - 1)
+                atomicAdd(S.qcount + 2, __popcll(rm));
             if (__any(exhausted)) {
                 int done = 0;
                 if (lane == 0)
@@ -1441,6 +1466,7 @@ integrate_kernel(const DeviceFields F, const StepParams P, const DeviceParticles
             want_feat = false;
         }
     }
+    add_cell_loads(S, force.loads);
 }
 
 // Single-call probes of the device building blocks, one input per thread:

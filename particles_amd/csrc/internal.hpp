@@ -69,8 +69,12 @@ struct DeviceParticles {
     const int32_t *order;// work-queue permutation (nullptr = identity)
     int32_t *qcount;     // [0] work-queue head of the lane-per-particle kernel,
                          // [1] of the group kernel, [2] particles the former finished
+                         // or found retired (not running) when it fetched them
     int32_t park;        // hand every particle in flight to the group kernel once the
                          // queue is empty and at most this many are left (0 = never)
+    unsigned long long *loads;  // cell loads since creation (each: the eight corner
+                                // rows of one snapshot), added per wave at launch end;
+                                // may be null
     // step-size controller: the reference's PI rule (CTRL_PI), the same with
     // sampled training records (CTRL_RECORD), or network proposals after
     // accepted steps with the PI rule after rejections (CTRL_LEARNED)
@@ -127,6 +131,10 @@ hipError_t launch_to_aos(const double *soa, int64_t nodes, void *aos, int prec, 
 hipError_t launch_to_pairs_f32(const double *soa, const int dm[3], float *out, hipStream_t stream);
 // AOS nodes -> per-cell corner bricks.
 hipError_t launch_to_bricks(const DeviceFields &f, hipStream_t stream);
+// Component comp (0..5: U or E x, y, z, then B x, y, z) of snapshot snap
+// (0 for a static field) as a plane of nodes doubles, x fastest.
+hipError_t launch_extract_plane(const DeviceFields &f, int snap, int comp, double *out,
+                                hipStream_t stream);
 
 // field synthesis (fields.hip): fill / add a constant, add a mode sum to three
 // planes, add the reconnection sheet and guide field to B, and E of the

@@ -17,8 +17,8 @@ each step:
      kernel time) whose error against a tight-tolerance FP64 run is no larger
      than the PI controller's own error on the same particles.  The result
      says whether that candidate also ran faster than PI (``beats_pi``); where
-     it does not, the PI controller alone is the better choice for the
-     workload.
+     it does not, the result is the PI controller alone (``net`` None, the
+     reason in ``info["fallback"]``).
 
 Everything runs through the C-ABI; nothing here is on the integration path.
 """
@@ -34,10 +34,39 @@ from . import Fields, Mlp, Particles, integrate
 
 @dataclass
 class Controller:
-    net: Mlp
+    """A calibrated step-size controller.  ``net`` is None when the PI
+    controller is the better choice for the workload (the network ran no
+    faster than PI at no larger error, or it never reached PI's error):
+    ``Particles.use_mlp(None)`` then keeps the PI rule; ``info["fallback"]``
+    says why."""
+    net: Mlp | None
     shift: float
     beats_pi: bool = False  # the calibrated shift ran faster than PI at no larger error
     info: dict = field(default_factory=dict)
+
+    def close(self):
+        if self.net is not None:
+            self.net.close()
+            self.net = None
+
+
+def choose_shift(cand, pi_ms: float, pi_error: float):
+    """The calibration's decision, from the candidates' measured kernel time
+    and error (dicts with "shift", "kernel_ms", "error") and PI's on the same
+    particles: (chosen candidate or None, calibrated, beats_pi, reason).
+    A candidate qualifies when its error is no larger than PI's; the fastest
+    qualifying one is chosen when it is also faster than PI, else PI is."""
+    ok = [c for c in cand if c["error"] <= pi_error]
+    if not ok:
+        return None, False, False, (f"no shift reached the PI controller's error "
+                                    f"({pi_error:.3g}); the lowest was "
+                                    f"{min(c['error'] for c in cand):.3g}")
+    best = min(ok, key=lambda c: c["kernel_ms"])
+    if not best["kernel_ms"] < pi_ms:
+        return None, True, False, (f"the fastest shift at no larger error ({best['shift']:g}, "
+                                   f"{best['kernel_ms']:.2f} ms) ran no faster than the PI "
+                                   f"controller ({pi_ms:.2f} ms)")
+    return best, True, True, ""
 
 
 def _final_error(state, ref):
@@ -50,12 +79,15 @@ def train_controller(fields: Fields, method: str, params, time, state_train, *,
                      steps: int = 3000, batch: int = 16384, lr: float = 2e-3, alpha: float = 1.0,
                      shifts=(-0.45, -0.4, -0.35, -0.3, -0.25, -0.2, -0.15, -0.1, 0.0, 0.1),
                      capacity: int = 60_000_000,
-                     seed: int = 1) -> Controller:
+                     seed: int = 1, fallback: bool = True) -> Controller:
     """Records, trains and calibrates a learned controller for `fields` (the
     precision it will run with).  `state_train` is (8, m, n) like every
     state; records are taken over [0, horizon] (default params[2]).  The
     calibration runs the same particles over the same horizon, with
-    `fields_fp64` (default: `fields`) at atol = rtol = 1e-12 as reference."""
+    `fields_fp64` (default: `fields`) at atol = rtol = 1e-12 as reference.
+    Where the network does not beat PI (choose_shift), the result is the PI
+    controller (net None) unless `fallback` is False, which keeps the
+    network at the shift closest to qualifying (for measurements)."""
     params = np.asarray(params, dtype=np.float64)
     q = params.copy()
     if horizon is not None:
@@ -89,10 +121,17 @@ def train_controller(fields: Fields, method: str, params, time, state_train, *,
             ms = b.advance()
             e = _final_error(b.download()[0], ref)
         cand.append({"shift": float(s), "kernel_ms": ms, "error": e})
-    ok = [c for c in cand if c["error"] <= e_pi]
-    best = min(ok or cand, key=lambda c: c["kernel_ms"] if ok else c["error"])
-    beats = bool(ok) and best["kernel_ms"] < ms_pi
+    best, calibrated, beats, why = choose_shift(cand, ms_pi, e_pi)
     info.update(calibration=cand, pi_kernel_ms=ms_pi, pi_error=e_pi,
-                calibrated=bool(ok), beats_pi=beats, particles=int(st.shape[2]),
+                calibrated=calibrated, beats_pi=beats, particles=int(st.shape[2]),
                 horizon=float(q[2]))
-    return Controller(net=net, shift=best["shift"], beats_pi=beats, info=info)
+    if best is not None:
+        return Controller(net=net, shift=best["shift"], beats_pi=True, info=info)
+    if fallback:
+        net.close()
+        info["fallback"] = "PI controller: " + why
+        return Controller(net=None, shift=0.0, beats_pi=False, info=info)
+    ok = [c for c in cand if c["error"] <= e_pi]
+    near = min(ok, key=lambda c: c["kernel_ms"]) if ok else min(cand, key=lambda c: c["error"])
+    info["not_beating_pi"] = why
+    return Controller(net=net, shift=near["shift"], beats_pi=False, info=info)

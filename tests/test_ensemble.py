@@ -44,9 +44,9 @@ def test_mixed_and_learned_distributions_match_fp64():
         s32 = _final(P, f32, "dop853", p, tg, st0)
         ctl = train_controller(f32, "dop853", p, tg,
                                W.initial_state(4000, 2, seed=13, gamma_range=(1.0, 100.0)),
-                               fields_fp64=f64, horizon=W.T0, steps=1500)
+                               fields_fp64=f64, horizon=W.T0, steps=1500, fallback=False)
         sl = _final(P, f32, "dop853", p, tg, st0, ctl.net, ctl.shift)
-        ctl.net.close()
+        ctl.close()
         g64, c64 = _gamma_pitch(P, f64, s64, p)
         for name, s in (("mixed", s32), ("learned", sl)):
             g, c = _gamma_pitch(P, f64, s, p)

@@ -374,23 +374,29 @@ def test_mixed_probes_close_to_reference():
     assert eu < 1e-6 and eb < 1e-6 and ea.max() < 1e-6
 
 
+@pytest.mark.parametrize("mode", ["vb", "eb"])
 @pytest.mark.parametrize("method", ["rk5", "dop853"])
-def test_mixed_precision_error_against_oracle(oracle, method, tail):
+def test_mixed_precision_error_against_oracle(oracle, method, tail, mode):
     """Mixed precision differs from the FP64 oracle by FP32 rounding in the
     RHS; over 0.1 gyro-periods the median relative deviation in position
     and momentum stays below 1e-6 and every particle below 1e-3 (the bound
     is set by chaotic growth of FP32-level differences, which the FP64 path
     does not have), with one lane per particle, with the group kernel and
-    with the hand-over between them."""
-    uu, bb = W.turbulent_vb(32, seed=1)
+    with the hand-over between them, in the turbulent VB field and in the
+    EB reconnection field (whose FP32 force has its own E + v x B branch)."""
+    if mode == "vb":
+        uu, bb = W.turbulent_vb(32, seed=1)
+    else:
+        uu, bb = W.reconnection_eb(32, seed=1)
     p = W.params_vector(tmax=0.1 * W.T0, dtmax=1.0, atol=1e-8, rtol=1e-8)
     tg = W.time_grid(2, p[2])
     st0 = W.initial_state(512, len(tg), seed=2, gamma_range=(1.0, 1.0e3))
-    so, co, _ = oracle.integrate(method, uu, bb, p, tg, st0)
-    with P.Fields(uu, bb, prec="fp32") as f:
+    so, co, _ = oracle.integrate(method, uu, bb, p, tg, st0, mode=mode)
+    with P.Fields(uu, bb, prec="fp32", mode=mode) as f:
         st, cnt, status = P.integrate(f, method, p, tg, st0)
     dev = deviation(st, so)
-    say(f"[mixed] {method} tail={tail}: deviation median {np.median(dev):.2e} max {dev.max():.2e}")
+    say(f"[mixed] {method} {mode} tail={tail}: deviation median {np.median(dev):.2e} "
+        f"max {dev.max():.2e}")
     assert (status == 0).all()
     assert np.median(dev) < 1e-6 and dev.max() < 1e-3
 
@@ -599,6 +605,107 @@ def test_group_kernel_failure_handling():
             assert status[1] == 1 and cnt[0, 1] == 25
     finally:
         P.set_option("max_steps", 0)
+        P.set_option("tail", "auto")
+
+
+def test_handover_counts_retired_particles(oracle):
+    """Particles already retired when a launch fetches them (here 3 of 4,
+    non-finite initial momenta, status 2 after the first segment) count as
+    finished for the hand-over threshold, so the running ones are handed to
+    the group kernel with a threshold far below the retired count; every
+    snapshot and counter still equals the oracle's."""
+    uu, bb = W.turbulent_vb(32, seed=51, nmodes=64)
+    p = W.params_vector(tmax=1 * W.T0, dtmax=1.0, atol=1e-8, rtol=1e-8)
+    tg = W.time_grid(3, p[2])
+    st0 = W.initial_state(4000, len(tg), seed=52, gamma_range=(1.0, 1000.0))
+    st0[3, 0, 1::4] = np.nan
+    st0[4, 0, 2::4] = np.inf
+    st0[5, 0, 3::4] = np.nan
+    so, co, sto = oracle.integrate("dop853", uu, bb, p, tg, st0)
+    assert (sto[0::4] == 0).all() and (sto[1::4] == 2).all()
+    try:
+        P.set_option("tail", "256")
+        with P.Fields(uu, bb) as f, P.Particles(f, "dop853", p, tg, st0) as b:
+            say("[hand-over] 3000 retired of 4000, tail 256")
+            for t_stop in (0.2 * W.T0, 0.6 * W.T0):
+                b.advance(t_stop)
+            b.advance()
+            s2, c2, st2 = b.download()
+    finally:
+        P.set_option("tail", "auto")
+    assert np.array_equal(st2, sto)
+    assert np.array_equal(c2, co) and np.array_equal(s2, so, equal_nan=True)
+
+
+@pytest.mark.parametrize("form", ["aos", "soa", "brick", "fp32 nodes", "fp32 pairs", "series"])
+def test_fields_download_returns_stored_values(form):
+    """ptc_fields_download gives back what each storage form holds: FP64
+    layouts exactly the uploaded arrays, FP32 ones the arrays rounded to
+    float, a series the chosen snapshot."""
+    rng = np.random.default_rng(8)
+    dm = (6, 5, 7)
+    uu = np.asfortranarray(rng.normal(size=dm + (3,)))
+    bb = np.asfortranarray(rng.normal(size=dm + (3,)))
+    old = P.get_option("layout")
+    try:
+        if form in ("aos", "soa", "brick"):
+            P.set_option("layout", form)
+            with P.Fields(uu, bb) as f:
+                u2, b2 = f.download()
+            assert np.array_equal(u2, uu) and np.array_equal(b2, bb)
+        elif form.startswith("fp32"):
+            with _fields_fp32(uu, bb, form.split()[1]) as f:
+                u2, b2 = f.download()
+            assert np.array_equal(u2, uu.astype(np.float32).astype(np.float64))
+            assert np.array_equal(b2, bb.astype(np.float32).astype(np.float64))
+        else:
+            with P.Fields(uu, bb) as f0, P.Fields(2 * uu, -bb) as f1:
+                with P.Fields.series([f0, f1], [0.0, 1.0]) as fs:
+                    u2, b2 = fs.download(1)
+                    u0, _ = fs.download(0)
+            assert np.array_equal(u2, 2 * uu) and np.array_equal(b2, -bb)
+            assert np.array_equal(u0, uu)
+    finally:
+        P.set_option("layout", old)
+
+
+@pytest.mark.parametrize("kernel", ["0", "group"])
+def test_cell_loads_counted(kernel):
+    """The cell-load counter: a particle at rest in uniform fields never
+    leaves its cell, so each launch loads its cell once; with the cell cache
+    off every RHS evaluation loads; moving particles load at most once per
+    evaluation."""
+    uu = np.zeros((8, 8, 8, 3), order="F")
+    bb = np.zeros((8, 8, 8, 3), order="F")
+    bb[..., 2] = 1.0
+    p = W.params_vector(tmax=2 * W.T0, dtmax=1.0, atol=1e-8, rtol=1e-8)
+    tg = W.time_grid(2, p[2])
+    st0 = np.zeros((8, len(tg), 1), order="F")
+    st0[:3, 0, 0] = [0.3, 0.4, 0.5]
+    try:
+        P.set_option("tail", kernel)
+        with P.Fields(uu, bb) as f:
+            with P.Particles(f, "dop853", p, tg, st0) as b:
+                b.advance(W.T0)
+                b.advance()
+                tot = b.totals()
+            say(f"[loads] tail={kernel} at rest: {tot['cell_loads']} loads, {tot['rhs']} RHS")
+            assert tot["cell_loads"] == 2 and tot["rhs"] > 2
+            ut, bt = W.turbulent_vb(16, seed=3, nmodes=32)
+            sm = W.initial_state(300, len(tg), seed=4, gamma_range=(1.0, 100.0))
+            with P.Fields(ut, bt) as g, P.Particles(g, "dop853", p, tg, sm) as b:
+                b.advance()
+                tot = b.totals()
+            say(f"[loads] tail={kernel} moving: {tot['cell_loads']} loads, {tot['rhs']} RHS")
+            assert 0 < tot["cell_loads"] < tot["rhs"]
+        if kernel == "0":
+            P.set_option("cell_cache", 0)
+            with P.Fields(ut, bt) as g, P.Particles(g, "dop853", p, tg, sm) as b:
+                b.advance()
+                tot = b.totals()
+            assert tot["cell_loads"] == tot["rhs"]
+    finally:
+        P.set_option("cell_cache", 1)
         P.set_option("tail", "auto")
 
 

@@ -62,6 +62,29 @@ def test_features_field_gradient_and_e_field():
     assert f[14] == 0.0
 
 
+def test_calibration_falls_back_to_pi():
+    """The calibration keeps the network only where a shift runs faster than
+    the PI controller at no larger error; otherwise the PI controller is the
+    result and the reason is stated (CPU: the decision on measured
+    candidates)."""
+    from particles_amd.learned import choose_shift
+    cand = [{"shift": -0.3, "kernel_ms": 30.0, "error": 0.8e-5},
+            {"shift": -0.1, "kernel_ms": 20.0, "error": 1.2e-5},
+            {"shift": 0.1, "kernel_ms": 15.0, "error": 2.0e-5}]
+    # the fastest shift at no larger error beats PI: kept
+    best, cal, beats, why = choose_shift(cand, pi_ms=25.0, pi_error=1.3e-5)
+    assert best["shift"] == -0.1 and cal and beats and why == ""
+    # qualifying shifts exist but none is faster than PI: PI
+    best, cal, beats, why = choose_shift(cand, pi_ms=19.0, pi_error=1.3e-5)
+    assert best is None and cal and not beats and "no faster" in why
+    # equal time is not faster
+    best, _, beats, _ = choose_shift(cand, pi_ms=20.0, pi_error=1.3e-5)
+    assert best is None and not beats
+    # no shift reaches PI's error: PI, never the least inaccurate network
+    best, cal, beats, why = choose_shift(cand, pi_ms=100.0, pi_error=0.5e-5)
+    assert best is None and not cal and not beats and "error" in why
+
+
 # ---------------------------------------------------------------------------
 # device (GPU)
 # ---------------------------------------------------------------------------
