@@ -134,6 +134,25 @@ int ptc_integrate(const ptc_fields *f, int method, int64_t np, int m, const doub
                   int nparams, const double *time, double *state, int64_t *counter,
                   int32_t *status);
 
+/* ptc_integrate without any device allocation, for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  No counterpart in the reference.  time, state
+ * and counter (and status, which may be NULL) are DEVICE pointers with the
+ * layouts of ptc_integrate; the batch state lives in a fixed workspace
+ * compiled into the library (7.3 MiB per device, held until the process
+ * ends), so np is limited to 0..65536 (PTC_EARG beyond; no hipMalloc,
+ * hipFree or allocating library call is made).  The work is enqueued on
+ * `stream` (a hipStream_t; NULL = the null stream) and the call returns
+ * without waiting: results are complete once the caller synchronises that
+ * stream.  Results equal ptc_integrate's bit for bit (PI controller; the
+ * work queue in index order, which changes no result).  Every call on one
+ * device shares that device's workspace: calls that overlap on one device
+ * (other streams or threads before the previous call's stream is
+ * synchronised) race and give undefined results. */
+int ptc_integrate_noalloc(const ptc_fields *f, int method, int64_t np, int m,
+                          const double *params, int nparams, const double *time, double *state,
+                          int64_t *counter, int32_t *status, void *stream);
+
 /* Device-resident particle batch: the same integration split into
  * segments, so callers can flush snapshots per interval and benchmark with
  * inputs already in HBM.  create uploads state(8,m,np) (only state(1:6,1,:)

@@ -478,24 +478,28 @@ int ptc_fields_destroy(ptc_fields *f)
 // kernel's series form keeps its particles to the end) takes the group
 // kernel in FP64 as well (measured on 8 snapshots of 256^3 with 100,000
 // particles: DOP853 1.15x, RK5 2.3x the lane kernel; FP32 DOP853 0.62x).
-static int resolve_tail(int tail, const ptc_particles *p)
+static int choose_tail(int tail, const DeviceFields &f, int method, int64_t np, int ctrl)
 {
-    const DeviceFields &f = p->fields->d;
     // the learned controller runs in the group kernel on FP32 node rows
     // only, and never hands over; training records come from one lane per
     // particle
-    const bool learned = p->ctrl == CTRL_LEARNED;
+    const bool learned = ctrl == CTRL_LEARNED;
     if (f.layout == LAYOUT_PAIRS) return kTailGroupOnly;  // read by the group kernel only
     if (learned && f.prec != PTC_FP32) return 0;
     if (tail != kTailAuto) return learned && tail > 0 ? 0 : tail;
-    if (f.layout != LAYOUT_AOS || (p->ctrl != CTRL_PI && !learned)) return 0;
+    if (f.layout != LAYOUT_AOS || (ctrl != CTRL_PI && !learned)) return 0;
     const double snap = (f.prec == PTC_FP32 ? 32.0 : 48.0) * (double)f.nodes;
     const double bytes = f.nsnap > 0 ? 2.0 * snap : snap;
-    if (p->method == PTC_RK5 || p->d.np <= 32768 || bytes > 2.0 * 1024 * 1024 * 1024 ||
+    if (method == PTC_RK5 || np <= 32768 || bytes > 2.0 * 1024 * 1024 * 1024 ||
         (f.nsnap > 0 && f.prec == PTC_FP64))
         return kTailGroupOnly;
     if (f.nsnap > 0 || learned) return 0;
     return f.prec == PTC_FP32 ? 0 : 32768;
+}
+
+static int resolve_tail(int tail, const ptc_particles *p)
+{
+    return choose_tail(tail, p->fields->d, p->method, p->d.np, p->ctrl);
 }
 
 static int build_step_params(const ptc_fields *f, int m, const double *params, int nparams,
@@ -1405,6 +1409,157 @@ int ptc_integrate(const ptc_fields *f, int method, int64_t np, int m, const doub
     int rc2 = ptc_particles_destroy(p);
     if (rc) { g_err = keep; return rc; }
     return rc2;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// allocation-free integration (ptc_integrate_noalloc)
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr int64_t kNoAllocMaxParticles = 65536;
+// per particle: x[6], t, dt, errold (double); cnt[3], nrhs (int64);
+// nout, flags, status (int32)
+constexpr size_t kNoAllocPerParticle =
+    9 * sizeof(double) + 4 * sizeof(int64_t) + 3 * sizeof(int32_t);
+// header: the queue heads (3 int32) at byte 0, the cell-load counter at 64
+constexpr size_t kNoAllocHeader = 256;
+constexpr size_t kNoAllocBytes =
+    kNoAllocHeader + (size_t)kNoAllocMaxParticles * kNoAllocPerParticle;
+
+}  // namespace
+
+// The batch state of ptc_integrate_noalloc (7.3 MiB): each device holds its
+// own copy until the process ends, and every call on a device reuses it
+// (the call's first kernel re-initialises what it reads).  Host code takes
+// its device address from g_scratch_ptr, never from this name.
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+namespace {
+
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[256];  // device address of `scratch`, per device id
+
+// The device address of the current device's `scratch`, looked up once per
+// device (with that device current).
+int scratch_address(char **out)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return hip_fail(e, "ptc_integrate_noalloc: hipGetDevice");
+    if (dev < 0 || dev >= 256) return fail(PTC_EARG, "ptc_integrate_noalloc: device %d", dev);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *ptr = nullptr;
+        e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return hip_fail(e != hipSuccess ? e : hipErrorInvalidSymbol,
+                            "ptc_integrate_noalloc: hipGetSymbolAddress(scratch)");
+        g_scratch_ptr[dev] = ptr;
+    }
+    *out = (char *)g_scratch_ptr[dev];
+    return PTC_OK;
+}
+
+// Initial batch state from the caller's state(8, m, np) (rows 1:6 of
+// snapshot 1), as ptc_particles_create sets it up; zeroes the queue heads
+// and the cell-load counter.
+__global__ void noalloc_start_kernel(const DeviceParticles S, int m)
+{
+    if (blockIdx.x == 0 && threadIdx.x < 3) S.qcount[threadIdx.x] = 0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *S.loads = 0ull;
+    const int64_t N = S.np;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const double *s = S.out + (size_t)8 * m * i;
+#pragma unroll
+        for (int c = 0; c < 6; c++) S.x[c][i] = s[c];
+        S.t[i] = 0.0;
+        S.dt[i] = 0.0;
+        S.errold[i] = 1.0e-04;  // integrations.F90:349
+        S.nout[i] = 2;
+        S.flags[i] = 0;
+        S.status[i] = PTC_STATUS_RUNNING;
+        S.cnt[i] = 0;
+        S.cnt[N + i] = 0;
+        S.cnt[2 * N + i] = 0;
+        S.nrhs[i] = 0;
+    }
+}
+
+// The step counters into the caller's counter(3, np), the status into
+// status(np) (if given).
+__global__ void noalloc_finish_kernel(const DeviceParticles S, int64_t *counter, int32_t *status)
+{
+    const int64_t N = S.np;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        counter[3 * i] = S.cnt[i];
+        counter[3 * i + 1] = S.cnt[N + i];
+        counter[3 * i + 2] = S.cnt[2 * N + i];
+        if (status) status[i] = S.status[i];
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int ptc_integrate_noalloc(const ptc_fields *f, int method, int64_t np, int m,
+                          const double *params, int nparams, const double *time, double *state,
+                          int64_t *counter, int32_t *status, void *stream)
+{
+    if (!f) return fail(PTC_EARG, "ptc_integrate_noalloc: fields handle is null");
+    if (method != PTC_RK5 && method != PTC_DOP853) return fail(PTC_EARG, "bad method %d", method);
+    if (np < 0 || np > kNoAllocMaxParticles)
+        return fail(PTC_EARG, "ptc_integrate_noalloc: %lld particles; its fixed workspace holds "
+                              "0 to %lld (ptc_integrate takes any number)",
+                    (long long)np, (long long)kNoAllocMaxParticles);
+    if (m < 1) return fail(PTC_EARG, "need at least one snapshot (m = %d)", m);
+    if (np > 0 && (!time || !state || !counter))
+        return fail(PTC_EARG, "ptc_integrate_noalloc: null time, state or counter");
+    StepParams sp;
+    int rc = build_step_params(f, m, params, nparams, sp);
+    if (rc) return rc;
+    if (np == 0) { ok(); return PTC_OK; }
+    char *base = nullptr;
+    rc = scratch_address(&base);
+    if (rc) return rc;
+
+    const int64_t n = kNoAllocMaxParticles;  // plane capacity; planes are np apart
+    DeviceParticles d{};
+    d.np = np;
+    double *dx = (double *)(base + kNoAllocHeader);
+    for (int k = 0; k < 6; k++) d.x[k] = dx + k * np;
+    d.t = dx + 6 * np;
+    d.dt = dx + 7 * np;
+    d.errold = dx + 8 * np;
+    d.cnt = (int64_t *)(dx + 9 * n);
+    d.nrhs = d.cnt + 3 * np;
+    d.nout = (int32_t *)(d.cnt + 4 * n);
+    d.flags = d.nout + np;
+    d.status = d.nout + 2 * np;
+    d.qcount = (int32_t *)base;
+    d.loads = (unsigned long long *)(base + 64);
+    d.out = state;
+    d.time = time;
+    d.order = nullptr;
+
+    Options o = options();
+    sp.max_steps = o.max_steps;
+    const int cache = f->d.layout == LAYOUT_AOS || f->d.layout == LAYOUT_PAIRS ? o.cell_cache : 0;
+    LaunchConfig lc{o.block, o.grid_blocks, cache, CTRL_PI,
+                    choose_tail(o.tail, f->d, method, np, CTRL_PI)};
+    hipStream_t s = (hipStream_t)stream;
+    const int blocks = (int)((np + 255) / 256);
+    hipLaunchKernelGGL(noalloc_start_kernel, dim3(blocks), dim3(256), 0, s, d, m);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(launch_integrate(method, f->d, sp, d, INFINITY, lc, s));
+    hipLaunchKernelGGL(noalloc_finish_kernel, dim3(blocks), dim3(256), 0, s, d, counter, status);
+    HIP_TRY(hipGetLastError());
+    ok();
+    return PTC_OK;
 }
 
 }  // extern "C"
