@@ -90,7 +90,7 @@ struct Options {
     int sort = 1;   // re-order the work queue every `sort` advances (0 = never)
     int order = 2;  // the order then: 1 spatial (Morton), 2 cost (most work first)
     int tail = kTailAuto;  // LaunchConfig::tail, or chosen per batch
-    int f32_layout = LAYOUT_AOS;  // FP32 fields: node rows or LAYOUT_PAIRS
+    int f32_layout = LAYOUT_AOS;  // FP32 fields: node rows, LAYOUT_PAIRS or LAYOUT_QUADS
 };
 
 std::mutex g_opt_mu;
@@ -132,11 +132,13 @@ int ptc_set_option(const char *key, const char *value)
         if (!strcmp(value, "soa")) g_opt.layout = LAYOUT_SOA;
         else if (!strcmp(value, "aos")) g_opt.layout = LAYOUT_AOS;
         else if (!strcmp(value, "brick")) g_opt.layout = LAYOUT_BRICK;
-        else return fail(PTC_EARG, "layout must be soa, aos or brick (got %s)", value);
+        else if (!strcmp(value, "pairs")) g_opt.layout = LAYOUT_PAIRS;
+        else return fail(PTC_EARG, "layout must be soa, aos, brick or pairs (got %s)", value);
     } else if (!strcmp(key, "f32_layout")) {
         if (!strcmp(value, "nodes")) g_opt.f32_layout = LAYOUT_AOS;
         else if (!strcmp(value, "pairs")) g_opt.f32_layout = LAYOUT_PAIRS;
-        else return fail(PTC_EARG, "f32_layout must be nodes or pairs (got %s)", value);
+        else if (!strcmp(value, "quads")) g_opt.f32_layout = LAYOUT_QUADS;
+        else return fail(PTC_EARG, "f32_layout must be nodes, pairs or quads (got %s)", value);
     } else if (!strcmp(key, "max_steps")) {
         long long v = strtoll(value, &end, 10);
         if (*end || v < 0) return fail(PTC_EARG, "bad max_steps %s", value);
@@ -179,9 +181,14 @@ int ptc_get_option(const char *key, char *buf, int len)
     if (!strcmp(key, "block")) snprintf(buf, len, "%d", o.block);
     else if (!strcmp(key, "grid_blocks")) snprintf(buf, len, "%d", o.grid_blocks);
     else if (!strcmp(key, "layout"))
-        snprintf(buf, len, "%s", o.layout == LAYOUT_SOA ? "soa" : o.layout == LAYOUT_AOS ? "aos" : "brick");
+        snprintf(buf, len, "%s", o.layout == LAYOUT_SOA     ? "soa"
+                                 : o.layout == LAYOUT_AOS   ? "aos"
+                                 : o.layout == LAYOUT_PAIRS ? "pairs"
+                                                            : "brick");
     else if (!strcmp(key, "f32_layout"))
-        snprintf(buf, len, "%s", o.f32_layout == LAYOUT_PAIRS ? "pairs" : "nodes");
+        snprintf(buf, len, "%s", o.f32_layout == LAYOUT_PAIRS   ? "pairs"
+                                 : o.f32_layout == LAYOUT_QUADS ? "quads"
+                                                                : "nodes");
     else if (!strcmp(key, "max_steps")) snprintf(buf, len, "%lld", (long long)o.max_steps);
     else if (!strcmp(key, "cell_cache")) snprintf(buf, len, "%d", o.cell_cache);
     else if (!strcmp(key, "sort")) snprintf(buf, len, "%d", o.sort);
@@ -205,13 +212,18 @@ static int fields_from_device_soa(const int dm[3], double *soa, int mode, int pr
 {
     const int64_t nodes = (int64_t)dm[0] * dm[1] * dm[2];
     Options o = options();
-    if (prec == PTC_FP32) o.layout = o.f32_layout;  // FP32: node rows or pair rows
+    if (prec == PTC_FP32) o.layout = o.f32_layout;  // FP32: node, pair or quad rows
     void *data = soa, *bricks = nullptr;
     hipError_t e = hipSuccess;
-    if (o.layout == LAYOUT_PAIRS) {
+    if (o.layout == LAYOUT_PAIRS || o.layout == LAYOUT_QUADS) {
         void *rows = nullptr;
-        e = hipMalloc(&rows, sizeof(float) * 16 * nodes);
-        if (e == hipSuccess) e = launch_to_pairs_f32(soa, dm, (float *)rows, nullptr);
+        const size_t esz = prec == PTC_FP32 ? sizeof(float) : sizeof(double);
+        e = hipMalloc(&rows, esz * (size_t)row_values(o.layout, prec) * (size_t)nodes);
+        if (e == hipSuccess) {
+            if (o.layout == LAYOUT_QUADS) e = launch_to_quads_f32(soa, dm, (float *)rows, nullptr);
+            else if (prec == PTC_FP32) e = launch_to_pairs_f32(soa, dm, (float *)rows, nullptr);
+            else e = launch_to_pairs_f64(soa, dm, (double *)rows, nullptr);
+        }
         if (e == hipSuccess) e = hipDeviceSynchronize();
         (void)hipFree(soa);
         if (e != hipSuccess) {
@@ -484,7 +496,8 @@ static int choose_tail(int tail, const DeviceFields &f, int method, int64_t np, 
     // only, and never hands over; training records come from one lane per
     // particle
     const bool learned = ctrl == CTRL_LEARNED;
-    if (f.layout == LAYOUT_PAIRS) return kTailGroupOnly;  // read by the group kernel only
+    if (f.layout == LAYOUT_PAIRS || f.layout == LAYOUT_QUADS)
+        return kTailGroupOnly;  // read by the group kernel only
     if (learned && f.prec != PTC_FP32) return 0;
     if (tail != kTailAuto) return learned && tail > 0 ? 0 : tail;
     if (f.layout != LAYOUT_AOS || (ctrl != CTRL_PI && !learned)) return 0;
@@ -660,7 +673,7 @@ int ptc_particles_advance(ptc_particles *p, double t_stop)
     }
     p->sp.max_steps = o.max_steps;
     LaunchConfig lc{o.block, o.grid_blocks,
-                    p->fields->d.layout == LAYOUT_AOS || p->fields->d.layout == LAYOUT_PAIRS
+                    p->fields->d.layout != LAYOUT_SOA && p->fields->d.layout != LAYOUT_BRICK
                         ? o.cell_cache
                         : 0,
                     p->ctrl, resolve_tail(o.tail, p)};
@@ -807,7 +820,7 @@ int ptc_particles_advance_graph(ptc_particles *p, int n, const double *t_stop)
     }
     p->sp.max_steps = o.max_steps;
     LaunchConfig lc{o.block, o.grid_blocks,
-                    p->fields->d.layout == LAYOUT_AOS || p->fields->d.layout == LAYOUT_PAIRS
+                    p->fields->d.layout != LAYOUT_SOA && p->fields->d.layout != LAYOUT_BRICK
                         ? o.cell_cache
                         : 0,
                     p->ctrl, resolve_tail(o.tail, p)};
@@ -994,8 +1007,9 @@ int ptc_probe(const ptc_fields *f, int op, int64_t n, const double *params, int 
     if (rc) return rc;
     if (op >= PTC_PROBE_FEATURES && f->d.layout != LAYOUT_AOS)
         return fail(PTC_EARG, "ptc_probe: op %d needs the aos layout", op);
-    if (f->d.layout == LAYOUT_PAIRS)
-        return fail(PTC_EARG, "ptc_probe: the probes read node rows (f32_layout nodes)");
+    if (f->d.layout == LAYOUT_PAIRS || f->d.layout == LAYOUT_QUADS)
+        return fail(PTC_EARG, "ptc_probe: the probes read node rows (layout aos, f32_layout "
+                              "nodes)");
     const int nin = op == PTC_PROBE_GATHER     ? 3
                     : op == PTC_PROBE_POW      ? 2
                     : op == PTC_PROBE_FEATURES ? 10
@@ -1100,10 +1114,12 @@ int ptc_particles_use_mlp(ptc_particles *p, const ptc_mlp *net, double shift)
         ok();
         return PTC_OK;
     }
-    if ((p->fields->d.layout != LAYOUT_AOS && p->fields->d.layout != LAYOUT_PAIRS) ||
-        p->fields->d.nsnap > 0)
+    const DeviceFields &fd = p->fields->d;
+    const bool rows_ok = fd.layout == LAYOUT_AOS ||
+                         (fd.prec == PTC_FP32 && (fd.layout == LAYOUT_PAIRS || fd.layout == LAYOUT_QUADS));
+    if (!rows_ok || fd.nsnap > 0)
         return fail(PTC_ESTATE, "ptc_particles_use_mlp: needs a static field in the aos layout "
-                                "(or FP32 pair rows)");
+                                "(or FP32 pair or quad rows)");
     if (!net->trained)
         return fail(PTC_ESTATE, "ptc_particles_use_mlp: the network still has its random "
                                 "initialisation (train it or set its parameters)");
@@ -1548,7 +1564,7 @@ int ptc_integrate_noalloc(const ptc_fields *f, int method, int64_t np, int m,
 
     Options o = options();
     sp.max_steps = o.max_steps;
-    const int cache = f->d.layout == LAYOUT_AOS || f->d.layout == LAYOUT_PAIRS ? o.cell_cache : 0;
+    const int cache = f->d.layout != LAYOUT_SOA && f->d.layout != LAYOUT_BRICK ? o.cell_cache : 0;
     LaunchConfig lc{o.block, o.grid_blocks, cache, CTRL_PI,
                     choose_tail(o.tail, f->d, method, np, CTRL_PI)};
     hipStream_t s = (hipStream_t)stream;

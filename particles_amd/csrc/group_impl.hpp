@@ -176,13 +176,16 @@ __device__ __forceinline__ void group_corners(const StepParams &P, const double 
 // rows of eight floats, FP32 weights, sums and force; the slope is a float.
 // FEAT = 1 also keeps, from the last evaluation, component comp of the
 // interpolated fields and the in-cell fraction (step-size features).
-// PAIRS = 1 (static FP32 field in LAYOUT_PAIRS): the four x-left corner rows
-// also hold the x-right corners, read from them when the cell's right x
-// index is the left one's periodic successor (else from their own rows).
-template <int MODE, int GEOM, int PREC, int TIME = 0, int FEAT = 0, int PAIRS = 0>
+// ROWS = ROWS_PAIRS (static field in LAYOUT_PAIRS, FP32 or FP64): the four
+// x-left corner rows also hold the x-right corners, read from them when the
+// cell's right x index is the left one's periodic successor (else from their
+// own rows).  ROWS = ROWS_QUADS (static FP32 field in LAYOUT_QUADS): the two
+// (x-left, y-left) corner rows hold all four corners of their z plane, read
+// from them where the right x and y indices are the left ones' successors.
+template <int MODE, int GEOM, int PREC, int TIME = 0, int FEAT = 0, int ROWS = ROWS_NODES>
 struct GroupForce {
-    static_assert(PAIRS == 0 || (PREC == PTC_FP32 && TIME == 0),
-                  "pair rows hold a static FP32 field");
+    static_assert(ROWS == ROWS_NODES || (TIME == 0 && (ROWS == ROWS_PAIRS || PREC == PTC_FP32)),
+                  "pair rows hold a static field, quad rows a static FP32 field");
     using K = std::conditional_t<PREC == PTC_FP32, float, double>;
     const DeviceFields &F;
     const StepParams &P;
@@ -252,10 +255,10 @@ struct GroupForce {
         }
     }
 
-    // the same from pair rows (PAIRS = 1)
+    // the same from pair rows (ROWS_PAIRS)
     __device__ __forceinline__ void load_pairs(const uint32_t *o, K *dst) const
     {
-        const float *__restrict__ d = (const float *)F.data + comp;
+        const K *__restrict__ d = (const K *)F.data + comp;
         // x1 - x0 (mod 2^32) from the x-right and x-left offsets
         const uint32_t dx = o[4] - o[0];
         const bool succ = dx == 1u || dx == (uint32_t)(1 - P.dm[0]);
@@ -267,6 +270,26 @@ struct GroupForce {
         } else {
 #pragma unroll
             for (int j = 0; j < 4; j++) dst[4 + j] = d[(size_t)o[4 + j] * 16];
+        }
+    }
+
+    // the same from quad rows (ROWS_QUADS): corner j sits in the row of
+    // corner j & 1 (its z plane's x-left, y-left corner) at slot 6 x + 12 y
+    // (x, y: its right flags) when the x and y right indices are the left
+    // ones' periodic successors, else at slot 0 of its own row
+    __device__ __forceinline__ void load_quads(const uint32_t *o, K *dst) const
+    {
+        const K *__restrict__ d = (const K *)F.data + comp;
+        const uint32_t dx = o[4] - o[0], dy = o[2] - o[0];
+        const bool xs = dx == 1u || dx == (uint32_t)(1 - P.dm[0]);
+        const bool ys = dy == (uint32_t)P.dm[0] ||
+                        dy == (uint32_t)((int64_t)P.dm[0] * (1 - P.dm[1]));
+        const bool quad = xs && ys;
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const uint32_t row = quad ? o[j & 1] : o[j];
+            const int slot = quad ? 6 * (j >> 2) + 12 * ((j >> 1) & 1) : 0;
+            dst[j] = d[(size_t)row * 32 + slot];
         }
     }
 
@@ -307,7 +330,8 @@ struct GroupForce {
         } else {
             if (c.o[0] != key0 || c.o[7] != key7) {
                 loads++;
-                if constexpr (PAIRS == 1) load_pairs(c.o, v);
+                if constexpr (ROWS == ROWS_PAIRS) load_pairs(c.o, v);
+                else if constexpr (ROWS == ROWS_QUADS) load_quads(c.o, v);
                 else load(c.o, 0, v);
                 key0 = c.o[0];
                 key7 = c.o[7];
@@ -543,7 +567,8 @@ constexpr int group_waves(int prec, int time, int ctrl)
 // proposes the next step, as in integrate_kernel, evaluated once per wave
 // for its ten particles (mlp_forward_groups; the network and a 64x72 BF16
 // buffer per wave in dynamic LDS).
-template <int METHOD, int MODE, int GEOM, int PREC, int TIME, int CTRL = CTRL_PI, int PAIRS = 0>
+template <int METHOD, int MODE, int GEOM, int PREC, int TIME, int CTRL = CTRL_PI,
+          int ROWS = ROWS_NODES>
 __global__ void __launch_bounds__(kBlock, group_waves(PREC, TIME, CTRL))
 integrate_group_kernel(const DeviceFields F, const StepParams P, const DeviceParticles S,
                        double t_stop)
@@ -568,7 +593,7 @@ integrate_group_kernel(const DeviceFields F, const StepParams P, const DevicePar
     const int comp = cl;
     const double t_end = t_stop < P.tmx ? t_stop : P.tmx;
     const int64_t N = S.np;
-    GroupForce<MODE, GEOM, PREC, TIME, CTRL == CTRL_LEARNED ? 1 : 0, PAIRS> force(F, P, base,
+    GroupForce<MODE, GEOM, PREC, TIME, CTRL == CTRL_LEARNED ? 1 : 0, ROWS> force(F, P, base,
                                                                                 comp);
     using K = typename decltype(force)::K;
     // this lane's state plane, chosen once (indexing the kernel argument's
@@ -833,11 +858,11 @@ integrate_group_kernel(const DeviceFields F, const StepParams P, const DevicePar
 }
 
 template <int METHOD, int MODE, int GEOM, int PREC, int TIME = 0, int CTRL = CTRL_PI,
-          int PAIRS = 0>
+          int ROWS = ROWS_NODES>
 hipError_t launch_group_t(const DeviceFields &f, const StepParams &sp, const DeviceParticles &p,
                           double t_stop, int block, hipStream_t stream)
 {
-    auto kern = integrate_group_kernel<METHOD, MODE, GEOM, PREC, TIME, CTRL, PAIRS>;
+    auto kern = integrate_group_kernel<METHOD, MODE, GEOM, PREC, TIME, CTRL, ROWS>;
     const size_t lds =
         CTRL == CTRL_LEARNED ? kNetLds + (size_t)(block / 64) * sizeof(MlpWaveBuf) : 0;
     if (CTRL == CTRL_LEARNED && !p.net) return hipErrorInvalidValue;

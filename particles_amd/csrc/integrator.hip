@@ -153,22 +153,68 @@ __global__ void to_pairs_f32_kernel(const double *__restrict__ soa, int dm0, int
     }
 }
 
+// SOA planes (FP64) -> FP64 pair rows: node n's six values, then those of
+// its +x neighbour (periodic in x), then four zeros (128 bytes)
+__global__ void to_pairs_f64_kernel(const double *__restrict__ soa, int dm0, int64_t nodes,
+                                    double *__restrict__ out)
+{
+    for (int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; n < nodes;
+         n += (int64_t)gridDim.x * blockDim.x) {
+        const int i = (int)(n % dm0);
+        const int64_t nx = n - i + (i + 1 == dm0 ? 0 : i + 1);
+        double *__restrict__ row = out + 16 * n;
+#pragma unroll
+        for (int c = 0; c < 6; c++) {
+            row[c] = soa[c * nodes + n];
+            row[6 + c] = soa[c * nodes + nx];
+        }
+#pragma unroll
+        for (int c = 12; c < 16; c++) row[c] = 0.0;
+    }
+}
+
+// SOA planes (FP64) -> FP32 quad rows: the six values of node n = (i, j,
+// k), of (i+1, j, k), (i, j+1, k) and (i+1, j+1, k) (periodic), then eight
+// zeros (128 bytes)
+__global__ void to_quads_f32_kernel(const double *__restrict__ soa, int dm0, int dm1,
+                                    int64_t nodes, float *__restrict__ out)
+{
+    for (int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; n < nodes;
+         n += (int64_t)gridDim.x * blockDim.x) {
+        const int i = (int)(n % dm0);
+        const int j = (int)((n / dm0) % dm1);
+        const int64_t base = n - i - (int64_t)j * dm0;  // node (0, 0, k)
+        const int i1 = i + 1 == dm0 ? 0 : i + 1, j1 = j + 1 == dm1 ? 0 : j + 1;
+        const int64_t q[4] = {n, base + i1 + (int64_t)j * dm0, base + i + (int64_t)j1 * dm0,
+                              base + i1 + (int64_t)j1 * dm0};
+        float *__restrict__ row = out + 32 * n;
+#pragma unroll
+        for (int a = 0; a < 4; a++)
+#pragma unroll
+            for (int c = 0; c < 6; c++) row[6 * a + c] = float(soa[c * nodes + q[a]]);
+#pragma unroll
+        for (int c = 24; c < 32; c++) row[c] = 0.0f;
+    }
+}
+
 // One component of a stored field back to a plane of doubles (the
 // reference's uu(:,:,:,c) / bb(:,:,:,c) order): from the six SOA planes,
 // FP64 node rows, FP32 node rows (eight floats) or FP32 pair rows (sixteen
-// floats, the node's own six first).
+// floats, the node's own six first), FP32 quad rows (32 floats) or FP64
+// pair rows (16 doubles).
 __global__ void extract_plane_kernel(const void *__restrict__ data, int layout, int prec,
                                      int64_t nodes, int comp, double *__restrict__ out)
 {
+    const int64_t rv = row_values(layout, prec);
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nodes;
          i += (int64_t)gridDim.x * blockDim.x) {
         double v;
         if (prec == PTC_FP32)
-            v = double(((const float *)data)[i * (layout == LAYOUT_PAIRS ? 16 : 8) + comp]);
+            v = double(((const float *)data)[i * rv + comp]);
         else if (layout == LAYOUT_SOA)
             v = ((const double *)data)[comp * nodes + i];
         else
-            v = ((const double *)data)[i * 6 + comp];
+            v = ((const double *)data)[i * rv + comp];
         out[i] = v;
     }
 }
@@ -202,6 +248,21 @@ hipError_t launch_integrate(int method, const DeviceFields &f, const StepParams 
                             hipStream_t stream)
 {
     if (f.prec == PTC_FP32) return launch_integrate_mixed(method, f, sp, p, t_stop, lc, stream);
+    if (f.layout == LAYOUT_PAIRS) {
+        // FP64 pair rows: static field, PI controller, the group kernel only
+        if (f.nsnap > 0 || lc.ctrl != CTRL_PI || lc.tail != kTailGroupOnly)
+            return hipErrorInvalidValue;
+        const int g = sp.pow2 ? 1 : 0;
+#define GROUP_P(M, MO, G)                                                      \
+    if (method == M && f.mode == MO && g == G)                                 \
+        return launch_group_t<M, MO, G, PTC_FP64, 0, CTRL_PI, ROWS_PAIRS>(f, sp, p, t_stop, kBlock, stream);
+        GROUP_P(PTC_RK5, PTC_MODE_VB, 0) GROUP_P(PTC_RK5, PTC_MODE_EB, 0)
+        GROUP_P(PTC_DOP853, PTC_MODE_VB, 0) GROUP_P(PTC_DOP853, PTC_MODE_EB, 0)
+        GROUP_P(PTC_RK5, PTC_MODE_VB, 1) GROUP_P(PTC_RK5, PTC_MODE_EB, 1)
+        GROUP_P(PTC_DOP853, PTC_MODE_VB, 1) GROUP_P(PTC_DOP853, PTC_MODE_EB, 1)
+#undef GROUP_P
+        return hipErrorInvalidValue;
+    }
     if (f.nsnap > 0) {
         // snapshot series: node rows, PI controller; the cell cache keeps
         // the second snapshot in LDS where the device has room for it
@@ -314,7 +375,8 @@ hipError_t launch_probe(const DeviceFields &f, const StepParams &sp, int op, int
     if (n <= 0) return hipSuccess;
     if (op < 0 || op > 7) return hipErrorInvalidValue;
     if (op >= 5 && f.layout != LAYOUT_AOS) return hipErrorInvalidValue;
-    if (f.layout == LAYOUT_PAIRS) return hipErrorInvalidValue;  // the probes read node rows
+    if (f.layout == LAYOUT_PAIRS || f.layout == LAYOUT_QUADS)
+        return hipErrorInvalidValue;  // the probes read node rows
     if (f.prec == PTC_FP32) return launch_probe_mixed(f, sp, op, n, in, out, stream);
     if (f.mode == PTC_MODE_VB && f.layout == LAYOUT_SOA)
         return launch_probe_t<PTC_MODE_VB, LAYOUT_SOA>(f, sp, op, n, in, out, stream);
@@ -383,6 +445,26 @@ hipError_t launch_to_pairs_f32(const double *soa, const int dm[3], float *out, h
     if (blocks > 65536) blocks = 65536;
     hipLaunchKernelGGL(to_pairs_f32_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, soa,
                        dm[0], nodes, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_to_pairs_f64(const double *soa, const int dm[3], double *out, hipStream_t stream)
+{
+    const int64_t nodes = (int64_t)dm[0] * dm[1] * dm[2];
+    int64_t blocks = (nodes + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(to_pairs_f64_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, soa,
+                       dm[0], nodes, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_to_quads_f32(const double *soa, const int dm[3], float *out, hipStream_t stream)
+{
+    const int64_t nodes = (int64_t)dm[0] * dm[1] * dm[2];
+    int64_t blocks = (nodes + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(to_quads_f32_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, soa,
+                       dm[0], dm[1], nodes, out);
     return hipGetLastError();
 }
 

@@ -12,15 +12,18 @@ hipError_t launch_integrate_mixed(int method, const DeviceFields &f, const StepP
                                   const DeviceParticles &p, double t_stop, const LaunchConfig &lc,
                                   hipStream_t stream)
 {
-    if (f.prec != PTC_FP32 || (f.layout != LAYOUT_AOS && f.layout != LAYOUT_PAIRS))
+    if (f.prec != PTC_FP32 ||
+        (f.layout != LAYOUT_AOS && f.layout != LAYOUT_PAIRS && f.layout != LAYOUT_QUADS))
         return hipErrorInvalidValue;
     const int geom = sp.pow2 ? 1 : 0;
-    if (f.layout == LAYOUT_PAIRS) {
-        // pair rows: static field, the group kernel only
+    if (f.layout == LAYOUT_PAIRS || f.layout == LAYOUT_QUADS) {
+        // pair or quad rows: static field, the group kernel only
         if (f.nsnap > 0 || lc.tail != kTailGroupOnly) return hipErrorInvalidValue;
-#define GROUP_P(M, MO, G, CT)                                                  \
-    if (method == M && f.mode == MO && geom == G && lc.ctrl == CT)             \
-        return launch_group_t<M, MO, G, PTC_FP32, 0, CT, 1>(f, sp, p, t_stop, kBlock, stream);
+        const int rows = f.layout == LAYOUT_QUADS ? ROWS_QUADS : ROWS_PAIRS;
+#define GROUP_R(M, MO, G, CT, R)                                               \
+    if (method == M && f.mode == MO && geom == G && lc.ctrl == CT && rows == R) \
+        return launch_group_t<M, MO, G, PTC_FP32, 0, CT, R>(f, sp, p, t_stop, kBlock, stream);
+#define GROUP_P(M, MO, G, CT) GROUP_R(M, MO, G, CT, ROWS_PAIRS) GROUP_R(M, MO, G, CT, ROWS_QUADS)
 #define GROUP_P_ALL(CT)                                                        \
         GROUP_P(PTC_RK5, PTC_MODE_VB, 0, CT) GROUP_P(PTC_RK5, PTC_MODE_EB, 0, CT) \
         GROUP_P(PTC_DOP853, PTC_MODE_VB, 0, CT) GROUP_P(PTC_DOP853, PTC_MODE_EB, 0, CT) \
@@ -30,6 +33,7 @@ hipError_t launch_integrate_mixed(int method, const DeviceFields &f, const StepP
         GROUP_P_ALL(CTRL_LEARNED)
 #undef GROUP_P_ALL
 #undef GROUP_P
+#undef GROUP_R
         return hipErrorInvalidValue;
     }
     if (f.nsnap > 0) {

@@ -9,16 +9,25 @@ namespace ptc {
 
 struct MlpPacked;
 
-enum Layout : int { LAYOUT_SOA = 0, LAYOUT_AOS = 1, LAYOUT_BRICK = 2, LAYOUT_PAIRS = 3 };
+enum Layout : int {
+    LAYOUT_SOA = 0, LAYOUT_AOS = 1, LAYOUT_BRICK = 2, LAYOUT_PAIRS = 3, LAYOUT_QUADS = 4
+};
+// the group kernel's row forms (its ROWS template argument)
+enum : int { ROWS_NODES = 0, ROWS_PAIRS = 1, ROWS_QUADS = 2 };
 
 // Device copy of one field snapshot.
 //   SOA: six planes of dm1*dm2*dm3 values (U/E x,y,z then B x,y,z), each x
 //        fastest -- the reference's uu(:,:,:,c) / bb(:,:,:,c) planes.
 //   AOS: one array of nodes, six values per node (U/E x,y,z, B x,y,z), so
 //        the eight corners of a cell are eight 48-byte rows.
-//   PAIRS (FP32 only): per node a 64-byte row of 16 floats, the node's six
-//        values, then those of its +x neighbour (periodic), then 4 zeros;
-//        a cell's eight corners are four aligned rows (the group kernel).
+//   PAIRS: per node a row of 16 values (64 bytes in FP32, 128 in FP64):
+//        the node's six, then those of its +x neighbour (periodic), then 4
+//        zeros; a cell's eight corners are four aligned rows, one cache
+//        line each in FP64 (the group kernel).
+//   QUADS (FP32 only): per node a 128-byte row of 32 floats: the six values
+//        of the node, of its +x, +y and +x+y neighbours (periodic), then 8
+//        zeros; a cell's eight corners are two aligned 128-byte rows (the
+//        group kernel).
 //   BRICK: AOS nodes plus, per cell, the eight corner rows of that cell in
 //        the reference's corner order (48 values, 384 bytes, three 128-byte
 //        lines) -- eight copies of every node, so a gather reads one
@@ -129,6 +138,13 @@ hipError_t launch_clear_flag(int32_t *flags, int64_t n, int32_t bit, hipStream_t
 // SOA planes -> AOS nodes (and FP64 -> FP32 when prec is PTC_FP32).
 hipError_t launch_to_aos(const double *soa, int64_t nodes, void *aos, int prec, hipStream_t stream);
 hipError_t launch_to_pairs_f32(const double *soa, const int dm[3], float *out, hipStream_t stream);
+hipError_t launch_to_pairs_f64(const double *soa, const int dm[3], double *out, hipStream_t stream);
+hipError_t launch_to_quads_f32(const double *soa, const int dm[3], float *out, hipStream_t stream);
+// values per node row of a stored layout (node rows: 6 FP64 or 8 FP32)
+constexpr int row_values(int layout, int prec)
+{
+    return layout == LAYOUT_QUADS ? 32 : layout == LAYOUT_PAIRS ? 16 : prec == 1 ? 8 : 6;
+}
 // AOS nodes -> per-cell corner bricks.
 hipError_t launch_to_bricks(const DeviceFields &f, hipStream_t stream);
 // Component comp (0..5: U or E x, y, z, then B x, y, z) of snapshot snap

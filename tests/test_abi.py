@@ -64,6 +64,10 @@ def test_options_roundtrip_and_validation():
             with pytest.raises(P.PtcError):
                 P.set_option("f32_layout", bad)
         assert P.get_option("f32_layout") == "pairs"
+        P.set_option("f32_layout", "quads")
+        assert P.get_option("f32_layout") == "quads"
+        P.set_option("layout", "pairs")
+        assert P.get_option("layout") == "pairs"
     finally:
         P.set_option("layout", old)
         P.set_option("tail", "auto")

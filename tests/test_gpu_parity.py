@@ -637,7 +637,8 @@ def test_handover_counts_retired_particles(oracle):
     assert np.array_equal(c2, co) and np.array_equal(s2, so, equal_nan=True)
 
 
-@pytest.mark.parametrize("form", ["aos", "soa", "brick", "fp32 nodes", "fp32 pairs", "series"])
+@pytest.mark.parametrize("form", ["aos", "soa", "brick", "pairs", "fp32 nodes", "fp32 pairs",
+                                  "fp32 quads", "series"])
 def test_fields_download_returns_stored_values(form):
     """ptc_fields_download gives back what each storage form holds: FP64
     layouts exactly the uploaded arrays, FP32 ones the arrays rounded to
@@ -648,7 +649,7 @@ def test_fields_download_returns_stored_values(form):
     bb = np.asfortranarray(rng.normal(size=dm + (3,)))
     old = P.get_option("layout")
     try:
-        if form in ("aos", "soa", "brick"):
+        if form in ("aos", "soa", "brick", "pairs"):
             P.set_option("layout", form)
             with P.Fields(uu, bb) as f:
                 u2, b2 = f.download()
@@ -717,17 +718,33 @@ def _fields_fp32(uu, bb, layout, **kw):
         P.set_option("f32_layout", "nodes")
 
 
+def _fields_rows(uu, bb, form, **kw):
+    """Fields in one of the group kernel's row forms: "nodes" / "fp64
+    nodes" (node rows), "pairs" (FP32 pair rows), "quads" (FP32 quad rows),
+    "fp64 pairs"."""
+    if form.startswith("fp64"):
+        P.set_option("layout", form.split()[1].replace("nodes", "aos"))
+        try:
+            return P.Fields(uu, bb, **kw)
+        finally:
+            P.set_option("layout", "aos")
+    return _fields_fp32(uu, bb, form, **kw)
+
+
+@pytest.mark.parametrize("form", ["pairs", "quads", "fp64 pairs"])
 @pytest.mark.parametrize("method", ["rk5", "dop853"])
-def test_pair_rows_match_node_rows(method):
-    """FP32 pair rows (a node and its +x neighbour in one 64-byte row) hold
-    the same values as node rows, and the group kernel reads the x-right
-    corners from the pair row only where the right index is the left one's
-    periodic successor: snapshots and counters identical to node rows with
-    the group kernel, in the unit box (power-of-two geometry), on a 12 x 10
-    x 8 grid in a shifted box, in EB mode, with particles started exactly on
+def test_pair_rows_match_node_rows(method, form):
+    """Pair rows (a node and its +x neighbour in one row: 64 bytes in FP32,
+    128 in FP64) and FP32 quad rows (a node and its +x, +y, +x+y
+    neighbours in one 128-byte row) hold the same values as node rows, and
+    the group kernel reads a right corner from the left corner's row only
+    where the right index is the left one's periodic successor: snapshots
+    and counters identical to node rows with the group kernel (FP64: to the
+    oracle), in the unit box (power-of-two geometry), on a 12 x 10 x 8 grid
+    in a shifted box, in EB mode, with particles started exactly on x and y
     cell faces (where the separately rounded right index can differ) and
-    across the periodic x boundary; every "tail" setting runs the group
-    kernel on pair rows."""
+    across the periodic x and y boundaries; every "tail" setting runs the
+    group kernel on these rows."""
     uu, bb = W.turbulent_vb(32, seed=51, nmodes=64)
     p = W.params_vector(tmax=1 * W.T0, dtmax=1.0, atol=1e-8, rtol=1e-8)
     tg = W.time_grid(3, p[2])
@@ -735,6 +752,9 @@ def test_pair_rows_match_node_rows(method):
     rng = np.random.default_rng(53)
     st0[0, 0, :200] = rng.integers(0, 32, 200) / 32.0  # on x faces
     st0[0, 0, 200:260] = 1.0 - 1e-12  # last cell in x
+    st0[1, 0, 260:460] = rng.integers(0, 32, 200) / 32.0  # on y faces
+    st0[1, 0, 460:520] = 1.0 - 1e-12  # last cell in y
+    st0[:2, 0, 520:560] = 1.0 - 1e-12  # last cell in x and y
     cases = [("unit box", uu, bb, {}, p, st0)]
     ug, bg = W.turbulent_vb(32, seed=54, nmodes=64)
     ug, bg = np.asfortranarray(ug[:12, :10, :8]), np.asfortranarray(bg[:12, :10, :8])
@@ -743,18 +763,20 @@ def test_pair_rows_match_node_rows(method):
     sg = W.initial_state(800, len(tg), seed=55, gamma_range=(1.0, 100.0),
                          xl=(-1.0, 0.5, 2.0), xs=(2.5, 3.0, 1.75))
     sg[0, 0, :200] = -1.0 + 2.5 * rng.integers(0, 12, 200) / 12.0
+    sg[1, 0, 200:400] = 0.5 + 3.0 * rng.integers(0, 10, 200) / 10.0
     cases.append(("12x10x8 shifted box", ug, bg, {}, g, sg))
     ee, be = W.reconnection_eb(32, seed=56, nmodes=64)
     cases.append(("EB", ee, be, {"mode": "eb"}, p, st0))
+    node_form = "fp64 nodes" if form.startswith("fp64") else "nodes"
     try:
         for name, a, b, kw, q, s0 in cases:
-            say(f"[pairs] {method} {name}")
+            say(f"[rows] {method} {form} {name}")
             P.set_option("tail", "group")
-            with _fields_fp32(a, b, "nodes", **kw) as f:
+            with _fields_rows(a, b, node_form, **kw) as f:
                 ref = P.integrate(f, method, q, tg, s0)
             for tail in ("group", "0", "auto"):
                 P.set_option("tail", tail)
-                with _fields_fp32(a, b, "pairs", **kw) as f:
+                with _fields_rows(a, b, form, **kw) as f:
                     got = P.integrate(f, method, q, tg, s0)
                 assert (got[2] == 0).all()
                 assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1]), (name, tail)
@@ -762,10 +784,13 @@ def test_pair_rows_match_node_rows(method):
         P.set_option("tail", "auto")
 
 
-def test_pair_rows_refused_where_node_rows_are_read():
+@pytest.mark.parametrize("form", ["pairs", "quads", "fp64 pairs"])
+def test_pair_rows_refused_where_node_rows_are_read(form):
     """Probes, record mode and snapshot series read node rows: they refuse
-    pair rows with an error; the learned controller runs on them (group
-    kernel) with the results it gives on node rows."""
+    pair and quad rows with an error; the learned controller runs on FP32
+    pair and quad rows (group kernel) with the results it gives on node
+    rows, and refuses FP64 pair rows (it runs FP64 fields one lane per
+    particle)."""
     from oracle import mlp_oracle as M
     uu, bb = W.turbulent_vb(16, seed=57, nmodes=32)
     p = W.params_vector(tmax=0.5 * W.T0, dtmax=1.0, atol=1e-8, rtol=1e-8)
@@ -773,7 +798,19 @@ def test_pair_rows_refused_where_node_rows_are_read():
     st0 = W.initial_state(300, len(tg), seed=58, gamma_range=(1.0, 100.0))
     zero = np.zeros(P.MLP_PARAMS, dtype=np.float32)
     zero[M.SIG:] = 1.0  # output 0: keep h, within the PI floor
-    with _fields_fp32(uu, bb, "pairs") as f, P.Mlp(zero) as net:
+    if form.startswith("fp64"):
+        with _fields_rows(uu, bb, form) as f, P.Mlp(zero) as net:
+            with pytest.raises(P.PtcError):
+                P.probe(f, "gather", np.full((4, 3), 0.3), p)
+            with pytest.raises(P.PtcError):
+                P.Fields.series([f, f], [0.0, 1.0])
+            with P.Particles(f, "dop853", p, tg, st0) as b:
+                with pytest.raises(P.PtcError):
+                    b.record(1.0, 1000)
+                with pytest.raises(P.PtcError):
+                    b.use_mlp(net)
+        return
+    with _fields_fp32(uu, bb, form) as f, P.Mlp(zero) as net:
         with pytest.raises(P.PtcError):
             P.probe(f, "gather", np.full((4, 3), 0.3), p)
         with P.Particles(f, "dop853", p, tg, st0) as b:
