@@ -316,60 +316,80 @@ def summary(r, npart, steps, prec, method, plain_tps=None, mixed_tps=None):
     return out
 
 
+def _fp32_field(fields64, make_fields, layout):
+    """The FP32 field in `layout`: converted on the device from the FP64
+    field where its storage allows (no second synthesis, no plane copy),
+    else built again."""
+    import particles_amd as P
+    P.set_option("f32_layout", layout)
+    try:
+        if fields64.prec == "fp64" and P.get_option("layout") != "brick":
+            return fields64.convert("fp32")
+        return make_fields("fp32")
+    finally:
+        P.set_option("f32_layout", "nodes")
+
+
 def variants(args, fields64, make_fields, p, tg, st0, method, plain):
     """Mixed precision and learned-h (mixed) on the same workload, then the
     accuracy of each against a tight FP64 run on the first particles."""
     import particles_amd as P
     from particles_amd import workloads as W
-    from particles_amd.learned import train_controller
+    from particles_amd.learned import calibrate, record_and_train, reference_states
 
     npart = st0.shape[2]
     out = {}
     plain_tps = npart * args.steps / plain["wall"]
-    P.set_option("f32_layout", args.f32_layout)
-    f32 = make_fields("fp32")
-    P.set_option("f32_layout", "nodes")
+    # accuracy: one T0 from the initial state, snapshot at the end, against
+    # the FP64 integrator at atol = rtol = 1e-12 (references first, while
+    # only the FP64 field is resident)
+    n = min(args.accuracy_particles, npart)
+    q = p.copy()
+    q[2] = W.T0
+    tga = np.array([0.0, W.T0])
+    sa = np.asfortranarray(st0[:, :1, :n].repeat(2, axis=1))
+    ref = reference_states(fields64, method, q, sa)
+    s_plain = P.integrate(fields64, method, q, tga, sa)[0]
+
+    # the network: records of the PI controller on FP32 node rows (record
+    # mode runs one lane per particle on node rows) of a separate particle
+    # set, trained on the GPU; the node rows are released before the run's
+    # FP32 field is built
+    st_train = W.initial_state(args.train_particles, 2, seed=args.seed + 101,
+                               gamma_range=tuple(args.train_gamma))
+    horizon = args.train_horizon * W.T0
+    f32n = _fp32_field(fields64, make_fields, "nodes")
+    net, info = record_and_train(f32n, method, p, st_train, horizon=horizon,
+                                 steps=args.train_steps, seed=args.seed)
+    f32n.close()
+    info["record_run"] = (f"{args.train_particles} particles (separate seed, gamma "
+                          f"{args.train_gamma[0]:g}..{args.train_gamma[1]:g}), "
+                          f"{args.train_horizon:g} T0, PI controller, FP32 node rows")
+    ref_cal = reference_states(fields64, method, p, st_train, horizon)
+
+    f32 = _fp32_field(fields64, make_fields, args.f32_layout)
     r = run_batch(f32, method, p, tg, st0, args.steps, args.warmup, label="mixed")
     out["mixed"] = summary(r, npart, args.steps, "fp32", method, plain_tps)
     out["mixed"]["f32_layout"] = args.f32_layout
     mixed_tps = out["mixed"]["particle_T0_per_s"]
     print(f"mixed: {out['mixed']['value']:.3e} accepted/s", file=sys.stderr, flush=True)
 
-    # network trained on records of a separate particle set under the PI
-    # controller, its log-step offset calibrated on that set against PI
-    st_train = W.initial_state(args.train_particles, 2, seed=args.seed + 101,
-                               gamma_range=tuple(args.train_gamma))
-    ctl = train_controller(f32, method, p, tg, st_train, fields_fp64=fields64,
-                           horizon=args.train_horizon * W.T0, steps=args.train_steps,
-                           seed=args.seed, fallback=False)
+    ctl = calibrate(net, f32, method, p, st_train, ref_cal, horizon=horizon, fallback=False,
+                    info=info)
     if args.shift is not None:
         ctl.shift = args.shift
     r = run_batch(f32, method, p, tg, st0, args.steps, args.warmup, ctl.net, ctl.shift,
                   label="learned")
     net_line = summary(r, npart, args.steps, "fp32", method, plain_tps, mixed_tps)
     net_line["controller"] = dict(
-        network="16-64-64-1 GELU, BF16 MFMA, FP32 accumulation", shift=ctl.shift,
-        record_run=f"{args.train_particles} particles (separate seed), "
-                   f"{args.train_horizon:g} T0, PI controller", **ctl.info)
+        network="16-64-64-1 GELU, BF16 MFMA, FP32 accumulation", shift=ctl.shift, **ctl.info)
     print(f"learned-h network: {net_line['value']:.3e} accepted/s (shift {ctl.shift})",
           file=sys.stderr, flush=True)
-
-    # accuracy: one T0 from the initial state, snapshot at the end, against
-    # the FP64 integrator at atol = rtol = 1e-12
-    n = min(args.accuracy_particles, npart)
-    q = p.copy()
-    q[2] = W.T0
-    tga = np.array([0.0, W.T0])
-    sa = np.asfortranarray(st0[:, :1, :n].repeat(2, axis=1))
-    tight = q.copy()
-    tight[5] = tight[6] = 1e-12
-    ref = P.integrate(fields64, method, tight, tga, sa)[0]
 
     def err(state):
         d = np.abs(state[:6, 1, :] - ref[:6, 1, :]).max(0)
         return {"median": float(np.median(d)), "p90": float(np.percentile(d, 90))}
 
-    s_plain = P.integrate(fields64, method, q, tga, sa)[0]
     s_mixed = P.integrate(f32, method, q, tga, sa)[0]
     with P.Particles(f32, method, q, tga, sa) as b:
         b.use_mlp(ctl.net, ctl.shift)
@@ -412,11 +432,14 @@ def main():
     ap.add_argument("--particles", type=int, default=0)
     ap.add_argument("--grid", type=int, default=0)
     ap.add_argument("--method", default="", choices=["", "rk5", "dop853"])
-    ap.add_argument("--layout", default="aos", choices=["aos", "soa", "brick"])
-    ap.add_argument("--f32-layout", default="nodes", choices=["nodes", "pairs"],
+    ap.add_argument("--layout", default=None, choices=["aos", "soa", "brick", "pairs"],
+                    help="FP64 field storage (default: pairs for reconnection1024, 128-byte "
+                         "rows of a node and its +x neighbour; aos node rows elsewhere)")
+    ap.add_argument("--f32-layout", default=None, choices=["nodes", "pairs", "quads"],
                     help="FP32 fields (--prec fp32 and the mixed / learned-h variants): "
-                         "32-byte node rows or 64-byte rows of a node and its +x neighbour "
-                         "(read by the group kernel only)")
+                         "32-byte node rows, 64-byte rows of a node and its +x neighbour, or "
+                         "128-byte rows of a node and its +x, +y, +x+y neighbours (default: "
+                         "quads for reconnection1024, nodes elsewhere)")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--cell-cache", type=int, default=1, choices=[0, 1])
     ap.add_argument("--tail", default="auto",
@@ -427,8 +450,9 @@ def main():
                     help="re-order the work queue every N steps (0 = never)")
     ap.add_argument("--autotune", action="store_true",
                     help="tune block size and queue order during the warm-up steps")
-    ap.add_argument("--order", default="cost", choices=["spatial", "cost"],
-                    help="cost: most work in the previous step first; spatial: Morton order")
+    ap.add_argument("--order", default="auto", choices=["spatial", "cost", "auto"],
+                    help="cost: most work in the previous step first; spatial: Morton order; "
+                         "auto: cost up to 2,000,000 particles, index order above")
     ap.add_argument("--prec", default="fp64", choices=["fp64", "fp32"],
                     help="fp32 = mixed precision (FP32 fields, weights and RHS; FP64 state)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
@@ -455,6 +479,11 @@ def main():
                     help="only copy a 1 GiB float64 device tensor three times (the known-size "
                          "copy that scales the rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
+    big = args.workload == "reconnection1024"
+    if args.layout is None:
+        args.layout = "pairs" if big else "aos"
+    if args.f32_layout is None:
+        args.f32_layout = "quads" if big else "nodes"
     if args.copy_bandwidth:
         import torch
         a = torch.ones(1 << 29, dtype=torch.float64, device="cuda")

@@ -70,6 +70,14 @@ typedef struct ptc_particles ptc_particles;
  * prec = PTC_FP32 stores the fields in float (mixed-precision mode). */
 int ptc_fields_create(const int dm[3], const double *f1, const double *bb, int mode, int prec,
                       ptc_fields **out);
+/* A second device copy of a static field in another precision and layout
+ * (those the "layout" / "f32_layout" options name for prec), built on the
+ * device from the stored values, so no host round trip or plane copy is
+ * needed: the FP32 field of mixed precision from an FP64 field (values
+ * rounded to float), or other row forms.  The same values as creating the
+ * new form from the same arrays.  The source stays valid; destroy both.
+ * No counterpart in the reference (one precision, fields.F90:33-35). */
+int ptc_fields_convert(const ptc_fields *src, int prec, ptc_fields **out);
 /* The stored field values back on the host, as the arrays ptc_fields_create
  * takes: f1 (U or E) and bb, each double (dm1, dm2, dm3, 3), x fastest
  * (the reference's uu / bb of sources/fields.F90:63-64).  FP32 storage
@@ -330,16 +338,22 @@ int ptc_particles_use_mlp(ptc_particles *p, const ptc_mlp *net, double shift);
  *   "grid_blocks" blocks of the persistent kernel (0 = fill the device)
  *   "layout"      field layout in HBM for the next ptc_fields_create:
  *                 "soa" (the reference's six planes), "aos" (six
- *                 components interleaved per node) or "brick" (aos plus
+ *                 components interleaved per node), "brick" (aos plus
  *                 each cell's eight corners in one 384-byte block; 9x the
- *                 memory of "soa"); FP64 fields only
- *   "f32_layout"  FP32 fields: "nodes" (default; 32-byte node rows) or
+ *                 memory of "soa") or "pairs" (128-byte rows holding a
+ *                 node and its +x neighbour, so a gather reads four
+ *                 aligned cache lines; 2.67x the memory of "aos"); FP64
+ *                 fields only
+ *   "f32_layout"  FP32 fields: "nodes" (default; 32-byte node rows),
  *                 "pairs" (64-byte rows holding a node and its +x
  *                 neighbour, so a gather reads four aligned 64-byte rows;
- *                 twice the memory).  Pair rows are read by the group
+ *                 twice the memory) or "quads" (128-byte rows holding a
+ *                 node and its +x, +y and +x+y neighbours, so a gather
+ *                 reads two aligned rows; four times the memory).  Pair
+ *                 and quad rows (FP64 and FP32) are read by the group
  *                 kernel only: every "tail" setting then runs it, and
  *                 probes, record mode and snapshot series refuse them.
- *                 Results are identical to "nodes" with the group kernel.
+ *                 Results are identical to node rows with the group kernel.
  *   "max_steps"   attempted-step cap per particle (0 = none)
  *   "cell_cache"  1 (default) keeps, per particle, the corner values of the
  *                 last cell it sampled (aos layout) in registers, so stage
@@ -349,11 +363,13 @@ int ptc_particles_use_mlp(ptc_particles *p, const ptc_mlp *net, double shift);
  *                 ("tail") always keeps its cell.
  *   "sort"        N > 0: before every N-th advance (default 1), re-order
  *                 the work queue ("order"); 0: index order
- *   "order"       "cost" (default): most RHS evaluations in the previous
+ *   "order"       "cost": most RHS evaluations in the previous
  *                 segment first, so the cheap particles fill the tail (the
  *                 first advance has no history and keeps index order);
  *                 "spatial": Morton order of the particles' cells
- *                 (neighbouring lanes gather from neighbouring memory).
+ *                 (neighbouring lanes gather from neighbouring memory);
+ *                 "auto" (default): cost for batches of at most 2,000,000
+ *                 particles, index order (no re-ordering) above.
  *                 Results are identical for every order.
  *   "tail"        node rows with the PI controller: which kernel
  *                 integrates.  "0": one lane per particle (all 64 lanes of

@@ -73,6 +73,7 @@ def _load():
         "ptc_particles_last_kernel_ms": ([vp, C.POINTER(dbl)], i32),
         "ptc_particles_cell_loads": ([vp, C.POINTER(i64)], i32),
         "ptc_fields_download": ([vp, i32, vp, vp], i32),
+        "ptc_fields_convert": ([vp, i32, C.POINTER(vp)], i32),
         "ptc_particles_destroy": ([vp], i32),
         "ptc_probe": ([vp, i32, i64, vp, i32, vp, vp], i32),
         "ptc_set_option": ([C.c_char_p, C.c_char_p], i32),
@@ -199,6 +200,17 @@ class Fields:
         _check(_lib.ptc_fields_series(arr, _ptr(times), len(snaps), C.byref(h)))
         self._h = h
         return self
+
+    def convert(self, prec: str = "fp32") -> "Fields":
+        """A second device copy in precision `prec` and the layout the
+        options name for it (ptc_fields_convert), built on the device."""
+        out = Fields.__new__(Fields)
+        out.dm = self.dm.copy()
+        out.mode, out.prec = self.mode, prec
+        h = C.c_void_p()
+        _check(_lib.ptc_fields_convert(self._h, PRECS[prec], C.byref(h)))
+        out._h = h
+        return out
 
     def download(self, snapshot: int = 0):
         """(f1, bb): the stored values back on the host as Fortran

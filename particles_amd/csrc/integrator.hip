@@ -197,6 +197,59 @@ __global__ void to_quads_f32_kernel(const double *__restrict__ soa, int dm0, int
     }
 }
 
+// value comp (0..5) of node n of a stored static field, any layout
+__device__ __forceinline__ double stored_value(const DeviceFields &F, int64_t n, int comp)
+{
+    const int64_t rv = row_values(F.layout, F.prec);
+    if (F.prec == PTC_FP32) return double(((const float *)F.data)[n * rv + comp]);
+    if (F.layout == LAYOUT_SOA) return ((const double *)F.data)[comp * F.nodes + n];
+    return ((const double *)F.data)[n * rv + comp];
+}
+
+// Rows of a new static field (layout, prec) from a stored one (any layout
+// and precision), node by node: node rows (FP64: six values; FP32: six, |B|
+// from the stored values in FP64, 0), pair rows (the node, its +x
+// neighbour, zeros) or FP32 quad rows (the node, +x, +y, +x+y, zeros), the
+// neighbours periodic.  The same values as building the new form from the
+// stored field's planes.
+__global__ void convert_rows_kernel(const DeviceFields F, void *__restrict__ out, int layout,
+                                    int prec)
+{
+    const int dm0 = F.dm[0], dm1 = F.dm[1];
+    const int64_t nodes = F.nodes;
+    const int rv = row_values(layout, prec);
+    for (int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; n < nodes;
+         n += (int64_t)gridDim.x * blockDim.x) {
+        const int i = (int)(n % dm0);
+        const int j = (int)((n / dm0) % dm1);
+        const int64_t base = n - i - (int64_t)j * dm0;
+        const int i1 = i + 1 == dm0 ? 0 : i + 1, j1 = j + 1 == dm1 ? 0 : j + 1;
+        const int64_t q[4] = {n, base + i1 + (int64_t)j * dm0, base + i + (int64_t)j1 * dm0,
+                              base + i1 + (int64_t)j1 * dm0};
+        const int nq = layout == LAYOUT_QUADS ? 4 : layout == LAYOUT_PAIRS ? 2 : 1;
+        for (int a = 0; a < 4; a++) {
+            if (a >= nq) break;
+#pragma unroll
+            for (int c = 0; c < 6; c++) {
+                const double v = stored_value(F, q[a], c);
+                if (prec == PTC_FP32) ((float *)out)[n * rv + 6 * a + c] = float(v);
+                else ((double *)out)[n * rv + 6 * a + c] = v;
+            }
+        }
+        if (prec == PTC_FP32 && layout == LAYOUT_AOS) {
+            const double bx = stored_value(F, n, 3), by = stored_value(F, n, 4),
+                         bz = stored_value(F, n, 5);
+            ((float *)out)[n * rv + 6] = float(sqrt(bx * bx + by * by + bz * bz));
+            ((float *)out)[n * rv + 7] = 0.0f;
+        } else {
+            for (int c = 6 * nq; c < rv; c++) {
+                if (prec == PTC_FP32) ((float *)out)[n * rv + c] = 0.0f;
+                else ((double *)out)[n * rv + c] = 0.0;
+            }
+        }
+    }
+}
+
 // One component of a stored field back to a plane of doubles (the
 // reference's uu(:,:,:,c) / bb(:,:,:,c) order): from the six SOA planes,
 // FP64 node rows, FP32 node rows (eight floats) or FP32 pair rows (sixteen
@@ -445,6 +498,19 @@ hipError_t launch_to_pairs_f32(const double *soa, const int dm[3], float *out, h
     if (blocks > 65536) blocks = 65536;
     hipLaunchKernelGGL(to_pairs_f32_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, soa,
                        dm[0], nodes, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_convert_rows(const DeviceFields &f, void *out, int layout, int prec,
+                               hipStream_t stream)
+{
+    if (f.nsnap > 0 || f.layout == LAYOUT_BRICK || layout == LAYOUT_SOA ||
+        layout == LAYOUT_BRICK || (layout == LAYOUT_QUADS && prec != PTC_FP32))
+        return hipErrorInvalidValue;
+    int64_t blocks = (f.nodes + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(convert_rows_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, f, out,
+                       layout, prec);
     return hipGetLastError();
 }
 

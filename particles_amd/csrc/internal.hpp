@@ -147,6 +147,11 @@ constexpr int row_values(int layout, int prec)
 }
 // AOS nodes -> per-cell corner bricks.
 hipError_t launch_to_bricks(const DeviceFields &f, hipStream_t stream);
+// Rows of a static field in another layout (node, pair or quad rows) and
+// precision from a stored static field (not bricks); out holds
+// row_values(layout, prec) values per node.
+hipError_t launch_convert_rows(const DeviceFields &f, void *out, int layout, int prec,
+                               hipStream_t stream);
 // Component comp (0..5: U or E x, y, z, then B x, y, z) of snapshot snap
 // (0 for a static field) as a plane of nodes doubles, x fastest.
 hipError_t launch_extract_plane(const DeviceFields &f, int snap, int comp, double *out,

@@ -74,20 +74,14 @@ def _final_error(state, ref):
     return float(np.median(d))
 
 
-def train_controller(fields: Fields, method: str, params, time, state_train, *,
-                     fields_fp64: Fields | None = None, horizon: float | None = None,
-                     steps: int = 3000, batch: int = 16384, lr: float = 2e-3, alpha: float = 1.0,
-                     shifts=(-0.45, -0.4, -0.35, -0.3, -0.25, -0.2, -0.15, -0.1, 0.0, 0.1),
-                     capacity: int = 60_000_000,
-                     seed: int = 1, fallback: bool = True) -> Controller:
-    """Records, trains and calibrates a learned controller for `fields` (the
-    precision it will run with).  `state_train` is (8, m, n) like every
-    state; records are taken over [0, horizon] (default params[2]).  The
-    calibration runs the same particles over the same horizon, with
-    `fields_fp64` (default: `fields`) at atol = rtol = 1e-12 as reference.
-    Where the network does not beat PI (choose_shift), the result is the PI
-    controller (net None) unless `fallback` is False, which keeps the
-    network at the shift closest to qualifying (for measurements)."""
+def record_and_train(fields: Fields, method: str, params, state_train, *,
+                     horizon: float | None = None, steps: int = 3000, batch: int = 16384,
+                     lr: float = 2e-3, alpha: float = 1.0, capacity: int = 60_000_000,
+                     seed: int = 1):
+    """Steps 1 and 2: records of the PI controller on `fields` (FP32 node
+    rows: record mode runs one lane per particle on node rows) over
+    [0, horizon], then the network normalised to them and trained on the
+    GPU.  Returns (network, info)."""
     params = np.asarray(params, dtype=np.float64)
     q = params.copy()
     if horizon is not None:
@@ -99,7 +93,7 @@ def train_controller(fields: Fields, method: str, params, time, state_train, *,
     net = Mlp(seed=seed)
     with Particles(fields, method, q, tg, st) as b:
         b.record(1.0, capacity)
-        ms_pi = b.advance()
+        b.advance()
         recs = b.records(device=True)
         info["records"] = len(recs)
         net.normalize(recs)
@@ -107,10 +101,38 @@ def train_controller(fields: Fields, method: str, params, time, state_train, *,
     info.update(loss_first=float(losses[:50].mean()), loss_last=float(losses[-50:].mean()),
                 train_steps=steps, batch=batch, alpha=alpha,
                 train_seconds=_time.perf_counter() - t0)
+    return net, info
 
-    tight = q.copy()
-    tight[5] = tight[6] = 1e-12
-    ref = integrate(fields_fp64 if fields_fp64 is not None else fields, method, tight, tg, st)[0]
+
+def reference_states(fields_fp64: Fields, method: str, params, state, horizon: float | None = None):
+    """The FP64 states at atol = rtol = 1e-12 over [0, horizon] from
+    state[:, 0, :], the calibration's accuracy reference."""
+    q = np.asarray(params, dtype=np.float64).copy()
+    if horizon is not None:
+        q[2] = horizon
+    q[5] = q[6] = 1e-12
+    tg = np.array([0.0, q[2]])
+    st = np.asfortranarray(np.asarray(state, dtype=np.float64)[:, :1, :].repeat(2, axis=1))
+    return integrate(fields_fp64, method, q, tg, st)[0]
+
+
+def calibrate(net: Mlp, fields: Fields, method: str, params, state_cal, ref, *,
+              horizon: float | None = None,
+              shifts=(-0.45, -0.4, -0.35, -0.3, -0.25, -0.2, -0.15, -0.1, 0.0, 0.1),
+              fallback: bool = True, info: dict | None = None) -> Controller:
+    """Step 3: each candidate shift of the network's log step on `fields`
+    (the precision and layout it will run with) against the PI controller,
+    both timed and compared with `ref` (reference_states of the same
+    particles); choose_shift decides.  Where the network does not beat PI,
+    the result is the PI controller (net None, closed here) unless
+    `fallback` is False, which keeps the network at the shift closest to
+    qualifying (for measurements)."""
+    info = dict(info or {})
+    q = np.asarray(params, dtype=np.float64).copy()
+    if horizon is not None:
+        q[2] = horizon
+    tg = np.array([0.0, q[2]])
+    st = np.asfortranarray(np.asarray(state_cal, dtype=np.float64)[:, :1, :].repeat(2, axis=1))
     with Particles(fields, method, q, tg, st) as b:
         ms_pi = b.advance()
         e_pi = _final_error(b.download()[0], ref)
@@ -135,3 +157,24 @@ def train_controller(fields: Fields, method: str, params, time, state_train, *,
     near = min(ok, key=lambda c: c["kernel_ms"]) if ok else min(cand, key=lambda c: c["error"])
     info["not_beating_pi"] = why
     return Controller(net=net, shift=near["shift"], beats_pi=False, info=info)
+
+
+def train_controller(fields: Fields, method: str, params, time, state_train, *,
+                     fields_fp64: Fields | None = None, horizon: float | None = None,
+                     steps: int = 3000, batch: int = 16384, lr: float = 2e-3, alpha: float = 1.0,
+                     shifts=(-0.45, -0.4, -0.35, -0.3, -0.25, -0.2, -0.15, -0.1, 0.0, 0.1),
+                     capacity: int = 60_000_000,
+                     seed: int = 1, fallback: bool = True) -> Controller:
+    """Records, trains and calibrates a learned controller for `fields` (the
+    precision it will run with, in node rows).  `state_train` is (8, m, n)
+    like every state; records are taken over [0, horizon] (default
+    params[2]).  The calibration runs the same particles over the same
+    horizon, with `fields_fp64` (default: `fields`) at atol = rtol = 1e-12
+    as reference (calibrate)."""
+    net, info = record_and_train(fields, method, params, state_train, horizon=horizon,
+                                 steps=steps, batch=batch, lr=lr, alpha=alpha,
+                                 capacity=capacity, seed=seed)
+    ref = reference_states(fields_fp64 if fields_fp64 is not None else fields, method, params,
+                           state_train, horizon)
+    return calibrate(net, fields, method, params, state_train, ref, horizon=horizon,
+                     shifts=shifts, fallback=fallback, info=info)

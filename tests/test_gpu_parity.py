@@ -670,6 +670,38 @@ def test_fields_download_returns_stored_values(form):
         P.set_option("layout", old)
 
 
+@pytest.mark.parametrize("target", ["fp32 nodes", "fp32 pairs", "fp32 quads", "fp64 pairs",
+                                    "fp64 aos"])
+@pytest.mark.parametrize("source", ["aos", "pairs", "soa"])
+def test_fields_convert_equals_direct_creation(source, target):
+    """ptc_fields_convert builds, on the device, the same stored values as
+    creating the target form from the host arrays (FP32: the values rounded
+    to float), and integrating on it gives the same snapshots and counters
+    as the directly created field (group kernel)."""
+    uu, bb = W.turbulent_vb(16, seed=61, nmodes=32)
+    prec, form = target.split()
+    opt = "f32_layout" if prec == "fp32" else "layout"
+    p = W.params_vector(tmax=0.3 * W.T0, dtmax=1.0, atol=1e-8, rtol=1e-8)
+    tg = W.time_grid(2, p[2])
+    st0 = W.initial_state(400, len(tg), seed=62, gamma_range=(1.0, 300.0))
+    old = P.get_option("layout"), P.get_option("f32_layout")
+    try:
+        P.set_option("layout", source)
+        src = P.Fields(uu, bb)
+        P.set_option("layout", "aos")
+        P.set_option(opt, form)
+        with src, src.convert(prec) as conv, P.Fields(uu, bb, prec=prec) as direct:
+            assert all(np.array_equal(a, b) for a, b in zip(conv.download(), direct.download()))
+            P.set_option("tail", "group")
+            a = P.integrate(conv, "dop853", p, tg, st0)
+            b = P.integrate(direct, "dop853", p, tg, st0)
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]) and (a[2] == 0).all()
+    finally:
+        P.set_option("layout", old[0])
+        P.set_option("f32_layout", old[1])
+        P.set_option("tail", "auto")
+
+
 @pytest.mark.parametrize("kernel", ["0", "group"])
 def test_cell_loads_counted(kernel):
     """The cell-load counter: a particle at rest in uniform fields never
