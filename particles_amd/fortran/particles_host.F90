@@ -11,6 +11,8 @@
 !!   * initial positions uniform in the region and momenta normal(0, vth)
 !!     with the intrinsic random_number in the reference's call order
 !!     (random.F90:104-220), and the moment printout (random.F90:231-353);
+!!     for an even number of particles the state equals, bit for bit, the
+!!     one the reference's own random module draws in a fresh process;
 !!   * the six field files (velx..magz[.fits|.fits.gz] in datadir) read as
 !!     FP64 and scaled by vunit / bunit (fields.F90:63-74);
 !!   * params(1:17) packed as particles.F90:317-333, the whole particle loop
@@ -20,7 +22,8 @@
 !! Extra command-line switches: --mixed (FP32 fields, mixed precision),
 !! --kernels auto|lanes|group|N (which device kernels integrate: the
 !! library's "tail" option, "lanes" = one lane per particle; default auto),
-!! --dry-run (stop before the integration; needs no GPU).
+!! --dry-run (write the initial state and params(1:17) to initial_state.fits
+!! and params.fits and stop before the integration; needs no GPU).
 !! Extra keys (the reference's field is static): field_snapshots = N > 1
 !! reads N field snapshots, snapshot s from <name>_<ssss>.fits[.gz]
 !! (s = 0001 ..), at times (s - 1) * field_dt in the unit of tmax, blended
@@ -431,7 +434,13 @@ program particles_host
   if (dry_run) then
     write(*, "(' params(1:17) = ',/,(3x,4es24.15))") params(1:17)
     write(*, "(' time(1:nsteps) = ',/,(3x,4es24.15))") time
-    write(*, '(a)') ' Dry run: stopping before the integration.'
+    ! the inputs of the integration, exactly: state(1:6,1,:) and params(1:17)
+    call write_f64('!initial_state.fits', 2, [6_c_int64_t, int(nparticles, c_int64_t)], &
+                   state(1:6, 1, :), 'TEMP', temp, 'Initial thermal temperature')
+    call write_f64('!params.fits', 1, [17_c_int64_t], params(1:17), 'TUNIT', tunit, &
+                   'Time unit in seconds')
+    write(*, '(a)') ' Dry run: initial_state.fits and params.fits written; stopping before' // &
+                    ' the integration.'
   else
     call integrate_and_store()
   end if

@@ -100,3 +100,107 @@ def test_errors(tmp_path):
     open(p, "wb").write(b"NOT FITS" * 400)
     with pytest.raises(fits.FitsError):
         fits.read(p)
+
+
+# ---- interoperability with CFITSIO, the library the reference program reads
+# and writes its files with (sources/fitsio.F90:188-191,248-265,300-308) ----
+
+def _cfitsio():
+    import ctypes as C
+    for name in ("/opt/conda/lib/libcfitsio.so", "libcfitsio.so", "libcfitsio.so.10",
+                 "libcfitsio.so.9", "libcfitsio.so.8"):
+        try:
+            lib = C.CDLL(name)
+        except OSError:
+            continue
+        ll = C.POINTER(C.c_longlong)
+        pi = C.POINTER(C.c_int)
+        lib.ffopen.argtypes = [C.POINTER(C.c_void_p), C.c_char_p, C.c_int, pi]
+        lib.ffinit.argtypes = [C.POINTER(C.c_void_p), C.c_char_p, pi]
+        lib.ffgidm.argtypes = [C.c_void_p, pi, pi]
+        lib.ffgidt.argtypes = [C.c_void_p, pi, pi]
+        lib.ffgiszll.argtypes = [C.c_void_p, C.c_int, ll, pi]
+        lib.ffgpvd.argtypes = [C.c_void_p, C.c_long, C.c_longlong, C.c_longlong, C.c_double,
+                               C.c_void_p, pi, pi]
+        lib.ffgpvjj.argtypes = [C.c_void_p, C.c_long, C.c_longlong, C.c_longlong, C.c_longlong,
+                                C.c_void_p, pi, pi]
+        lib.ffgkyd.argtypes = [C.c_void_p, C.c_char_p, C.POINTER(C.c_double), C.c_char_p, pi]
+        lib.ffcrimll.argtypes = [C.c_void_p, C.c_int, C.c_int, ll, pi]
+        lib.ffpprd.argtypes = [C.c_void_p, C.c_long, C.c_longlong, C.c_longlong, C.c_void_p, pi]
+        lib.ffpkyd.argtypes = [C.c_void_p, C.c_char_p, C.c_double, C.c_int, C.c_char_p, pi]
+        lib.ffclos.argtypes = [C.c_void_p, pi]
+        return lib
+    pytest.skip("CFITSIO not installed")
+
+
+def _cf_read(lib, path, key=None):
+    """(array in Fortran order, bitpix, key value) read by CFITSIO."""
+    import ctypes as C
+    f, st = C.c_void_p(), C.c_int(0)
+    lib.ffopen(C.byref(f), path.encode(), 0, C.byref(st))
+    assert st.value == 0, f"ffopen status {st.value}"
+    nax, bp = C.c_int(), C.c_int()
+    lib.ffgidm(f, C.byref(nax), C.byref(st))
+    lib.ffgidt(f, C.byref(bp), C.byref(st))
+    dims = (C.c_longlong * nax.value)()
+    lib.ffgiszll(f, nax.value, dims, C.byref(st))
+    shape = tuple(int(d) for d in dims)
+    n = int(np.prod(shape))
+    anynul = C.c_int()
+    if bp.value == 64:
+        out = np.zeros(n, dtype=np.int64)
+        lib.ffgpvjj(f, 0, 1, n, 0, out.ctypes.data, C.byref(anynul), C.byref(st))
+    else:
+        out = np.zeros(n, dtype=np.float64)
+        lib.ffgpvd(f, 0, 1, n, 0.0, out.ctypes.data, C.byref(anynul), C.byref(st))
+    val = None
+    if key is not None:
+        v = C.c_double()
+        com = C.create_string_buffer(81)
+        lib.ffgkyd(f, key.encode(), C.byref(v), com, C.byref(st))
+        val = v.value
+    lib.ffclos(f, C.byref(st))
+    assert st.value == 0, f"CFITSIO status {st.value}"
+    return out.reshape(shape, order="F"), bp.value, val
+
+
+def test_cfitsio_reads_what_the_library_writes(tmp_path):
+    """counter (int64 3 x np), time (FP64, TUNIT) and state (FP64 8 x m x np,
+    TEMP) files written by the library, read back by CFITSIO as the
+    reference's users read them."""
+    lib = _cfitsio()
+    rng = np.random.default_rng(3)
+    state = np.asfortranarray(rng.normal(size=(8, 5, 7)))
+    counter = np.asfortranarray(rng.integers(-2 ** 62, 2 ** 62, size=(3, 7)))
+    time = np.linspace(0, 1, 5)
+    ps, pc, pt = (str(tmp_path / n) for n in ("state.fits", "counter.fits", "time.fits"))
+    fits.write_f64("!" + ps, state, {"TEMP": 1.0e3}, {"TEMP": "Initial thermal temperature"})
+    fits.write_i64("!" + pc, counter)
+    fits.write_f64("!" + pt, time, {"TUNIT": 2.5e-6}, {"TUNIT": "Time unit in seconds"})
+    s, bp, temp = _cf_read(lib, ps, "TEMP")
+    assert bp == -64 and s.shape == state.shape and np.array_equal(s, state) and temp == 1.0e3
+    c, bp, _ = _cf_read(lib, pc)
+    assert bp == 64 and c.shape == counter.shape and np.array_equal(c, counter)
+    t, bp, tu = _cf_read(lib, pt, "TUNIT")
+    assert bp == -64 and np.array_equal(t, time) and tu == 2.5e-6
+
+
+def test_library_reads_what_cfitsio_writes(tmp_path):
+    """A 3-D FP64 field file written by CFITSIO (as a simulation code would
+    write velx.fits ...) read by the library: identical values and shape."""
+    import ctypes as C
+    lib = _cfitsio()
+    a = np.asfortranarray(np.random.default_rng(4).normal(size=(6, 5, 4)))
+    p = str(tmp_path / "magx.fits")
+    f, st = C.c_void_p(), C.c_int(0)
+    lib.ffinit(C.byref(f), p.encode(), C.byref(st))
+    dims = (C.c_longlong * 3)(*a.shape)
+    lib.ffcrimll(f, -64, 3, dims, C.byref(st))
+    flat = np.ascontiguousarray(a.ravel(order="F"))
+    lib.ffpprd(f, 0, 1, flat.size, flat.ctypes.data, C.byref(st))
+    lib.ffpkyd(f, b"BUNIT", 1.5, 10, b"field unit", C.byref(st))
+    lib.ffclos(f, C.byref(st))
+    assert st.value == 0
+    assert fits.dims(p) == (a.shape, -64)
+    assert np.array_equal(fits.read(p), a)
+    assert float(fits.header(p)["BUNIT"]) == 1.5

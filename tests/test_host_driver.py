@@ -2,9 +2,12 @@
 reference program's parameter file, unit algebra, time grid and output
 files around the device integrator.  CPU: a dry run (no GPU) checks the
 parsed parameters and derived quantities against the reference's formulas
-(sources/particles.F90:163-214,240-280,317-333).  GPU: a full run writes
-counter.fits, time.fits and state.fits, and its snapshots equal the device
-integrator's on the initial states it wrote; with field_snapshots = 3 (a
+(sources/particles.F90:163-214,240-280,317-333), and the initial state it
+writes equals the reference program's own draw (its random module, compiled
+from the reference sources into oracle/_ref/ref_init).  GPU: a full run
+writes counter.fits, time.fits and state.fits, and its snapshots equal the
+device integrator's and the reference's compiled integrator's on the initial
+states it wrote; with field_snapshots = 3 (a
 time-dependent field, no counterpart in the reference) they equal the
 library's series integration of the same snapshots."""
 import os
@@ -102,6 +105,41 @@ def test_dry_run_parameters_and_units(tmp_path):
     assert re.search(r"momz:\s+[-+0-9.E]+\s+[0-9.E+-]+", out)
 
 
+REF_INIT = os.path.join(ROOT, "oracle", "_ref", "ref_init")
+
+
+@pytest.mark.parametrize("ptype,npart,temp", [("electron", 1000, 5.0e4), ("proton", 64, 1.0e6)])
+def test_dry_run_initial_state_is_the_reference_programs(tmp_path, ptype, npart, temp):
+    """initial_state.fits of a dry run equals, bit for bit, the state the
+    reference program draws with its own random module (sources/random.F90
+    uniform and normal, called as in particles.F90:289-296) in a fresh
+    process, and params.fits holds the printed params(1:17).  The region is
+    the reference's clamp of the parameter file's bounds to the domain
+    (particles.F90:240-280): x in [0, 0.5], y in [0, 1], z in [0, 0.25].
+    (The reference's normal() fills an even count, so npart is even.)"""
+    if not os.path.exists(REF_INIT):
+        pytest.skip("reference random module (oracle/_ref/ref_init) not built")
+    tmp = str(tmp_path)
+    text = PARAMS.replace("nparticles  = 9", f"nparticles  = {npart}").replace(
+        "particle_type = electron", f"particle_type = {ptype}").replace(
+        "temperature = 5.0e4", f"temperature = {temp:.6e}")
+    _setup(tmp, text)
+    r = _run(tmp, "--dry-run")
+    assert r.returncode == 0, r.stdout + r.stderr
+    path = os.path.join(tmp, "initial_state.fits")
+    state = fits.read(path)
+    assert state.shape == (6, npart) and float(fits.header(path)["TEMP"]) == temp
+    ref = os.path.join(tmp, "ref_init.bin")
+    subprocess.run([REF_INIT, str(npart), "0", "0.5", "0", "1", "0", "0.25", repr(temp), ref],
+                   check=True, timeout=60)
+    expect = np.fromfile(ref, dtype=np.float64).reshape(npart, 6).T
+    assert np.array_equal(state, expect)
+    params = fits.read(os.path.join(tmp, "params.fits"))
+    # printed with 16 significant digits
+    assert np.allclose(params, _floats(r.stdout, "params(1:17)"), rtol=1e-15, atol=0)
+    assert float(fits.header(os.path.join(tmp, "params.fits"))["TUNIT"]) == 2.0e-6
+
+
 def test_dry_run_errors(tmp_path):
     tmp = str(tmp_path)
     _setup(tmp, PARAMS.replace("'dop853'", "'euler'"))
@@ -170,6 +208,13 @@ def test_full_run_writes_reference_files(tmp_path):
     assert (status == 0).all()
     assert np.array_equal(st[:, 1:, :], state[:, 1:, :])
     assert np.array_equal(cnt.astype(np.float64), counter)
+    # and through the reference's own integrator (its modules compiled from
+    # the reference sources, CPU) on the same initial state and parameters
+    from oracle.oracle import REF_SO, Reference
+    if os.path.exists(REF_SO):
+        sr, cr = Reference().integrate("dop853", vunit * uu, 1.0 * bb, p, time, st0)
+        assert np.array_equal(sr[:, 1:, :], state[:, 1:, :])
+        assert np.array_equal(cr.astype(np.float64), counter)
 
 
 # ---- field snapshot series (field_snapshots / field_dt) ----
