@@ -196,11 +196,20 @@ int ptc_particles_generate(ptc_particles *p, int kind, uint64_t seed, const doub
  * prints, random.F90:231-353), reduced on the device: moments[4 c + k]. */
 int ptc_particles_moments(ptc_particles *p, double moments[24]);
 /* n advances in one HIP graph: the launches for the stop times t_stop[0..n-1]
- * (typically the snapshot times) are captured once and submitted together;
- * the same result as n calls of ptc_particles_advance.  The work-queue order
- * is computed once, before the graph, and kept for all n intervals.
- * ptc_particles_last_kernel_ms then reports the whole graph. */
+ * (typically the snapshot times) are captured and submitted together; the
+ * same result as n calls of ptc_particles_advance.  The work-queue order is
+ * computed once per call, before the graph, and kept for all n intervals.
+ * The graph is instantiated once per batch and kept: a later call with the
+ * same n and the same launch settings (options, step-size controller, queue
+ * order on or off) replays it with its stop-time arguments replaced
+ * (kernel-node parameter update); any other call captures it anew.
+ * ptc_particles_last_kernel_ms then reports the whole graph.  (The reference
+ * has no counterpart: it integrates every particle through all snapshots
+ * in one loop, sources/particles.F90:337-356.) */
 int ptc_particles_advance_graph(ptc_particles *p, int n, const double *t_stop);
+/* stats[0]: graphs instantiated by ptc_particles_advance_graph for this
+ * batch; stats[1]: calls that replayed the kept graph with new stop times. */
+int ptc_particles_graph_stats(const ptc_particles *p, int64_t stats[2]);
 /* Autotuner: integrates the batch up to t_end in 1 + 7 equal segments, the
  * first to warm up, then one per candidate launch setting (work-queue order
  * by cost at 256 threads per block with one lane per particle, with the

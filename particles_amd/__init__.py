@@ -82,6 +82,7 @@ def _load():
         "ptc_particles_moments": ([vp, vp], i32),
         "ptc_particles_autotune": ([vp, dbl, vp], i32),
         "ptc_particles_advance_graph": ([vp, i32, vp], i32),
+        "ptc_particles_graph_stats": ([vp, vp], i32),
         "ptc_particles_record": ([vp, dbl, i64], i32),
         "ptc_particles_records": ([vp, C.POINTER(i64), vp, C.POINTER(vp)], i32),
         "ptc_particles_use_mlp": ([vp, vp, dbl], i32),
@@ -357,6 +358,13 @@ class Particles:
         ms = C.c_double()
         _check(_lib.ptc_particles_last_kernel_ms(self._h, C.byref(ms)))
         return ms.value
+
+    def graph_stats(self) -> dict:
+        """Graphs instantiated by advance_graph for this batch, and calls
+        that replayed the kept graph with new stop times."""
+        st = np.zeros(2, dtype=np.int64)
+        _check(_lib.ptc_particles_graph_stats(self._h, _ptr(st)))
+        return dict(instantiated=int(st[0]), replayed=int(st[1]))
 
     def autotune(self, t_end: float) -> dict:
         """Integrates to t_end while timing candidate launch settings, keeps

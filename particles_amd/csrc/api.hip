@@ -40,6 +40,25 @@ struct ptc_particles {
     int64_t advances;     // advance calls so far (sort cadence)
     int tuned;            // launch settings below chosen by ptc_particles_autotune
     int tuned_block, tuned_sort, tuned_order, tuned_tail;
+    // the graph of ptc_particles_advance_graph, kept for the batch's life and
+    // replayed with new stop times while everything else its launches depend
+    // on (the key below) is unchanged
+    struct AdvanceGraph {
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        int n = 0;                          // intervals
+        LaunchConfig lc{};
+        int64_t max_steps = 0;
+        const int32_t *order = nullptr;
+        int ctrl = 0;
+        const void *net = nullptr;
+        float shift = 0.f;
+        uint64_t rec_thresh = 0;
+        std::vector<hipGraphNode_t> nodes;  // the integrator kernel nodes
+        std::vector<int> interval;          // the stop time each one takes
+        bool replayable = false;            // the nodes' arguments checked at capture
+        int64_t instantiated = 0, replayed = 0;
+    } graph;
 };
 
 struct ptc_mlp {
@@ -860,6 +879,112 @@ int ptc_particles_moments(ptc_particles *p, double moments[24])
     return PTC_OK;
 }
 
+namespace {
+
+void drop_graph(ptc_particles::AdvanceGraph &g)
+{
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    if (g.graph) (void)hipGraphDestroy(g.graph);
+    g.exec = nullptr;
+    g.graph = nullptr;
+    g.n = 0;
+    g.nodes.clear();
+    g.interval.clear();
+    g.replayable = false;
+}
+
+// the node a capture on stream s currently ends in (the captured work is one
+// chain, so there is exactly one)
+hipError_t capture_end(hipStream_t s, hipGraphNode_t *node)
+{
+    hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+    unsigned long long id = 0;
+    hipGraph_t graph = nullptr;
+    const hipGraphNode_t *deps = nullptr;
+    size_t nd = 0;
+    const hipError_t e = hipStreamGetCaptureInfo_v2(s, &st, &id, &graph, &deps, &nd);
+    if (e != hipSuccess) return e;
+    if (st != hipStreamCaptureStatusActive || nd != 1 || !deps) return hipErrorInvalidValue;
+    *node = deps[0];
+    return hipSuccess;
+}
+
+// Captures the n intervals (per interval: reset the queue heads with a fill
+// kernel, then the integrator kernels to that interval's stop time) into
+// p->graph, instantiates it, and records each integrator kernel node with its
+// interval: the nodes between the interval's fill node and the capture's end
+// after its launches.
+hipError_t capture_graph(ptc_particles *p, int n, const double *t_stop, const LaunchConfig &lc)
+{
+    ptc_particles::AdvanceGraph &g = p->graph;
+    std::vector<hipGraphNode_t> fill(n), last(n);
+    hipError_t e = hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return e;
+    for (int k = 0; k < n && e == hipSuccess; k++) {
+        e = launch_zero_i32(p->d.qcount, 3, p->stream);
+        if (e == hipSuccess) e = capture_end(p->stream, &fill[k]);
+        if (e == hipSuccess)
+            e = launch_integrate(p->method, p->fields->d, p->sp, p->d, t_stop[k], lc, p->stream);
+        if (e == hipSuccess) e = capture_end(p->stream, &last[k]);
+    }
+    const hipError_t e2 = hipStreamEndCapture(p->stream, &g.graph);
+    if (e == hipSuccess) e = e2;
+    for (int k = 0; k < n && e == hipSuccess; k++) {
+        hipGraphNode_t node = last[k];
+        while (e == hipSuccess && node != fill[k]) {
+            hipGraphNodeType type;
+            e = hipGraphNodeGetType(node, &type);
+            if (e == hipSuccess && type != hipGraphNodeTypeKernel) e = hipErrorInvalidValue;
+            if (e != hipSuccess) break;
+            g.nodes.push_back(node);
+            g.interval.push_back(k);
+            size_t nd = 0;
+            e = hipGraphNodeGetDependencies(node, nullptr, &nd);
+            if (e == hipSuccess && nd != 1) e = hipErrorInvalidValue;
+            if (e == hipSuccess) e = hipGraphNodeGetDependencies(node, &node, &nd);
+        }
+    }
+    if (e == hipSuccess) e = hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0);
+    // replays rewrite argument 3 of these nodes: only where each node holds
+    // one pointer per argument and they are the values just launched
+    g.replayable = e == hipSuccess && !g.nodes.empty();
+    for (size_t j = 0; j < g.nodes.size() && g.replayable; j++) {
+        hipKernelNodeParams kp{};
+        if (hipGraphKernelNodeGetParams(g.nodes[j], &kp) != hipSuccess || !kp.kernelParams ||
+            kp.extra) {
+            g.replayable = false;
+            break;
+        }
+        const DeviceFields *fa = (const DeviceFields *)kp.kernelParams[0];
+        const DeviceParticles *pa = (const DeviceParticles *)kp.kernelParams[2];
+        g.replayable = fa && pa && kp.kernelParams[3] && fa->data == p->fields->d.data &&
+                       pa->np == p->d.np && pa->out == p->d.out && pa->qcount == p->d.qcount &&
+                       std::memcmp(kp.kernelParams[3], &t_stop[g.interval[j]], sizeof(double)) == 0;
+    }
+    return e;
+}
+
+// New stop times into the instantiated graph: every integrator kernel takes
+// (fields, step parameters, particles, stop time); the first three keep the
+// values the node was captured with.
+hipError_t set_graph_stops(ptc_particles::AdvanceGraph &g, const double *t_stop)
+{
+    for (size_t j = 0; j < g.nodes.size(); j++) {
+        hipKernelNodeParams kp{};
+        hipError_t e = hipGraphKernelNodeGetParams(g.nodes[j], &kp);
+        if (e != hipSuccess) return e;
+        if (!kp.kernelParams || kp.extra) return hipErrorNotSupported;
+        double stop = t_stop[g.interval[j]];
+        void *args[4] = {kp.kernelParams[0], kp.kernelParams[1], kp.kernelParams[2], &stop};
+        kp.kernelParams = args;
+        e = hipGraphExecKernelNodeSetParams(g.exec, g.nodes[j], &kp);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+}  // namespace
+
 int ptc_particles_advance_graph(ptc_particles *p, int n, const double *t_stop)
 {
     if (!p || (n > 0 && !t_stop)) return fail(PTC_EARG, "ptc_particles_advance_graph: null argument");
@@ -909,31 +1034,54 @@ int ptc_particles_advance_graph(ptc_particles *p, int n, const double *t_stop)
         p->d.order = nullptr;
     }
     HIP_TRY(hipStreamSynchronize(p->stream));
-    // capture: per interval, reset the queue head (a fill kernel) and run the
-    // integrator to that interval's stop time
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    HIP_TRY(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal));
+    ptc_particles::AdvanceGraph &g = p->graph;
+    const bool same = g.exec && g.replayable && g.n == n && g.lc.block == lc.block &&
+                      g.lc.grid_blocks == lc.grid_blocks && g.lc.cell_cache == lc.cell_cache &&
+                      g.lc.ctrl == lc.ctrl && g.lc.tail == lc.tail &&
+                      g.max_steps == p->sp.max_steps && g.order == p->d.order &&
+                      g.ctrl == p->ctrl && g.net == (const void *)p->d.net &&
+                      g.shift == p->d.ctrl_shift && g.rec_thresh == p->d.rec_thresh;
     hipError_t e = hipSuccess;
-    for (int k = 0; k < n && e == hipSuccess; k++) {
-        e = launch_zero_i32(p->d.qcount, 3, p->stream);
-        if (e == hipSuccess) e = launch_integrate(p->method, p->fields->d, p->sp, p->d, t_stop[k],
-                                                  lc, p->stream);
+    if (same) {
+        // replay: only the stop-time argument of each integrator node changes
+        e = set_graph_stops(g, t_stop);
+        if (e == hipSuccess) g.replayed++;
     }
-    hipError_t e2 = hipStreamEndCapture(p->stream, &graph);
-    if (e == hipSuccess) e = e2;
-    if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (!same || e != hipSuccess) {
+        drop_graph(g);
+        e = capture_graph(p, n, t_stop, lc);
+        if (e == hipSuccess) {
+            g.n = n;
+            g.lc = lc;
+            g.max_steps = p->sp.max_steps;
+            g.order = p->d.order;
+            g.ctrl = p->ctrl;
+            g.net = (const void *)p->d.net;
+            g.shift = p->d.ctrl_shift;
+            g.rec_thresh = p->d.rec_thresh;
+            g.instantiated++;
+        } else {
+            drop_graph(g);
+        }
+    }
     if (e == hipSuccess) e = hipEventRecord(p->ev0, p->stream);
-    if (e == hipSuccess) e = hipGraphLaunch(exec, p->stream);
+    if (e == hipSuccess) e = hipGraphLaunch(g.exec, p->stream);
     if (e == hipSuccess) e = hipEventRecord(p->ev1, p->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(p->stream);
     float ms = 0.f;
     if (e == hipSuccess) e = hipEventElapsedTime(&ms, p->ev0, p->ev1);
-    if (exec) (void)hipGraphExecDestroy(exec);
-    if (graph) (void)hipGraphDestroy(graph);
     if (e != hipSuccess) return hip_fail(e, "ptc_particles_advance_graph");
     p->advances += n;
     p->last_ms = ms;
+    ok();
+    return PTC_OK;
+}
+
+int ptc_particles_graph_stats(const ptc_particles *p, int64_t stats[2])
+{
+    if (!p || !stats) return fail(PTC_EARG, "ptc_particles_graph_stats: null argument");
+    stats[0] = p->graph.instantiated;
+    stats[1] = p->graph.replayed;
     ok();
     return PTC_OK;
 }
@@ -1044,6 +1192,7 @@ int ptc_particles_destroy(ptc_particles *p)
     hipError_t e2 = hipFree(p->block);
     if (p->rec_block) (void)hipFree(p->rec_block);
     if (p->order_block) (void)hipFree(p->order_block);
+    drop_graph(p->graph);
     delete p;
     if (e1 != hipSuccess) return hip_fail(e1, "ptc_particles_destroy");
     if (e2 != hipSuccess) return hip_fail(e2, "ptc_particles_destroy");

@@ -478,7 +478,7 @@ def test_spatial_order_keeps_results():
                 out.append(b.download())
         finally:
             P.set_option("sort", 1)
-            P.set_option("order", "cost")
+            P.set_option("order", "auto")
     for s, c, status in out[1:]:
         assert np.array_equal(s, out[0][0]) and np.array_equal(c, out[0][1])
         assert (status == 0).all()
@@ -524,6 +524,46 @@ def test_graph_advances_equal_eager_advances():
             got = b.download()
     assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1])
     assert (got[2] == 0).all()
+
+
+@pytest.mark.parametrize("tail,sort", [("0", 1), ("group", 0), ("600", 1)])
+def test_graph_is_kept_and_replayed_with_new_stop_times(tail, sort):
+    """advance_graph instantiates its graph once per batch and replays it
+    with the stop times replaced while n and the launch settings stay; a
+    change of either captures a new one.  After every call the batch equals,
+    bit for bit, one advanced eagerly through the same stop times (eager
+    downloads and synchronisation between the replays)."""
+    uu, bb = W.turbulent_vb(16, seed=2, nmodes=32)
+    p = W.params_vector(tmax=2.0 * W.T0, dtmax=1.0, atol=1e-8, rtol=1e-8)
+    tg = W.time_grid(10, p[2])
+    st0 = W.initial_state(3000, len(tg), seed=8, gamma_range=(1.0, 100.0))
+    calls = [tg[1:3], tg[3:5], tg[5:7], tg[7:8], tg[8:]]
+    # instantiated / replayed after each call: n = 2, 2, 2, then 1, 2
+    want = [(1, 0), (1, 1), (1, 2), (2, 2), (3, 2)]
+    P.set_option("tail", tail)
+    P.set_option("sort", sort)
+    try:
+        with P.Fields(uu, bb) as f, P.Particles(f, "dop853", p, tg, st0) as eager, \
+                P.Particles(f, "dop853", p, tg, st0) as b:
+            for k, stops in enumerate(calls):
+                for t in stops:
+                    eager.advance(t)
+                b.advance_graph(stops)
+                ref, got = eager.download(), b.download()
+                st = b.graph_stats()
+                say(f"[graph tail {tail} sort {sort}] call {k}: {len(stops)} intervals, {st}")
+                assert (st["instantiated"], st["replayed"]) == want[k]
+                assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1])
+                assert np.array_equal(got[2], ref[2])
+            assert (got[2] == 0).all() and got[1][1].sum() > 0
+            # another setting captures anew
+            P.set_option("block", 128)
+            b.advance_graph(tg[-1:])
+            assert b.graph_stats()["instantiated"] == 4
+    finally:
+        P.set_option("tail", "auto")
+        P.set_option("sort", 1)
+        P.set_option("block", 256)
 
 
 @pytest.fixture(params=["0", "group", "600", "100"])

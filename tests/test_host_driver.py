@@ -215,6 +215,7 @@ def test_full_run_writes_reference_files(tmp_path):
         sr, cr = Reference().integrate("dop853", vunit * uu, 1.0 * bb, p, time, st0)
         assert np.array_equal(sr[:, 1:, :], state[:, 1:, :])
         assert np.array_equal(cr.astype(np.float64), counter)
+        print("[driver] state.fits and counter.fits equal the reference integrator's", flush=True)
 
 
 # ---- field snapshot series (field_snapshots / field_dt) ----

@@ -424,7 +424,7 @@ def test_series_graph_segments_and_queue_orders_keep_results():
             P.set_option("sort", "0")  # index order, one call
             ref, rc, _ = P.integrate(f, "dop853", p, tg, st0)
             P.set_option("sort", "1")
-            P.set_option("order", "cost")
+            P.set_option("order", "auto")
             with P.Particles(f, "dop853", p, tg, st0) as b:
                 b.advance_graph(tg[1:])
                 b.advance()
