@@ -255,41 +255,53 @@ struct GroupForce {
         }
     }
 
-    // the same from pair rows (ROWS_PAIRS)
+    // the same from pair rows (ROWS_PAIRS): the x-left corner rows hold the
+    // lane's component of the corner and of its +x neighbour side by side
+    // (one 8- or 16-byte load per row); those are the x-right corners when
+    // the cell's right x index is the left one's periodic successor, else
+    // the x-right corners come from their own rows
     __device__ __forceinline__ void load_pairs(const uint32_t *o, K *dst) const
     {
-        const K *__restrict__ d = (const K *)F.data + comp;
+        using K2 = std::conditional_t<PREC == PTC_FP32, float2, double2>;
+        const K2 *__restrict__ d2 = (const K2 *)F.data + comp;
         // x1 - x0 (mod 2^32) from the x-right and x-left offsets
         const uint32_t dx = o[4] - o[0];
         const bool succ = dx == 1u || dx == (uint32_t)(1 - P.dm[0]);
+        K2 p[4];
 #pragma unroll
-        for (int j = 0; j < 4; j++) dst[j] = d[(size_t)o[j] * 16];
-        if (succ) {
+        for (int j = 0; j < 4; j++) p[j] = d2[(size_t)o[j] * 8];
 #pragma unroll
-            for (int j = 0; j < 4; j++) dst[4 + j] = d[(size_t)o[j] * 16 + 6];
-        } else {
+        for (int j = 0; j < 4; j++) {
+            dst[j] = p[j].x;
+            dst[4 + j] = p[j].y;
+        }
+        if (!succ) {
+            const K *__restrict__ d = (const K *)F.data + 2 * comp;
 #pragma unroll
             for (int j = 0; j < 4; j++) dst[4 + j] = d[(size_t)o[4 + j] * 16];
         }
     }
 
-    // the same from quad rows (ROWS_QUADS): corner j sits in the row of
-    // corner j & 1 (its z plane's x-left, y-left corner) at slot 6 x + 12 y
-    // (x, y: its right flags) when the x and y right indices are the left
-    // ones' periodic successors, else at slot 0 of its own row
+    // the same from quad rows (ROWS_QUADS): the (x-left, y-left) corner row
+    // of each z plane holds the lane's component of its four corners side by
+    // side (one 16-byte load per plane), corner (x, y) at x + 2 y, when the
+    // right x and y indices are the left ones' periodic successors; else
+    // every corner comes from its own row
     __device__ __forceinline__ void load_quads(const uint32_t *o, K *dst) const
     {
-        const K *__restrict__ d = (const K *)F.data + comp;
         const uint32_t dx = o[4] - o[0], dy = o[2] - o[0];
         const bool xs = dx == 1u || dx == (uint32_t)(1 - P.dm[0]);
         const bool ys = dy == (uint32_t)P.dm[0] ||
                         dy == (uint32_t)((int64_t)P.dm[0] * (1 - P.dm[1]));
-        const bool quad = xs && ys;
+        const float4 *__restrict__ d4 = (const float4 *)F.data + comp;
+        // corner j: x = j >> 2, y = (j >> 1) & 1, z = j & 1
+        const float4 q0 = d4[(size_t)o[0] * 8], q1 = d4[(size_t)o[1] * 8];
+        dst[0] = q0.x; dst[1] = q1.x; dst[2] = q0.z; dst[3] = q1.z;
+        dst[4] = q0.y; dst[5] = q1.y; dst[6] = q0.w; dst[7] = q1.w;
+        if (!(xs && ys)) {
+            const float *__restrict__ d = (const float *)F.data + 4 * comp;
 #pragma unroll
-        for (int j = 0; j < 8; j++) {
-            const uint32_t row = quad ? o[j & 1] : o[j];
-            const int slot = quad ? 6 * (j >> 2) + 12 * ((j >> 1) & 1) : 0;
-            dst[j] = d[(size_t)row * 32 + slot];
+            for (int j = 2; j < 8; j++) dst[j] = d[(size_t)o[j] * 32];
         }
     }
 

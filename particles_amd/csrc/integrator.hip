@@ -131,8 +131,9 @@ __global__ void to_nodes_f32_kernel(const double *__restrict__ soa, int64_t node
     }
 }
 
-// SOA planes (FP64) -> FP32 pair rows: node n's six values, then those of
-// its +x neighbour (periodic in x), then four zeros
+// SOA planes (FP64) -> FP32 pair rows (64 bytes): value c of node n at
+// position 2 c, of its +x neighbour (periodic in x) at 2 c + 1, then four
+// zeros
 __global__ void to_pairs_f32_kernel(const double *__restrict__ soa, int dm0, int64_t nodes,
                                     float *__restrict__ out)
 {
@@ -140,21 +141,20 @@ __global__ void to_pairs_f32_kernel(const double *__restrict__ soa, int dm0, int
          n += (int64_t)gridDim.x * blockDim.x) {
         const int i = (int)(n % dm0);
         const int64_t nx = n - i + (i + 1 == dm0 ? 0 : i + 1);
-        float4 r[4];
-        float *v = (float *)r;
+        float *__restrict__ row = out + 16 * n;
 #pragma unroll
         for (int c = 0; c < 6; c++) {
-            v[c] = float(soa[c * nodes + n]);
-            v[6 + c] = float(soa[c * nodes + nx]);
+            row[2 * c] = float(soa[c * nodes + n]);
+            row[2 * c + 1] = float(soa[c * nodes + nx]);
         }
-        v[12] = v[13] = v[14] = v[15] = 0.0f;
 #pragma unroll
-        for (int k = 0; k < 4; k++) ((float4 *)out)[4 * n + k] = r[k];
+        for (int c = 12; c < 16; c++) row[c] = 0.0f;
     }
 }
 
-// SOA planes (FP64) -> FP64 pair rows: node n's six values, then those of
-// its +x neighbour (periodic in x), then four zeros (128 bytes)
+// SOA planes (FP64) -> FP64 pair rows (128 bytes): value c of node n at
+// position 2 c, of its +x neighbour (periodic in x) at 2 c + 1, then four
+// zeros
 __global__ void to_pairs_f64_kernel(const double *__restrict__ soa, int dm0, int64_t nodes,
                                     double *__restrict__ out)
 {
@@ -165,17 +165,17 @@ __global__ void to_pairs_f64_kernel(const double *__restrict__ soa, int dm0, int
         double *__restrict__ row = out + 16 * n;
 #pragma unroll
         for (int c = 0; c < 6; c++) {
-            row[c] = soa[c * nodes + n];
-            row[6 + c] = soa[c * nodes + nx];
+            row[2 * c] = soa[c * nodes + n];
+            row[2 * c + 1] = soa[c * nodes + nx];
         }
 #pragma unroll
         for (int c = 12; c < 16; c++) row[c] = 0.0;
     }
 }
 
-// SOA planes (FP64) -> FP32 quad rows: the six values of node n = (i, j,
-// k), of (i+1, j, k), (i, j+1, k) and (i+1, j+1, k) (periodic), then eight
-// zeros (128 bytes)
+// SOA planes (FP64) -> FP32 quad rows (128 bytes): value c of node a of
+// n = (i, j, k), (i+1, j, k), (i, j+1, k), (i+1, j+1, k) (periodic), a = 0..3,
+// at position 4 c + a, then eight zeros
 __global__ void to_quads_f32_kernel(const double *__restrict__ soa, int dm0, int dm1,
                                     int64_t nodes, float *__restrict__ out)
 {
@@ -191,7 +191,7 @@ __global__ void to_quads_f32_kernel(const double *__restrict__ soa, int dm0, int
 #pragma unroll
         for (int a = 0; a < 4; a++)
 #pragma unroll
-            for (int c = 0; c < 6; c++) row[6 * a + c] = float(soa[c * nodes + q[a]]);
+            for (int c = 0; c < 6; c++) row[4 * c + a] = float(soa[c * nodes + q[a]]);
 #pragma unroll
         for (int c = 24; c < 32; c++) row[c] = 0.0f;
     }
@@ -201,17 +201,18 @@ __global__ void to_quads_f32_kernel(const double *__restrict__ soa, int dm0, int
 __device__ __forceinline__ double stored_value(const DeviceFields &F, int64_t n, int comp)
 {
     const int64_t rv = row_values(F.layout, F.prec);
-    if (F.prec == PTC_FP32) return double(((const float *)F.data)[n * rv + comp]);
+    const int pos = row_nodes(F.layout) * comp;
+    if (F.prec == PTC_FP32) return double(((const float *)F.data)[n * rv + pos]);
     if (F.layout == LAYOUT_SOA) return ((const double *)F.data)[comp * F.nodes + n];
-    return ((const double *)F.data)[n * rv + comp];
+    return ((const double *)F.data)[n * rv + pos];
 }
 
 // Rows of a new static field (layout, prec) from a stored one (any layout
 // and precision), node by node: node rows (FP64: six values; FP32: six, |B|
 // from the stored values in FP64, 0), pair rows (the node, its +x
-// neighbour, zeros) or FP32 quad rows (the node, +x, +y, +x+y, zeros), the
-// neighbours periodic.  The same values as building the new form from the
-// stored field's planes.
+// neighbour, zeros) or FP32 quad rows (the node, +x, +y, +x+y, zeros; value
+// c of the row's node a at row_nodes * c + a), the neighbours periodic.  The
+// same values as building the new form from the stored field's planes.
 __global__ void convert_rows_kernel(const DeviceFields F, void *__restrict__ out, int layout,
                                     int prec)
 {
@@ -226,14 +227,14 @@ __global__ void convert_rows_kernel(const DeviceFields F, void *__restrict__ out
         const int i1 = i + 1 == dm0 ? 0 : i + 1, j1 = j + 1 == dm1 ? 0 : j + 1;
         const int64_t q[4] = {n, base + i1 + (int64_t)j * dm0, base + i + (int64_t)j1 * dm0,
                               base + i1 + (int64_t)j1 * dm0};
-        const int nq = layout == LAYOUT_QUADS ? 4 : layout == LAYOUT_PAIRS ? 2 : 1;
+        const int nq = row_nodes(layout);
         for (int a = 0; a < 4; a++) {
             if (a >= nq) break;
 #pragma unroll
             for (int c = 0; c < 6; c++) {
                 const double v = stored_value(F, q[a], c);
-                if (prec == PTC_FP32) ((float *)out)[n * rv + 6 * a + c] = float(v);
-                else ((double *)out)[n * rv + 6 * a + c] = v;
+                if (prec == PTC_FP32) ((float *)out)[n * rv + nq * c + a] = float(v);
+                else ((double *)out)[n * rv + nq * c + a] = v;
             }
         }
         if (prec == PTC_FP32 && layout == LAYOUT_AOS) {
@@ -252,22 +253,23 @@ __global__ void convert_rows_kernel(const DeviceFields F, void *__restrict__ out
 
 // One component of a stored field back to a plane of doubles (the
 // reference's uu(:,:,:,c) / bb(:,:,:,c) order): from the six SOA planes,
-// FP64 node rows, FP32 node rows (eight floats) or FP32 pair rows (sixteen
-// floats, the node's own six first), FP32 quad rows (32 floats) or FP64
-// pair rows (16 doubles).
+// FP64 node rows, FP32 node rows (eight floats), FP32 pair rows (sixteen
+// floats), FP32 quad rows (32 floats) or FP64 pair rows (16 doubles); in
+// pair and quad rows the row's own node's value comp at row_nodes * comp.
 __global__ void extract_plane_kernel(const void *__restrict__ data, int layout, int prec,
                                      int64_t nodes, int comp, double *__restrict__ out)
 {
     const int64_t rv = row_values(layout, prec);
+    const int pos = row_nodes(layout) * comp;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nodes;
          i += (int64_t)gridDim.x * blockDim.x) {
         double v;
         if (prec == PTC_FP32)
-            v = double(((const float *)data)[i * rv + comp]);
+            v = double(((const float *)data)[i * rv + pos]);
         else if (layout == LAYOUT_SOA)
             v = ((const double *)data)[comp * nodes + i];
         else
-            v = ((const double *)data)[i * rv + comp];
+            v = ((const double *)data)[i * rv + pos];
         out[i] = v;
     }
 }

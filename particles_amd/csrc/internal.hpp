@@ -145,6 +145,15 @@ constexpr int row_values(int layout, int prec)
 {
     return layout == LAYOUT_QUADS ? 32 : layout == LAYOUT_PAIRS ? 16 : prec == 1 ? 8 : 6;
 }
+// nodes per row: a pair row holds node n and its +x neighbour, a quad row
+// node n and its +x, +y and +x+y neighbours; value c of the row's node a
+// (a < row_nodes) sits at position row_nodes * c + a, so the values of one
+// component are adjacent (one vector load per row and component), and the
+// row's own node's value c at row_nodes * c
+constexpr int row_nodes(int layout)
+{
+    return layout == LAYOUT_QUADS ? 4 : layout == LAYOUT_PAIRS ? 2 : 1;
+}
 // AOS nodes -> per-cell corner bricks.
 hipError_t launch_to_bricks(const DeviceFields &f, hipStream_t stream);
 // Rows of a static field in another layout (node, pair or quad rows) and
