@@ -15,7 +15,8 @@
 // its lanes (per-particle work such as the cell lookup and the divisions by
 // gamma is split over the group, not repeated), so those last trajectories
 // finish sooner, and with under 256 registers per lane two waves share a
-// SIMD (three for FP32 static fields, group_waves), which hides the latency
+// SIMD (three for FP32 static fields and FP64 pair rows, group_waves), which
+// hides the latency
 // of gathers from fields too large for the caches.  The lane-per-particle kernel hands its particles over once the
 // queue is exhausted and few are left in flight (DeviceParticles::park);
 // this kernel then finishes them.  It can also run a whole launch.
@@ -561,13 +562,18 @@ __device__ __forceinline__ void group_features(const GF &force, const StepParams
 }
 
 // Waves per SIMD the kernel is compiled for.  FP32 static fields with the
-// PI controller: three (168 registers, 0-20 B of spills), 1.04-1.24x the
-// accepted steps per second of two (1024^3 DOP853 +15 %, 512^3 RK5 +22 %);
-// the other forms need 180-230 registers and run two (forced to three they
-// spill 60-250 B per lane; FP64 then measured within 1.3 % of two).
-constexpr int group_waves(int prec, int time, int ctrl)
+// PI controller: three (168 registers, 0-28 B of spills), 1.04-1.24x the
+// accepted steps per second of two (1024^3 DOP853 +15 %, 512^3 RK5 +22 %).
+// FP64 static fields in pair rows with the PI controller: three as well
+// (168 registers; RK5 spills 68 B, DOP853 116 B, mostly loop-invariant
+// constants reloaded once per attempt), 6.5 % faster than two at 1024^3
+// (DOP853, 4,000,000 particles).  The other forms need 180-230 registers
+// and run two (FP64 node rows forced to three measured within 1.3 % of two;
+// keeping the FP64 cell cache in LDS instead of registers, to cut the
+// spills, was 5 % slower).
+constexpr int group_waves(int prec, int time, int ctrl, int rows)
 {
-    return prec == PTC_FP32 && time == 0 && ctrl == CTRL_PI ? 3 : 1;
+    return time == 0 && ctrl == CTRL_PI && (prec == PTC_FP32 || rows == ROWS_PAIRS) ? 3 : 1;
 }
 
 // The same particle loop as integrate_kernel (static field or snapshot
@@ -581,7 +587,7 @@ constexpr int group_waves(int prec, int time, int ctrl)
 // buffer per wave in dynamic LDS).
 template <int METHOD, int MODE, int GEOM, int PREC, int TIME, int CTRL = CTRL_PI,
           int ROWS = ROWS_NODES>
-__global__ void __launch_bounds__(kBlock, group_waves(PREC, TIME, CTRL))
+__global__ void __launch_bounds__(kBlock, group_waves(PREC, TIME, CTRL, ROWS))
 integrate_group_kernel(const DeviceFields F, const StepParams P, const DeviceParticles S,
                        double t_stop)
 {
