@@ -39,72 +39,88 @@ constexpr int kGroupsPerWave = 64 / kGroup;
 template <class T>
 __device__ __forceinline__ T gsh(T x, int base, int j) { return __shfl(x, base + j); }
 
+// The group lanes that lane c (0..5) of a group reads for its component of
+// the Lorentz force (accelerations.F90:84-92).  Component c + 1 of the
+// force is, for c >= 3, q/m (E_k + (v_a B_d - v_d B_a)) in EB mode and
+// q/m (w_a B_d - w_d B_a), w = v - U, in VB mode, with k = c - 3,
+// a = (k + 1) % 3, d = (k + 2) % 3; for c < 3 it is v_c / vun.  Lanes k
+// and k + 3 both hold v_k (the shared divisions) and w_k, and lane k + 3
+// holds B_k, so lane c reads
+//   own: lane k (its axis, for the stage point's position and E_k or U_k),
+//   ra:  lane 3 + a (v_a, w_a, B_a), itself for c < 3 (v_c),
+//   rd:  lane 3 + d (v_d, w_d, B_d),
+// each lane exactly the values its own term uses, from three lanes fixed
+// for the whole kernel.
+struct ForceLanes {
+    int own, ra, rd;
+    __device__ __forceinline__ ForceLanes(int base, int c)
+    {
+        const int k = c < 3 ? c : c - 3;
+        own = base + k;
+        ra = c < 3 ? base + c : base + 3 + (k + 1) % 3;
+        rd = base + 3 + (k + 2) % 3;
+    }
+};
+
 // Component c of the Lorentz force (accelerations.F90:59-96, the operations
-// of force_from_fields) from the momentum (u0, u1, u2) of the stage point and
-// the interpolated fields (U or E: f0..f2, B: f3..f5), known on every lane.
-// The operands are scalars selected per lane: selected from an array, they
-// would be indexed per lane and go through scratch.
-// The three divisions v = u / gamma are shared: lane c (and c + 3) of the
-// group divides component c % 3, and the quotients are shuffled.
+// of force_from_fields) from the momentum (u0, u1, u2) of the stage point,
+// known on every lane, and the lane's component fc of the interpolated
+// fields (U or E for c < 3, B for c >= 3).  The three divisions v = u /
+// gamma are shared: lane c (and c + 3) of the group divides component
+// c % 3, and each lane takes the quotients and field components its own
+// term uses from the lanes ForceLanes names.
 template <int MODE, int GEOM>
 __device__ __forceinline__ double force_component(const StepParams &P, double u0, double u1,
-                                                  double u2, double f0, double f1, double f2,
-                                                  double f3, double f4, double f5, int c,
-                                                  int base)
+                                                  double u2, double fc, int c,
+                                                  const ForceLanes &L)
 {
     const double g = sqrt(u0 * u0 + u1 * u1 + u2 * u2 + 1.0);
     const int c3 = c < 3 ? c : c - 3;
     const double q = (c3 == 0 ? u0 : c3 == 1 ? u1 : u2) / g;
-    const double v0 = gsh(q, base, 0), v1 = gsh(q, base, 1), v2 = gsh(q, base, 2);
-    const double vc = c == 0 ? v0 : c == 1 ? v1 : v2;
-    const double lin = GEOM ? vc * P.rvun : vc / P.vun;
-    const bool is3 = c == 3, is4 = c == 4;
-    const double b = is3 ? f5 : is4 ? f3 : f4;
-    const double e = is3 ? f4 : is4 ? f5 : f3;
+    const double va = __shfl(q, L.ra), vd = __shfl(q, L.rd);
+    const double lin = GEOM ? va * P.rvun : va / P.vun;
+    const double bd = __shfl(fc, L.rd), ba = __shfl(fc, L.ra);
     double rot;
     if constexpr (MODE == PTC_MODE_EB) {
         // E + v x B
-        const double a = is3 ? v1 : is4 ? v2 : v0;
-        const double d = is3 ? v2 : is4 ? v0 : v1;
-        const double ec = is3 ? f0 : is4 ? f1 : f2;
-        rot = P.qom * (ec + (a * b - d * e));
+        const double ek = __shfl(fc, L.own);
+        rot = P.qom * (ek + (va * bd - vd * ba));
     } else {
-        // (v - U) x B
-        const double w0 = v0 - f0, w1 = v1 - f1, w2 = v2 - f2;
-        const double a = is3 ? w1 : is4 ? w2 : w0;
-        const double d = is3 ? w2 : is4 ? w0 : w1;
-        rot = P.qom * (a * b - d * e);
+        // (v - U) x B: w_k on lanes k and k + 3
+        const double w = q - __shfl(fc, L.own);
+        const double wa = __shfl(w, L.ra), wd = __shfl(w, L.rd);
+        rot = P.qom * (wa * bd - wd * ba);
     }
     return c < 3 ? lin : rot;
 }
 
 // Mixed precision: component c of the force in FP32 (the operations of
-// force_from_fields_f32: FP32 momentum, hardware reciprocal of gamma).
+// force_from_fields_f32: FP32 momentum, hardware reciprocal square root of
+// gamma squared); every lane forms the three velocity components itself.
 template <int MODE>
 __device__ __forceinline__ float force_component_f32(const StepParams &P, double u0, double u1,
-                                                     double u2, float f0, float f1, float f2,
-                                                     float f3, float f4, float f5, int c)
+                                                     double u2, float fc, int c,
+                                                     const ForceLanes &L)
 {
     const float ux = float(u0), uy = float(u1), uz = float(u2);
-    const float rg = __builtin_amdgcn_rcpf(sqrtf(ux * ux + uy * uy + uz * uz + 1.0f));
+    const float rg = __builtin_amdgcn_rsqf(ux * ux + uy * uy + uz * uz + 1.0f);
     const float v0 = ux * rg, v1 = uy * rg, v2 = uz * rg;
     const float rvun = float(P.rvun), qom = float(P.qom);
-    const float vc = c == 0 ? v0 : c == 1 ? v1 : v2;
-    const float lin = vc * rvun;
-    const bool is3 = c == 3, is4 = c == 4;
-    const float b = is3 ? f5 : is4 ? f3 : f4;
-    const float e = is3 ? f4 : is4 ? f5 : f3;
+    // v_a and v_d of component c (for c < 3, v_c in va)
+    const bool a0 = c == 0 || c == 5, a1 = c == 1 || c == 3;
+    const float va = a0 ? v0 : a1 ? v1 : v2;
+    const float vd = c == 4 ? v0 : c == 5 ? v1 : v2;
+    const float lin = va * rvun;
+    const float bd = __shfl(fc, L.rd), ba = __shfl(fc, L.ra);
     float rot;
     if constexpr (MODE == PTC_MODE_EB) {
-        const float a = is3 ? v1 : is4 ? v2 : v0;
-        const float d = is3 ? v2 : is4 ? v0 : v1;
-        const float ec = is3 ? f0 : is4 ? f1 : f2;
-        rot = qom * (ec + (a * b - d * e));
+        const float ek = __shfl(fc, L.own);
+        rot = qom * (ek + (va * bd - vd * ba));
     } else {
-        const float w0 = v0 - f0, w1 = v1 - f1, w2 = v2 - f2;
-        const float a = is3 ? w1 : is4 ? w2 : w0;
-        const float d = is3 ? w2 : is4 ? w0 : w1;
-        rot = qom * (a * b - d * e);
+        // U_a and U_d from lanes a and d (3 below ra and rd), which hold
+        // them as fc; w = v - U formed here
+        const float ua = __shfl(fc, L.ra - 3), ud = __shfl(fc, L.rd - 3);
+        rot = qom * ((va - ua) * bd - (vd - ud) * ba);
     }
     return c < 3 ? lin : rot;
 }
@@ -117,17 +133,16 @@ __device__ __forceinline__ bool group_finite(double x, int base)
 }
 
 // corners() (interpolations.F90:140-149, :207-210) with the per-axis work
-// shared: lane c (and c + 3) of the group computes axis c % 3 from the
-// stage point's position x, the group exchanges the left and right indices
+// shared: lane c (and c + 3) of the group computes axis c % 3 from pos, the
+// stage point's position on that axis, the group exchanges the left and right indices
 // and the in-cell fraction, and every lane forms the eight offsets and
 // weights from them in corners()'s order (same operations, same results).
 // xdo (if given): the in-cell fraction of the three axes.
 template <class W, int GEOM>
-__device__ __forceinline__ void group_corners(const StepParams &P, const double *x, int base,
+__device__ __forceinline__ void group_corners(const StepParams &P, double pos, int base,
                                               int c, CornersT<W> &cr, float *xdo = nullptr)
 {
     const int a = c < 3 ? c : c - 3;
-    const double pos = a == 0 ? x[0] : a == 1 ? x[1] : x[2];
     const double xl = a == 0 ? P.xl[0] : a == 1 ? P.xl[1] : P.xl[2];
     const double inv_dm = a == 0 ? P.inv_dm[0] : a == 1 ? P.inv_dm[1] : P.inv_dm[2];
     const double dmd = a == 0 ? P.dmd[0] : a == 1 ? P.dmd[1] : P.dmd[2];
@@ -203,8 +218,10 @@ struct GroupForce {
     K fint = K(0);
     float xdl[3] = {0.0f, 0.0f, 0.0f};
 
+    const ForceLanes L;
+
     __device__ __forceinline__ GroupForce(const DeviceFields &F_, const StepParams &P_, int b, int c)
-        : F(F_), P(P_), base(b), comp(c)
+        : F(F_), P(P_), base(b), comp(c), L(b, c)
     {
     }
 
@@ -309,11 +326,11 @@ struct GroupForce {
     // component comp of f(s), s given by its component comp on each lane
     __device__ __forceinline__ K operator()(double s)
     {
-        double x[6];
-#pragma unroll
-        for (int j = 0; j < 6; j++) x[j] = gsh(s, base, j);
+        // the position on this lane's axis (comp % 3) and the momentum
+        const double pos = __shfl(s, L.own);
+        const double u0 = gsh(s, base, 3), u1 = gsh(s, base, 4), u2 = gsh(s, base, 5);
         CornersT<K> c;
-        group_corners<K, GEOM>(P, x, base, comp, c, FEAT ? xdl : nullptr);
+        group_corners<K, GEOM>(P, pos, base, comp, c, FEAT ? xdl : nullptr);
         K f;
         if constexpr (TIME == 1) {
             // the two bracketing snapshots' interpolants blended in time,
@@ -356,13 +373,9 @@ struct GroupForce {
         }
         if constexpr (FEAT == 1) fint = f;
         if constexpr (PREC == PTC_FP32)
-            return force_component_f32<MODE>(P, x[3], x[4], x[5], gsh(f, base, 0),
-                                             gsh(f, base, 1), gsh(f, base, 2), gsh(f, base, 3),
-                                             gsh(f, base, 4), gsh(f, base, 5), comp);
+            return force_component_f32<MODE>(P, u0, u1, u2, f, comp, L);
         else
-            return force_component<MODE, GEOM>(P, x[3], x[4], x[5], gsh(f, base, 0),
-                                               gsh(f, base, 1), gsh(f, base, 2), gsh(f, base, 3),
-                                               gsh(f, base, 4), gsh(f, base, 5), comp, base);
+            return force_component<MODE, GEOM>(P, u0, u1, u2, f, comp, L);
     }
 };
 

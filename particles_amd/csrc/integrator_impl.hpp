@@ -179,9 +179,10 @@ template <int MODE>
 __device__ __forceinline__ void force_from_fields_f32(const StepParams &P, const double *s,
                                                       const float *f, float *a)
 {
-    // hardware reciprocal (1 ulp) in place of the IEEE divisions
+    // hardware reciprocal square root (1 ulp) of gamma squared in place of
+    // the square root and the IEEE divisions
     const float ux = float(s[3]), uy = float(s[4]), uz = float(s[5]);
-    const float rg = __builtin_amdgcn_rcpf(sqrtf(ux * ux + uy * uy + uz * uz + 1.0f));
+    const float rg = __builtin_amdgcn_rsqf(ux * ux + uy * uy + uz * uz + 1.0f);
     const float v0 = ux * rg, v1 = uy * rg, v2 = uz * rg;
     const float rvun = float(P.rvun), qom = float(P.qom);
     a[0] = v0 * rvun;
