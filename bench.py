@@ -135,7 +135,7 @@ def build_workload(args):
            "snapshots_every_T0": tmax / W.T0 / nsnap, "layout": args.layout,
            "cell_cache": args.cell_cache, "field_precision": args.prec, "sort": args.sort,
            "order": args.order, "field_snapshots": args.snapshots, "tail": args.tail,
-           "f32_layout": args.f32_layout, "recipe": recipe}
+           "f32_layout": args.f32_layout, "group_lanes": args.group_lanes, "recipe": recipe}
     if args.snapshots > 0:
         cfg["snapshot_every_T0"] = args.snapshot_every
     return make_fields, uu, bb, p, tg, st0, method, cfg
@@ -219,8 +219,9 @@ def traffic_from_profile(cfg):
     configuration."""
     import glob
     keys = ("grid", "particles", "method", "tol", "layout", "cell_cache", "field_precision",
-            "sort", "order", "field_snapshots", "snapshot_every_T0", "tail", "f32_layout")
-    defaults = {"field_snapshots": 0, "tail": "0", "f32_layout": "nodes"}
+            "sort", "order", "field_snapshots", "snapshot_every_T0", "tail", "f32_layout",
+            "group_lanes")
+    defaults = {"field_snapshots": 0, "tail": "0", "f32_layout": "nodes", "group_lanes": "6"}
     for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.json"))):
         try:
             with open(path) as fh:
@@ -240,6 +241,9 @@ def _norm(cfg, key, defaults):
         return None
     if key == "f32_layout" and cfg.get("field_precision") != "fp32":
         return None  # FP64 fields ignore it
+    if key == "group_lanes" and (cfg.get("field_snapshots", 0) > 0 or
+                                 (cfg.get("layout") == "aos" and str(cfg.get("tail")) == "0")):
+        return None  # snapshot series and one-lane runs have no trio form
     return cfg.get(key, defaults.get(key))
 
 
@@ -445,6 +449,10 @@ def main():
     ap.add_argument("--tail", default="auto",
                     help="FP64 static runs: hand a wave's last particles (at most this many) to "
                          "the group kernel once the queue is empty; 'group': that kernel only")
+    ap.add_argument("--group-lanes", default="auto", choices=["3", "6", "auto"],
+                    help="lanes per particle of the kernel that runs static fields with the PI "
+                         "controller when a wave's particles share lanes: 3 (trio kernel), 6, "
+                         "or auto (3 for FP32 DOP853 batches of 2,000,000 particles or more)")
     ap.add_argument("--grid-blocks", type=int, default=0)
     ap.add_argument("--sort", type=int, default=1,
                     help="re-order the work queue every N steps (0 = never)")
@@ -540,6 +548,7 @@ def main():
     P.set_option("sort", args.sort)
     P.set_option("order", args.order)
     P.set_option("tail", args.tail)
+    P.set_option("group_lanes", args.group_lanes)
     fields = make_fields(args.prec)
     P.set_option("f32_layout", "nodes")
     npart = st0.shape[2]

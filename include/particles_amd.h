@@ -210,17 +210,20 @@ int ptc_particles_advance_graph(ptc_particles *p, int n, const double *t_stop);
 /* stats[0]: graphs instantiated by ptc_particles_advance_graph for this
  * batch; stats[1]: calls that replayed the kept graph with new stop times. */
 int ptc_particles_graph_stats(const ptc_particles *p, int64_t stats[2]);
-/* Autotuner: integrates the batch up to t_end in 1 + 7 equal segments, the
+/* Autotuner: integrates the batch up to t_end in 1 + 9 equal segments, the
  * first to warm up, then one per candidate launch setting (work-queue order
  * by cost at 256 threads per block with one lane per particle, with the
- * group kernel alone and with the last 32768 particles handed to it ("tail"
- * option); cost order at 128 / 64 threads per block, spatial order at 256,
- * index order at 256), and keeps for this batch the one with the most RHS
- * evaluations per kernel millisecond.  The segments are real progress: every
- * setting gives the same results.  chosen (may be NULL) = block, sort
- * cadence, order (1 spatial, 2 cost), tail (0 lane per particle, -1 group
- * kernel alone, N > 0 hand-over count). */
-int ptc_particles_autotune(ptc_particles *p, double t_end, int chosen[4]);
+ * group kernel of six or three lanes per particle alone and with the last
+ * 32768 particles handed to it ("tail" and "group_lanes" options); cost
+ * order at 128 / 64 threads per block, spatial order at 256, index order at
+ * 256), and keeps for this batch the one with the most RHS evaluations per
+ * kernel millisecond.  The segments are real progress: every setting gives
+ * the same results.  chosen (may be NULL) = block, sort cadence, order
+ * (1 spatial, 2 cost), tail (0 lane per particle, -1 group kernel alone,
+ * N > 0 hand-over count), lanes per particle of the group kernel (3 or 6).
+ * No counterpart in the reference (its OpenMP loop has no launch settings,
+ * sources/particles.F90:337-356). */
+int ptc_particles_autotune(ptc_particles *p, double t_end, int chosen[5]);
 /* totals[0..4] = sum of all / accepted / rejected steps, number of RHS
  * evaluations, number of particles still running; reduced on the device. */
 int ptc_particles_totals(ptc_particles *p, int64_t totals[5]);

@@ -369,11 +369,12 @@ class Particles:
     def autotune(self, t_end: float) -> dict:
         """Integrates to t_end while timing candidate launch settings, keeps
         the fastest for this batch (ptc_particles_autotune)."""
-        c = np.zeros(4, dtype=np.int32)
+        c = np.zeros(5, dtype=np.int32)
         _check(_lib.ptc_particles_autotune(self._h, float(t_end), _ptr(c)))
         return {"block": int(c[0]), "sort": int(c[1]),
                 "order": {1: "spatial", 2: "cost"}.get(int(c[2]), "index"),
-                "tail": "group" if int(c[3]) < 0 else str(int(c[3]))}
+                "tail": "group" if int(c[3]) < 0 else str(int(c[3])),
+                "group_lanes": int(c[4])}
 
     def moments(self) -> np.ndarray:
         """(6, 4): mean, standard deviation, skewness, excess kurtosis of each

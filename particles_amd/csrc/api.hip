@@ -39,7 +39,7 @@ struct ptc_particles {
     size_t order_bytes;
     int64_t advances;     // advance calls so far (sort cadence)
     int tuned;            // launch settings below chosen by ptc_particles_autotune
-    int tuned_block, tuned_sort, tuned_order, tuned_tail;
+    int tuned_block, tuned_sort, tuned_order, tuned_tail, tuned_lanes;
     // the graph of ptc_particles_advance_graph, kept for the batch's life and
     // replayed with new stop times while everything else its launches depend
     // on (the key below) is unchanged
@@ -110,6 +110,7 @@ struct Options {
     int order = 3;  // the order then: 1 spatial (Morton), 2 cost (most work first), 3 auto
     int tail = kTailAuto;  // LaunchConfig::tail, or chosen per batch
     int f32_layout = LAYOUT_AOS;  // FP32 fields: node rows, LAYOUT_PAIRS or LAYOUT_QUADS
+    int lanes = 0;                // LaunchConfig::lanes, or 0: chosen per batch (choose_lanes)
 };
 
 std::mutex g_opt_mu;
@@ -175,6 +176,11 @@ int ptc_set_option(const char *key, const char *value)
         else if (!strcmp(value, "cost")) g_opt.order = 2;
         else if (!strcmp(value, "auto")) g_opt.order = 3;
         else return fail(PTC_EARG, "order must be spatial, cost or auto (got %s)", value);
+    } else if (!strcmp(key, "group_lanes")) {
+        if (!strcmp(value, "3")) g_opt.lanes = 3;
+        else if (!strcmp(value, "6")) g_opt.lanes = 6;
+        else if (!strcmp(value, "auto")) g_opt.lanes = 0;
+        else return fail(PTC_EARG, "group_lanes must be 3, 6 or auto (got %s)", value);
     } else if (!strcmp(key, "tail")) {
         if (!strcmp(value, "group")) {
             g_opt.tail = kTailGroupOnly;
@@ -214,6 +220,10 @@ int ptc_get_option(const char *key, char *buf, int len)
     else if (!strcmp(key, "sort")) snprintf(buf, len, "%d", o.sort);
     else if (!strcmp(key, "order"))
         snprintf(buf, len, "%s", o.order == 2 ? "cost" : o.order == 3 ? "auto" : "spatial");
+    else if (!strcmp(key, "group_lanes")) {
+        if (o.lanes == 0) snprintf(buf, len, "auto");
+        else snprintf(buf, len, "%d", o.lanes);
+    }
     else if (!strcmp(key, "tail")) {
         if (o.tail == kTailGroupOnly) snprintf(buf, len, "group");
         else if (o.tail == kTailAuto) snprintf(buf, len, "auto");
@@ -590,6 +600,20 @@ static int resolve_tail(int tail, const ptc_particles *p)
     return choose_tail(tail, p->fields->d, p->method, p->d.np, p->ctrl);
 }
 
+// Lanes per particle of the kernel that runs a static field with the PI
+// controller where particles share a wave's lanes ("group_lanes" auto,
+// measured on the MI355X, both giving the same FP64 results): three (the
+// trio kernel) for FP32 DOP853 batches of at least 2,000,000 particles
+// (1024^3, 4,000,000 particles: 408 against 472 ms per gyro-period), six
+// elsewhere (the trio kernel measured 6-13 % slower on 512^3 RK5 and FP32
+// DOP853 batches of 1,000,000 particles and on FP64 at 1024^3, whose
+// 256-register form spills).
+static int choose_lanes(int lanes, const DeviceFields &f, int method, int64_t np)
+{
+    if (lanes != 0) return lanes;
+    return f.prec == PTC_FP32 && method == PTC_DOP853 && np >= 2000000 ? 3 : 6;
+}
+
 static int build_step_params(const ptc_fields *f, int m, const double *params, int nparams,
                              StepParams &sp)
 {
@@ -745,6 +769,7 @@ int ptc_particles_advance(ptc_particles *p, double t_stop)
         o.sort = p->tuned_sort;
         o.order = p->tuned_order;
         o.tail = p->tuned_tail;
+        o.lanes = p->tuned_lanes;
     }
     resolve_order(o, p->d.np);
     p->sp.max_steps = o.max_steps;
@@ -752,7 +777,8 @@ int ptc_particles_advance(ptc_particles *p, double t_stop)
                     p->fields->d.layout != LAYOUT_SOA && p->fields->d.layout != LAYOUT_BRICK
                         ? o.cell_cache
                         : 0,
-                    p->ctrl, resolve_tail(o.tail, p)};
+                    p->ctrl, resolve_tail(o.tail, p),
+                    choose_lanes(o.lanes, p->fields->d, p->method, p->d.np)};
     if (p->ctrl != CTRL_PI && lc.cell_cache != 1)
         return fail(PTC_ESTATE, "record and learned step-size modes need the aos layout with "
                                 "cell_cache 1");
@@ -999,6 +1025,7 @@ int ptc_particles_advance_graph(ptc_particles *p, int n, const double *t_stop)
         o.sort = p->tuned_sort;
         o.order = p->tuned_order;
         o.tail = p->tuned_tail;
+        o.lanes = p->tuned_lanes;
     }
     resolve_order(o, p->d.np);
     p->sp.max_steps = o.max_steps;
@@ -1006,7 +1033,8 @@ int ptc_particles_advance_graph(ptc_particles *p, int n, const double *t_stop)
                     p->fields->d.layout != LAYOUT_SOA && p->fields->d.layout != LAYOUT_BRICK
                         ? o.cell_cache
                         : 0,
-                    p->ctrl, resolve_tail(o.tail, p)};
+                    p->ctrl, resolve_tail(o.tail, p),
+                    choose_lanes(o.lanes, p->fields->d, p->method, p->d.np)};
     if (p->ctrl != CTRL_PI && lc.cell_cache != 1)
         return fail(PTC_ESTATE, "record and learned step-size modes need the aos layout with "
                                 "cell_cache 1");
@@ -1037,7 +1065,7 @@ int ptc_particles_advance_graph(ptc_particles *p, int n, const double *t_stop)
     ptc_particles::AdvanceGraph &g = p->graph;
     const bool same = g.exec && g.replayable && g.n == n && g.lc.block == lc.block &&
                       g.lc.grid_blocks == lc.grid_blocks && g.lc.cell_cache == lc.cell_cache &&
-                      g.lc.ctrl == lc.ctrl && g.lc.tail == lc.tail &&
+                      g.lc.ctrl == lc.ctrl && g.lc.tail == lc.tail && g.lc.lanes == lc.lanes &&
                       g.max_steps == p->sp.max_steps && g.order == p->d.order &&
                       g.ctrl == p->ctrl && g.net == (const void *)p->d.net &&
                       g.shift == p->d.ctrl_shift && g.rec_thresh == p->d.rec_thresh;
@@ -1086,7 +1114,7 @@ int ptc_particles_graph_stats(const ptc_particles *p, int64_t stats[2])
     return PTC_OK;
 }
 
-int ptc_particles_autotune(ptc_particles *p, double t_end, int chosen[4])
+int ptc_particles_autotune(ptc_particles *p, double t_end, int chosen[5])
 {
     if (!p) return fail(PTC_EARG, "ptc_particles_autotune: null handle");
     if (!std::isfinite(t_end)) return fail(PTC_EARG, "ptc_particles_autotune: t_end not finite");
@@ -1098,15 +1126,18 @@ int ptc_particles_autotune(ptc_particles *p, double t_end, int chosen[4])
     HIP_TRY(hipMemcpy(ht.data(), p->d.t, sizeof(double) * ht.size(), hipMemcpyDeviceToHost));
     double t0 = ht[0];
     for (double v : ht) t0 = v < t0 ? v : t0;
-    struct Cand { int block, sort, order, tail; };
-    const Cand cand[] = {{256, 1, 2, 0},     {256, 1, 2, kTailGroupOnly}, {256, 1, 2, 32768},
-                         {128, 1, 2, 0},     {64, 1, 2, 0},               {256, 1, 1, 0},
-                         {256, 0, 2, 0}};
+    struct Cand { int block, sort, order, tail, lanes; };
+    const Cand cand[] = {{256, 1, 2, 0, 6},     {256, 1, 2, kTailGroupOnly, 6},
+                         {256, 1, 2, 32768, 6}, {256, 1, 2, kTailGroupOnly, 3},
+                         {256, 1, 2, 32768, 3}, {128, 1, 2, 0, 6},
+                         {64, 1, 2, 0, 6},      {256, 1, 1, 0, 6},
+                         {256, 0, 2, 0, 6}};
     const int nc = (int)(sizeof cand / sizeof cand[0]);
     if (!(t_end > t0)) return fail(PTC_EARG, "ptc_particles_autotune: t_end must exceed the "
                                              "batch's time %g", t0);
     const int saved = p->tuned;
-    const int sb = p->tuned_block, ss = p->tuned_sort, so = p->tuned_order, st = p->tuned_tail;
+    const int sb = p->tuned_block, ss = p->tuned_sort, so = p->tuned_order, st = p->tuned_tail,
+              sl = p->tuned_lanes;
     double best = -1.0;
     int pick = 0;
     int64_t tot[5];
@@ -1118,6 +1149,7 @@ int ptc_particles_autotune(ptc_particles *p, double t_end, int chosen[4])
         p->tuned_sort = c.sort;
         p->tuned_order = c.order;
         p->tuned_tail = c.tail;
+        p->tuned_lanes = c.lanes;
         int rc = ptc_particles_totals(p, tot);
         const int64_t rhs0 = tot[3];
         if (!rc) rc = ptc_particles_advance(p, t0 + (t_end - t0) * (k + 1) / (nc + 1));
@@ -1125,6 +1157,7 @@ int ptc_particles_autotune(ptc_particles *p, double t_end, int chosen[4])
         if (rc) {
             p->tuned = saved;
             p->tuned_block = sb; p->tuned_sort = ss; p->tuned_order = so; p->tuned_tail = st;
+            p->tuned_lanes = sl;
             return rc;
         }
         const double rate = p->last_ms > 0.0 ? (double)(tot[3] - rhs0) / p->last_ms : 0.0;
@@ -1138,11 +1171,13 @@ int ptc_particles_autotune(ptc_particles *p, double t_end, int chosen[4])
     p->tuned_sort = cand[pick].sort;
     p->tuned_order = cand[pick].order;
     p->tuned_tail = cand[pick].tail;
+    p->tuned_lanes = cand[pick].lanes;
     if (chosen) {
         chosen[0] = p->tuned_block;
         chosen[1] = p->tuned_sort;
         chosen[2] = p->tuned_order;
         chosen[3] = p->tuned_tail;
+        chosen[4] = p->tuned_lanes;
     }
     ok();
     return PTC_OK;
@@ -1773,7 +1808,8 @@ int ptc_integrate_noalloc(const ptc_fields *f, int method, int64_t np, int m,
     sp.max_steps = o.max_steps;
     const int cache = f->d.layout != LAYOUT_SOA && f->d.layout != LAYOUT_BRICK ? o.cell_cache : 0;
     LaunchConfig lc{o.block, o.grid_blocks, cache, CTRL_PI,
-                    choose_tail(o.tail, f->d, method, np, CTRL_PI)};
+                    choose_tail(o.tail, f->d, method, np, CTRL_PI),
+                    choose_lanes(o.lanes, f->d, method, np)};
     hipStream_t s = (hipStream_t)stream;
     const int blocks = (int)((np + 255) / 256);
     hipLaunchKernelGGL(noalloc_start_kernel, dim3(blocks), dim3(256), 0, s, d, m);

@@ -24,7 +24,7 @@
 //     to the next step's stage 1) -- f(sf) is evaluated once and reused;
 //   * dti_guess evaluates f at a trial point and never uses it (:576);
 //   * a rejected attempt reuses f(si) (stage 1 does not depend on dt).
-#include "group_impl.hpp"
+#include "trio_impl.hpp"
 
 namespace ptc {
 namespace {
@@ -304,13 +304,15 @@ hipError_t launch_integrate(int method, const DeviceFields &f, const StepParams 
 {
     if (f.prec == PTC_FP32) return launch_integrate_mixed(method, f, sp, p, t_stop, lc, stream);
     if (f.layout == LAYOUT_PAIRS) {
-        // FP64 pair rows: static field, PI controller, the group kernel only
+        // FP64 pair rows: static field, PI controller, the trio or group kernel only
         if (f.nsnap > 0 || lc.ctrl != CTRL_PI || lc.tail != kTailGroupOnly)
             return hipErrorInvalidValue;
         const int g = sp.pow2 ? 1 : 0;
 #define GROUP_P(M, MO, G)                                                      \
     if (method == M && f.mode == MO && g == G)                                 \
-        return launch_group_t<M, MO, G, PTC_FP64, 0, CTRL_PI, ROWS_PAIRS>(f, sp, p, t_stop, kBlock, stream);
+        return lc.lanes == 3                                                   \
+                   ? launch_trio_t<M, MO, G, PTC_FP64, ROWS_PAIRS>(f, sp, p, t_stop, kBlock, stream) \
+                   : launch_group_t<M, MO, G, PTC_FP64, 0, CTRL_PI, ROWS_PAIRS>(f, sp, p, t_stop, kBlock, stream);
         GROUP_P(PTC_RK5, PTC_MODE_VB, 0) GROUP_P(PTC_RK5, PTC_MODE_EB, 0)
         GROUP_P(PTC_DOP853, PTC_MODE_VB, 0) GROUP_P(PTC_DOP853, PTC_MODE_EB, 0)
         GROUP_P(PTC_RK5, PTC_MODE_VB, 1) GROUP_P(PTC_RK5, PTC_MODE_EB, 1)
@@ -345,8 +347,9 @@ hipError_t launch_integrate(int method, const DeviceFields &f, const StepParams 
 #undef SERIES
         return hipErrorInvalidValue;
     }
-    // FP64 static field in node rows with the PI controller: the group kernel
-    // (six lanes per particle) takes the tail of the launch, or all of it
+    // FP64 static field in node rows with the PI controller: the trio kernel
+    // (three lanes per particle) or the group kernel (six) takes the tail of
+    // the launch, or all of it
     if (lc.tail != 0 && f.layout == LAYOUT_AOS && lc.ctrl == CTRL_PI) {
         const int g = sp.pow2 ? 1 : 0;
         DeviceParticles q = p;
@@ -357,7 +360,9 @@ hipError_t launch_integrate(int method, const DeviceFields &f, const StepParams 
         }
 #define GROUP(M, MO, G)                                                        \
     if (method == M && f.mode == MO && g == G)                                 \
-        return launch_group_t<M, MO, G, PTC_FP64>(f, sp, q, t_stop, kBlock, stream);
+        return lc.lanes == 3                                                   \
+                   ? launch_trio_t<M, MO, G, PTC_FP64, ROWS_NODES>(f, sp, q, t_stop, kBlock, stream) \
+                   : launch_group_t<M, MO, G, PTC_FP64>(f, sp, q, t_stop, kBlock, stream);
         GROUP(PTC_RK5, PTC_MODE_VB, 0) GROUP(PTC_RK5, PTC_MODE_EB, 0)
         GROUP(PTC_DOP853, PTC_MODE_VB, 0) GROUP(PTC_DOP853, PTC_MODE_EB, 0)
         GROUP(PTC_RK5, PTC_MODE_VB, 1) GROUP(PTC_RK5, PTC_MODE_EB, 1)

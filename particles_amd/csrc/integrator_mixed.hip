@@ -1,10 +1,13 @@
 // Mixed-precision kernels (FP32 fields): the integrator of integrator.hip
 // instantiated for FP32 node storage, FP32 trilinear weights and RHS, with
 // FP64 positions, time, stage accumulation and error norm.  This unit is
-// compiled with FMA contraction: the mixed path is not bitwise comparable to
-// the reference anyway, and fused multiply-adds in the FP64 stage sums and
-// the FP32 gathers both cut instructions and round less.
-#include "group_impl.hpp"
+// compiled with FMA contraction within expressions (-ffp-contract=on): the
+// mixed path is not bitwise comparable to the reference anyway, fused
+// multiply-adds in the FP64 stage sums and the FP32 gathers both cut
+// instructions and round less, and contracting only what one expression
+// holds makes every instantiation (node, pair or quad rows; trio or group
+// kernel) round the same way.
+#include "trio_impl.hpp"
 
 namespace ptc {
 
@@ -22,7 +25,9 @@ hipError_t launch_integrate_mixed(int method, const DeviceFields &f, const StepP
         const int rows = f.layout == LAYOUT_QUADS ? ROWS_QUADS : ROWS_PAIRS;
 #define GROUP_R(M, MO, G, CT, R)                                               \
     if (method == M && f.mode == MO && geom == G && lc.ctrl == CT && rows == R) \
-        return launch_group_t<M, MO, G, PTC_FP32, 0, CT, R>(f, sp, p, t_stop, kBlock, stream);
+        return CT == CTRL_PI && lc.lanes == 3                                  \
+                   ? launch_trio_t<M, MO, G, PTC_FP32, R>(f, sp, p, t_stop, kBlock, stream) \
+                   : launch_group_t<M, MO, G, PTC_FP32, 0, CT, R>(f, sp, p, t_stop, kBlock, stream);
 #define GROUP_P(M, MO, G, CT) GROUP_R(M, MO, G, CT, ROWS_PAIRS) GROUP_R(M, MO, G, CT, ROWS_QUADS)
 #define GROUP_P_ALL(CT)                                                        \
         GROUP_P(PTC_RK5, PTC_MODE_VB, 0, CT) GROUP_P(PTC_RK5, PTC_MODE_EB, 0, CT) \
@@ -105,7 +110,9 @@ hipError_t launch_integrate_mixed(int method, const DeviceFields &f, const StepP
         }
 #define GROUP(M, MO, G)                                                        \
     if (method == M && f.mode == MO && geom == G)                              \
-        return launch_group_t<M, MO, G, PTC_FP32>(f, sp, q, t_stop, kBlock, stream);
+        return lc.lanes == 3                                                   \
+                   ? launch_trio_t<M, MO, G, PTC_FP32, ROWS_NODES>(f, sp, q, t_stop, kBlock, stream) \
+                   : launch_group_t<M, MO, G, PTC_FP32>(f, sp, q, t_stop, kBlock, stream);
         GROUP(PTC_RK5, PTC_MODE_VB, 0) GROUP(PTC_RK5, PTC_MODE_EB, 0)
         GROUP(PTC_DOP853, PTC_MODE_VB, 0) GROUP(PTC_DOP853, PTC_MODE_EB, 0)
         GROUP(PTC_RK5, PTC_MODE_VB, 1) GROUP(PTC_RK5, PTC_MODE_EB, 1)

@@ -109,6 +109,9 @@ struct LaunchConfig {
     int tail;          // FP64 static PI runs: 0 lane per particle only; N > 0 hand the
                        // particles in flight to the group kernel once the queue is empty
                        // and at most N are left; kTailGroupOnly: the group kernel only
+    int lanes;         // lanes per particle of the group kernel for static fields with
+                       // the PI controller: 3 (integrate_trio_kernel) or 6
+                       // (integrate_group_kernel, also every series and learned run)
 };
 constexpr int kTailGroupOnly = -1;
 constexpr int kTailAuto = -2;  // option value: chosen per batch (ptc_particles_advance)

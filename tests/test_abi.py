@@ -68,10 +68,20 @@ def test_options_roundtrip_and_validation():
         assert P.get_option("f32_layout") == "quads"
         P.set_option("layout", "pairs")
         assert P.get_option("layout") == "pairs"
+        assert P.get_option("group_lanes") == "auto"
+        P.set_option("group_lanes", "3")
+        assert P.get_option("group_lanes") == "3"
+        P.set_option("group_lanes", "6")
+        assert P.get_option("group_lanes") == "6"
+        for bad in ("2", "4", "three", ""):
+            with pytest.raises(P.PtcError):
+                P.set_option("group_lanes", bad)
+        assert P.get_option("group_lanes") == "6"
     finally:
         P.set_option("layout", old)
         P.set_option("tail", "auto")
         P.set_option("f32_layout", "nodes")
+        P.set_option("group_lanes", "auto")
 
 
 def test_fields_argument_validation_before_device():

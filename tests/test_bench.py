@@ -18,7 +18,8 @@ def _cfg(argv):
     import argparse
     ap = argparse.ArgumentParser()
     for a, d in (("--workload", "turbulent256"), ("--layout", "aos"), ("--order", "cost"), ("--tail", "auto"),
-                 ("--prec", "fp64"), ("--method", ""), ("--f32-layout", "nodes")):
+                 ("--prec", "fp64"), ("--method", ""), ("--f32-layout", "nodes"),
+                 ("--group-lanes", "6")):
         ap.add_argument(a, default=d)
     for a, d in (("--particles", 0), ("--grid", 0), ("--seed", 1), ("--cell-cache", 1),
                  ("--sort", 1), ("--snapshots", 0), ("--steps", 10), ("--warmup", 2)):
@@ -29,7 +30,8 @@ def _cfg(argv):
         # the static default builds its field on the host: skip that here
         cfg = {"grid": 256, "particles": 100000, "method": "dop853", "tol": 1e-8,
                "layout": args.layout, "cell_cache": args.cell_cache, "field_precision": args.prec,
-               "sort": args.sort, "order": args.order, "field_snapshots": 0, "tail": args.tail}
+               "sort": args.sort, "order": args.order, "field_snapshots": 0, "tail": args.tail,
+               "group_lanes": args.group_lanes}
         return cfg
     return bench.build_workload(args)[-1]
 
@@ -54,6 +56,10 @@ def test_traffic_matches_only_its_configuration():
     assert bench.traffic_from_profile(_cfg(["--prec", "fp32"])) is None
     assert bench.traffic_from_profile(_cfg(["--tail", "16"])) is None
     assert bench.traffic_from_profile(_cfg(["--tail", "0"])) is None
+    # the static figure was measured with six lanes per particle; a series
+    # never runs the trio kernel, so the lanes setting does not split it
+    assert bench.traffic_from_profile(_cfg(["--group-lanes", "3"])) is None
+    assert bench.traffic_from_profile(_cfg(["--snapshots", "8", "--group-lanes", "3"])) == series
 
 
 def test_more_than_one_gpu_is_refused():

@@ -500,6 +500,7 @@ def test_autotune_keeps_results():
     say(f"[autotune] chose {chosen}")
     assert chosen["block"] in (64, 128, 256)
     assert chosen["tail"] in ("0", "group", "32768")
+    assert chosen["group_lanes"] in (3, 6)
     assert np.array_equal(s, s_ref) and np.array_equal(c, c_ref) and (status == 0).all()
 
 
@@ -854,6 +855,64 @@ def test_pair_rows_match_node_rows(method, form):
                 assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1]), (name, tail)
     finally:
         P.set_option("tail", "auto")
+
+
+@pytest.mark.parametrize("form", ["fp64 nodes", "fp64 pairs", "nodes", "pairs", "quads"])
+@pytest.mark.parametrize("method", ["rk5", "dop853"])
+def test_trio_kernel_matches_group_kernel(method, form):
+    """The trio kernel (three lanes per particle, group_lanes 3) against
+    the group kernel (six lanes, group_lanes 6) on the
+    same fields and particles, with the trio or group kernel running the
+    whole launch ("tail" group) and taking the lane-per-particle kernel's
+    last particles (a hand-over, node rows): FP64 snapshots and counters
+    bit-identical (both restate the reference's operations per component);
+    FP32 (mixed precision, FMA contraction left to the compiler in each
+    kernel) within the mixed-precision bound of test_mixed_precision_error_
+    against_oracle (median relative deviation below 1e-6, every particle
+    below 1e-3).  VB in the unit box, EB in a shifted box with
+    particles on x and y cell faces and in the last cells, so that both
+    the pair and quad rows' successor rows and the per-corner fallback are
+    read."""
+    uu, bb = W.turbulent_vb(24, seed=71, nmodes=48)
+    p = W.params_vector(tmax=1 * W.T0, dtmax=1.0, atol=1e-8, rtol=1e-8)
+    tg = W.time_grid(3, p[2])
+    st0 = W.initial_state(1000, len(tg), seed=72, gamma_range=(1.0, 1000.0))
+    rng = np.random.default_rng(73)
+    st0[0, 0, :100] = rng.integers(0, 24, 100) / 24.0
+    st0[1, 0, 100:200] = rng.integers(0, 24, 100) / 24.0
+    st0[:2, 0, 200:240] = 1.0 - 1e-12
+    ee, be = W.reconnection_eb(24, seed=74, nmodes=48)
+    g = W.params_vector(vun=0.5, tmax=1 * W.T0, dtmax=1.0, atol=1e-8, rtol=1e-8,
+                        xl=(-1.0, 0.5, 2.0), xs=(2.5, 3.0, 1.75))
+    sg = W.initial_state(1000, len(tg), seed=75, gamma_range=(1.0, 100.0),
+                         xl=(-1.0, 0.5, 2.0), xs=(2.5, 3.0, 1.75))
+    sg[0, 0, :100] = -1.0 + 2.5 * rng.integers(0, 24, 100) / 24.0
+    sg[1, 0, 100:200] = 0.5 + 3.0 * rng.integers(0, 24, 100) / 24.0
+    cases = [("VB unit box", uu, bb, {}, p, st0), ("EB shifted box", ee, be, {"mode": "eb"}, g, sg)]
+    tails = ["group"] + (["300"] if form == "fp64 nodes" or form == "nodes" else [])
+    try:
+        for name, a, b, kw, q, s0 in cases:
+            for tail in tails:
+                P.set_option("tail", tail)
+                got = {}
+                for lanes in ("6", "3"):
+                    say(f"[trio] {method} {form} {name} tail {tail} group_lanes {lanes}")
+                    P.set_option("group_lanes", lanes)
+                    with _fields_rows(a, b, form, **kw) as f:
+                        got[lanes] = P.integrate(f, method, q, tg, s0)
+                    assert (got[lanes][2] == 0).all(), (name, tail, lanes)
+                (s6, c6, _), (s3, c3, _) = got["6"], got["3"]
+                if form.startswith("fp64"):
+                    assert np.array_equal(s3, s6) and np.array_equal(c3, c6), (name, tail)
+                else:
+                    same = (c3 == c6).all(0).mean()
+                    dev = deviation(s3, s6)
+                    say(f"[trio] FP32 {name}: counters equal for {same:.3f} of the particles, "
+                        f"deviation median {np.median(dev):.2e} max {dev.max():.2e}")
+                    assert np.median(dev) < 1e-6 and dev.max() < 1e-3, (name, tail)
+    finally:
+        P.set_option("tail", "auto")
+        P.set_option("group_lanes", "auto")
 
 
 @pytest.mark.parametrize("form", ["pairs", "quads", "fp64 pairs"])
