@@ -404,6 +404,17 @@ int ptc_particles_use_mlp(ptc_particles *p, const ptc_mlp *net, double shift);
  *                 hands over (N > 0 runs one lane per particle); on FP64
  *                 fields, and in record mode, it runs one lane per
  *                 particle.
+ *   "group_lanes" static fields with the PI controller, wherever a group
+ *                 kernel runs ("tail" group or a hand-over, and pair and
+ *                 quad rows): "6" the group kernel (one component per
+ *                 lane, ten particles per wave); "3" the trio kernel
+ *                 (lane a holds position a and momentum a, 21 particles
+ *                 per wave: less than half the instructions per particle,
+ *                 more particles in flight, each trajectory slower);
+ *                 "auto" (default): three for FP32 DOP853 batches of at
+ *                 least 2,000,000 particles, six elsewhere.  FP64 results
+ *                 are identical for both; snapshot series and the learned
+ *                 controller run six lanes.
  * Takes effect for the next create/advance. */
 int ptc_set_option(const char *key, const char *value);
 /* Reads an option back as text into buf (size len). */
