@@ -304,7 +304,7 @@ def run_batch(fields, method, p, tg, st, steps, warmup, net=None, shift=0.0, lab
     return dict(tot=_delta(tot0, tot1), wall=wall, kernel_ms=kms)
 
 
-def summary(r, npart, steps, prec, method, plain_tps=None, mixed_tps=None):
+def summary(r, npart, steps, prec, method, plain_tps=None, mixed_tps=None, traffic=None):
     tot = r["tot"]
     tps = npart * steps / r["wall"]
     out = {"value": tot["accepted"] / r["wall"], "unit": "accepted particle-steps/s",
@@ -312,7 +312,7 @@ def summary(r, npart, steps, prec, method, plain_tps=None, mixed_tps=None):
            "rejected_per_accepted": tot["rejected"] / max(tot["accepted"], 1),
            "rhs_per_accepted": tot["rhs"] / max(tot["accepted"], 1),
            "kernel_ms": r["kernel_ms"], "wall_s": r["wall"],
-           "roofline": roofline(tot, r["kernel_ms"], steps, prec, method)}
+           "roofline": roofline(tot, r["kernel_ms"], steps, prec, method, traffic=traffic)}
     if plain_tps:
         out["time_throughput_vs_plain"] = tps / plain_tps
     if mixed_tps:
@@ -334,9 +334,11 @@ def _fp32_field(fields64, make_fields, layout):
         P.set_option("f32_layout", "nodes")
 
 
-def variants(args, fields64, make_fields, p, tg, st0, method, plain):
+def variants(args, fields64, make_fields, p, tg, st0, method, plain, cfg):
     """Mixed precision and learned-h (mixed) on the same workload, then the
-    accuracy of each against a tight FP64 run on the first particles."""
+    accuracy of each against a tight FP64 run on the first particles.  The
+    mixed run's roofline carries the traffic of a profile of the workload's
+    FP32 configuration, where one is committed."""
     import particles_amd as P
     from particles_amd import workloads as W
     from particles_amd.learned import calibrate, record_and_train, reference_states
@@ -373,7 +375,8 @@ def variants(args, fields64, make_fields, p, tg, st0, method, plain):
 
     f32 = _fp32_field(fields64, make_fields, args.f32_layout)
     r = run_batch(f32, method, p, tg, st0, args.steps, args.warmup, label="mixed")
-    out["mixed"] = summary(r, npart, args.steps, "fp32", method, plain_tps)
+    out["mixed"] = summary(r, npart, args.steps, "fp32", method, plain_tps,
+                           traffic=traffic_from_profile(dict(cfg, field_precision="fp32")))
     out["mixed"]["f32_layout"] = args.f32_layout
     mixed_tps = out["mixed"]["particle_T0_per_s"]
     print(f"mixed: {out['mixed']['value']:.3e} accepted/s", file=sys.stderr, flush=True)
@@ -585,7 +588,8 @@ def main():
     var = None
     if not args.no_variants and args.prec == "fp64" and args.snapshots == 0:
         print("variants ...", file=sys.stderr, flush=True)
-        var = variants(args, fields, make_fields, p, tg, st0, method, dict(acc=acc, wall=wall))
+        var = variants(args, fields, make_fields, p, tg, st0, method, dict(acc=acc, wall=wall),
+                       cfg)
 
     cpu = cpu_oracle = None
     if not args.no_cpu and args.snapshots == 0:

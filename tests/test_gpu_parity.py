@@ -578,8 +578,17 @@ def tail(request):
     P.set_option("tail", "auto")
 
 
+@pytest.fixture(params=["6", "3"])
+def lanes(request):
+    """Lanes per particle where a group kernel runs: "6" the group kernel,
+    "3" the trio kernel (two components per lane)."""
+    P.set_option("group_lanes", request.param)
+    yield request.param
+    P.set_option("group_lanes", "auto")
+
+
 @pytest.mark.parametrize("method", ["rk5", "dop853"])
-def test_group_kernel_matches_oracle(oracle, method, tail):
+def test_group_kernel_matches_oracle(oracle, method, tail, lanes):
     """The group kernel forms every per-component operation as the lane-per-
     particle kernel does and every sum over components in the same order on
     every lane: snapshots and counters bit-identical to the oracle, in the
@@ -590,7 +599,9 @@ def test_group_kernel_matches_oracle(oracle, method, tail):
     p = W.params_vector(tmax=2 * W.T0, dtmax=1.0, atol=1e-8, rtol=1e-8)
     tg = W.time_grid(4, p[2])
     st0 = W.initial_state(700, len(tg), seed=42, gamma_range=(1.0, 1000.0))
-    say(f"[group] {method} tail={tail} unit box")
+    if tail == "0" and lanes == "3":
+        pytest.skip("one lane per particle: no group kernel runs")
+    say(f"[group] {method} tail={tail} lanes={lanes} unit box")
     so, co, _ = oracle.integrate(method, uu, bb, p, tg, st0)
     with P.Fields(uu, bb) as f:
         st, cnt, status = P.integrate(f, method, p, tg, st0)
@@ -620,10 +631,10 @@ def test_group_kernel_matches_oracle(oracle, method, tail):
     assert np.array_equal(st, so) and np.array_equal(cnt, co)
 
 
-def test_group_kernel_failure_handling():
-    """Status codes from the group kernel: a NaN field node retires the
-    particle that meets it (2), a step cap retires with status 1 after exactly
-    that many attempts, the others finish (0)."""
+def test_group_kernel_failure_handling(lanes):
+    """Status codes from the group and trio kernels: a NaN field node retires
+    the particle that meets it (2), a step cap retires with status 1 after
+    exactly that many attempts, the others finish (0)."""
     dm = (8, 8, 8)
     uu = np.zeros(dm + (3,), order="F")
     bb = np.zeros(dm + (3,), order="F")
@@ -638,7 +649,7 @@ def test_group_kernel_failure_handling():
     try:
         P.set_option("tail", "group")
         with P.Fields(uu, bb) as f:
-            say("[group failure] NaN node")
+            say(f"[group failure] NaN node, lanes {lanes}")
             _, cnt, status = P.integrate(f, "dop853", p, tg, st0)
             assert status.tolist() == [2, 0, 0]
             P.set_option("max_steps", 25)
@@ -649,7 +660,7 @@ def test_group_kernel_failure_handling():
         P.set_option("tail", "auto")
 
 
-def test_handover_counts_retired_particles(oracle):
+def test_handover_counts_retired_particles(oracle, lanes):
     """Particles already retired when a launch fetches them (here 3 of 4,
     non-finite initial momenta, status 2 after the first segment) count as
     finished for the hand-over threshold, so the running ones are handed to
@@ -667,7 +678,7 @@ def test_handover_counts_retired_particles(oracle):
     try:
         P.set_option("tail", "256")
         with P.Fields(uu, bb) as f, P.Particles(f, "dop853", p, tg, st0) as b:
-            say("[hand-over] 3000 retired of 4000, tail 256")
+            say(f"[hand-over] 3000 retired of 4000, tail 256, lanes {lanes}")
             for t_stop in (0.2 * W.T0, 0.6 * W.T0):
                 b.advance(t_stop)
             b.advance()
