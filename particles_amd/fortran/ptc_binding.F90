@@ -115,7 +115,7 @@ module ptc_binding
       import :: c_int, c_double, c_ptr
       type(c_ptr), value           :: p
       real(c_double), value        :: t_end
-      integer(c_int), intent(out)  :: chosen(4)
+      integer(c_int), intent(out)  :: chosen(5)  ! block, sort, order, tail, group lanes
     end function
 
     integer(c_int) function ptc_particles_totals(p, totals) bind(C, name="ptc_particles_totals")
