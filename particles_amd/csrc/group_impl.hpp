@@ -575,12 +575,12 @@ __device__ __forceinline__ void group_features(const GF &force, const StepParams
 }
 
 // Waves per SIMD the kernel is compiled for.  FP32 static fields with the
-// PI controller: three (168 registers, 0-28 B of spills), 1.04-1.24x the
-// accepted steps per second of two (1024^3 DOP853 +15 %, 512^3 RK5 +22 %).
-// FP64 static fields in pair rows with the PI controller: three as well
-// (168 registers; RK5 spills 68 B, DOP853 116 B, mostly loop-invariant
-// constants reloaded once per attempt), 6.5 % faster than two at 1024^3
-// (DOP853, 4,000,000 particles).  The other forms need 180-230 registers
+// PI controller: three (168 registers, 12-44 B of spills on quad rows),
+// 1.04-1.24x the accepted steps per second of two (1024^3 DOP853 +15 %,
+// 512^3 RK5 +22 %).  FP64 static fields in pair rows with the PI
+// controller: three as well (168 registers; RK5 spills 76 B, DOP853 124 B,
+// mostly loop-invariant constants reloaded once per attempt), 6.5 % faster
+// than two at 1024^3 (DOP853, 4,000,000 particles).  The other forms need 180-230 registers
 // and run two (FP64 node rows forced to three measured within 1.3 % of two;
 // keeping the FP64 cell cache in LDS instead of registers, to cut the
 // spills, was 5 % slower).

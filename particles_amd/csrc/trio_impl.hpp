@@ -448,9 +448,11 @@ __device__ __forceinline__ void trio_hermite_out(double *st, int a, int base, do
     }
 }
 
-// Waves per SIMD the kernel is compiled for (two: the DOP853 forms need
-// about 200-250 registers with two components per lane).
-constexpr int trio_waves(int prec) { return prec == PTC_FP32 ? 2 : 2; }
+// Waves per SIMD the kernel is compiled for: two.  With two components per
+// lane the FP32 forms need 231-238 registers and no scratch, the FP64 forms
+// 256 registers with 36-68 B of spills; held to 168 registers for a third
+// wave, the FP32 forms would spill 212-236 B per lane.
+constexpr int kTrioWaves = 2;
 
 // The particle loop of integrate_group_kernel (static field, PI controller),
 // one particle per trio, particles taken from the second work-queue head
@@ -459,7 +461,7 @@ constexpr int trio_waves(int prec) { return prec == PTC_FP32 ? 2 : 2; }
 // running, or that were started and already reached the end of the segment,
 // are skipped.
 template <int METHOD, int MODE, int GEOM, int PREC, int ROWS>
-__global__ void __launch_bounds__(kBlock, trio_waves(PREC))
+__global__ void __launch_bounds__(kBlock, kTrioWaves)
 integrate_trio_kernel(const DeviceFields F, const StepParams P, const DeviceParticles S,
                       double t_stop)
 {
