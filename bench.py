@@ -334,6 +334,35 @@ def _fp32_field(fields64, make_fields, layout):
         P.set_option("f32_layout", "nodes")
 
 
+def host_buffer_rate(fields, method, p, st0, n_t0):
+    """The same particles integrated from t = 0 to n_t0 gyro-periods through
+    ptc_integrate, the reference driver's call (host arrays in and out: the
+    batch's allocation, upload, integration and download inside the timed
+    call), next to the integrator kernel time of the same span on a
+    device-resident batch.  Not `value`: the PCIe-inclusive rate of the
+    host-buffer boundary."""
+    import particles_amd as P
+    from particles_amd import workloads as W
+    q = np.asarray(p, dtype=np.float64).copy()
+    q[2] = n_t0 * W.T0
+    tgh = np.array([0.0, q[2]])
+    sth = np.asfortranarray(st0[:, :1, :].repeat(2, axis=1))
+    with P.Particles(fields, method, q, tgh, sth) as b:
+        kms = b.advance()
+        tot = b.totals()
+    t0 = time.perf_counter()
+    _, cnt, status = P.integrate(fields, method, q, tgh, sth)
+    wall = time.perf_counter() - t0
+    acc = int(cnt[1].sum())
+    assert acc == tot["accepted"] and (status == 0).all()
+    return {"span_T0": n_t0, "accepted": acc, "wall_s": wall, "value": acc / wall,
+            "unit": "accepted particle-steps/s", "kernel_s": kms * 1e-3,
+            "kernel_value": acc / (kms * 1e-3),
+            "host_bytes_each_way": int(sth.nbytes),
+            "note": "ptc_integrate with host arrays (allocation, upload, integration from t = 0, "
+                    "download) against the integrator kernel time of the same span"}
+
+
 def variants(args, fields64, make_fields, p, tg, st0, method, plain, cfg):
     """Mixed precision and learned-h (mixed) on the same workload, then the
     accuracy of each against a tight FP64 run on the first particles.  The
@@ -468,6 +497,9 @@ def main():
                     help="fp32 = mixed precision (FP32 fields, weights and RHS; FP64 state)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--host-buffers", type=float, default=0.0,
+                    help="also time ptc_integrate with host arrays (upload, integration from "
+                         "t = 0 over this many gyro-periods, download): the PCIe-inclusive rate")
     ap.add_argument("--no-variants", action="store_true",
                     help="skip the mixed-precision and learned-h runs")
     ap.add_argument("--train-particles", type=int, default=100_000)
@@ -585,6 +617,11 @@ def main():
     rl = roofline(tot, kernel_ms, args.steps, args.prec, method, args.snapshots,
                   traffic_from_profile(cfg))
 
+    hostbuf = None
+    if args.host_buffers > 0 and args.snapshots == 0:
+        print("host buffers ...", file=sys.stderr, flush=True)
+        hostbuf = host_buffer_rate(fields, method, p, st0, args.host_buffers)
+
     var = None
     if not args.no_variants and args.prec == "fp64" and args.snapshots == 0:
         print("variants ...", file=sys.stderr, flush=True)
@@ -631,6 +668,7 @@ def main():
         "cpu_baseline_oracle_eb": cpu_oracle,
         "variants": var,
         "autotuned": tuned,
+        "host_buffers": hostbuf,
         "energy_note": ("FP64 is bit-identical to the oracle (the reference's algorithm); with "
                         "E = 0 its relative gamma drift over 10 T0 at tol 1e-8 has median "
                         "1.3e-10 and maximum 1.3e-6 (tests/test_ensemble.py), so the 1e-10 "
