@@ -518,6 +518,10 @@ def main():
     ap.add_argument("--copy-bandwidth", action="store_true",
                     help="only time device-to-device copies and full reductions of a 4 GiB buffer "
                          "(bytes per second: practical HBM figures next to the nominal 8 TB/s)")
+    ap.add_argument("--gather-bandwidth", type=float, nargs="*", default=None,
+                    help="only measure random aligned 128-byte line reads from device buffers "
+                         "of these sizes in GiB (default 1 16 128): the ceiling of gathers "
+                         "from fields larger than the caches")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only copy a 1 GiB float64 device tensor three times (the known-size "
                          "copy that scales the rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")
@@ -553,6 +557,19 @@ def main():
                           "copy_read_plus_write_GBps": gbps,
                           "sum_read_GBps": 20 * a.numel() * 8 / rsec / 1e9,
                           "check": float(r)}), flush=True)
+        return
+    if args.gather_bandwidth is not None:
+        import particles_amd as P
+        sizes = args.gather_bandwidth or [1.0, 16.0, 128.0]
+        rows = []
+        for gib in sizes:
+            r = P.diag_line_gather(int(gib * (1 << 30)), 4096)
+            rows.append({"buffer_GiB": gib, **r})
+            print(f"{gib:g} GiB: {r['GBps']:.0f} GB/s ({r['ms']:.1f} ms)", file=sys.stderr,
+                  flush=True)
+        print(json.dumps({"metric": "random aligned 128-byte line reads", "unit": "GB/s",
+                          "lines_in_flight_per_lane": 8, "bytes_per_lane_load": 16,
+                          "rows": rows}), flush=True)
         return
     if args.calibrate_copy:
         import torch

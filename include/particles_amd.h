@@ -432,6 +432,16 @@ const char *ptc_version(void);
  * (sources/integrations.F90:177,186,466,486,581). */
 double ptc_pow_host(double x, double y);
 
+/* Diagnostic, no counterpart in the reference: the rate at which the
+ * device serves random, aligned 128-byte lines from a buffer of `bytes`
+ * (allocated and freed here): groups of eight lanes each read one line per
+ * load (16 bytes a lane), eight independent loads in flight per lane, for
+ * `rounds` rounds of loads, timed with HIP events after one untimed launch.
+ * This is the access pattern of the integrator's cell loads from fields far
+ * larger than the caches (pair and quad rows are one line each), so its
+ * rate bounds a gather-bound launch.  result = GB/s, ms, bytes read. */
+int ptc_diag_line_gather(int64_t bytes, int rounds, double result[3]);
+
 #ifdef __cplusplus
 }
 #endif

@@ -98,6 +98,7 @@ def _load():
         "ptc_mlp_probe_gemm": ([vp, vp, vp], i32),
         "ptc_last_error": ([], C.c_char_p),
         "ptc_version": ([], C.c_char_p),
+        "ptc_diag_line_gather": ([i64, i32, vp], i32),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -120,6 +121,15 @@ def _ptr(a: np.ndarray):
 
 def version() -> str:
     return _lib.ptc_version().decode()
+
+
+def diag_line_gather(nbytes: int, rounds: int = 4096) -> dict:
+    """Random aligned 128-byte lines read from a device buffer of nbytes
+    (ptc_diag_line_gather): the rate that bounds gathers from fields far
+    larger than the caches."""
+    r = np.zeros(3, dtype=np.float64)
+    _check(_lib.ptc_diag_line_gather(int(nbytes), int(rounds), _ptr(r)))
+    return {"GBps": float(r[0]), "ms": float(r[1]), "bytes": float(r[2])}
 
 
 def set_option(key: str, value) -> None:

@@ -134,6 +134,47 @@ const char *ptc_version(void) { return "particles_amd 0.1 (gfx950)"; }
 
 double ptc_pow_host(double x, double y) { return ptc_libm_pow(x, y); }
 
+int ptc_diag_line_gather(int64_t bytes, int rounds, double result[3])
+{
+    if (!result || bytes < 128 * 1024 || rounds < 1)
+        return fail(PTC_EARG, "ptc_diag_line_gather: needs bytes >= 128 KiB and rounds >= 1");
+    const uint64_t nlines = (uint64_t)bytes / 128;
+    void *buf = nullptr;
+    uint32_t *sink = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int dev = 0, cus = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    // the buffer's contents do not matter: it is read, never used
+    if (e == hipSuccess) e = hipMalloc(&buf, nlines * 128);
+    if (e == hipSuccess) e = hipMalloc((void **)&sink, sizeof(uint32_t));
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    // eight 256-thread blocks per CU, 32 waves per CU
+    const unsigned blocks = (unsigned)(cus > 0 ? cus : 1) * 8u;
+    const double lines = (double)blocks * 256.0 / kDiagLineLanes * rounds * kDiagInFlight;
+    float ms = 0.f;
+    for (int pass = 0; pass < 2 && e == hipSuccess; pass++) {
+        // pass 0 warms the code and the address translation; pass 1 is timed
+        e = hipEventRecord(e0, nullptr);
+        if (e == hipSuccess)
+            e = launch_line_gather(buf, nlines, (uint64_t)(pass + 1), rounds, sink, blocks, nullptr);
+        if (e == hipSuccess) e = hipEventRecord(e1, nullptr);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (sink) (void)hipFree(sink);
+    if (buf) (void)hipFree(buf);
+    if (e != hipSuccess) return hip_fail(e, "ptc_diag_line_gather");
+    result[0] = lines * 128.0 / (ms * 1e-3) / 1e9;
+    result[1] = ms;
+    result[2] = lines * 128.0;
+    ok();
+    return PTC_OK;
+}
+
 int ptc_set_option(const char *key, const char *value)
 {
     if (!key || !value) return fail(PTC_EARG, "ptc_set_option: null key or value");

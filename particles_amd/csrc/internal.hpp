@@ -133,6 +133,14 @@ hipError_t launch_probe(const DeviceFields &f, const StepParams &sp, int op, int
                         const double *in, double *out, hipStream_t stream);
 // Sums counters on the device: out[0..4] = all, accepted, rejected, rhs, running.
 hipError_t launch_totals(const DeviceParticles &p, int64_t *d_out, hipStream_t stream);
+// Line-gather diagnostic (diag.hip): `blocks` blocks of 256 threads, each
+// group of kDiagLineLanes lanes reading one random aligned 128-byte line of
+// buf (nlines lines) per load, kDiagInFlight loads in flight, `rounds`
+// rounds.
+constexpr int kDiagLineLanes = 8;
+constexpr int kDiagInFlight = 8;
+hipError_t launch_line_gather(const void *buf, uint64_t nlines, uint64_t seed, int rounds,
+                              uint32_t *sink, unsigned blocks, hipStream_t stream);
 // Zeroes n int32 values (graph-safe replacement for hipMemsetAsync).
 hipError_t launch_zero_i32(int32_t *ptr, int64_t n, hipStream_t stream);
 hipError_t launch_zero_i64(int64_t *ptr, int64_t n, hipStream_t stream);

@@ -983,3 +983,17 @@ def test_pair_rows_refused_where_node_rows_are_read(form):
             P.set_option("tail", "auto")
     assert (got[2] == 0).all()
     assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1])
+
+
+def test_line_gather_diagnostic():
+    """The line-gather diagnostic reads what it says it reads and reports a
+    positive rate below the nominal HBM peak."""
+    r = P.diag_line_gather(256 << 20, 16)
+    say(f"[diag] 256 MiB buffer: {r['GBps']:.0f} GB/s")
+    # 8 blocks of 256 threads per CU, 8 lanes a line (32 lines per block and
+    # load), 8 loads in flight, 16 rounds: a whole number of CUs' worth
+    per_cu = 8 * (256 // 8) * 8 * 16 * 128
+    assert r["bytes"] > 0 and r["bytes"] % per_cu == 0
+    assert 0 < r["GBps"] < 2.0e4 and r["ms"] > 0
+    with pytest.raises(P.PtcError):
+        P.diag_line_gather(1024, 16)
