@@ -25,8 +25,8 @@ hipError_t launch_integrate_mixed(int method, const DeviceFields &f, const StepP
         const int rows = f.layout == LAYOUT_QUADS ? ROWS_QUADS : ROWS_PAIRS;
 #define GROUP_R(M, MO, G, CT, R)                                               \
     if (method == M && f.mode == MO && geom == G && lc.ctrl == CT && rows == R) \
-        return CT == CTRL_PI && lc.lanes == 3                                  \
-                   ? launch_trio_t<M, MO, G, PTC_FP32, R>(f, sp, p, t_stop, kBlock, stream) \
+        return lc.lanes == 3                                                   \
+                   ? launch_trio_t<M, MO, G, PTC_FP32, R, CT>(f, sp, p, t_stop, kBlock, stream) \
                    : launch_group_t<M, MO, G, PTC_FP32, 0, CT, R>(f, sp, p, t_stop, kBlock, stream);
 #define GROUP_P(M, MO, G, CT) GROUP_R(M, MO, G, CT, ROWS_PAIRS) GROUP_R(M, MO, G, CT, ROWS_QUADS)
 #define GROUP_P_ALL(CT)                                                        \
@@ -68,12 +68,14 @@ hipError_t launch_integrate_mixed(int method, const DeviceFields &f, const StepP
 #undef SERIES
         return hipErrorInvalidValue;
     }
-    // learned controller: the group kernel runs the whole launch ("tail"
-    // group) or one lane per particle does (no hand-over)
+    // learned controller: the group or trio kernel ("group_lanes") runs the
+    // whole launch ("tail" group) or one lane per particle does (no hand-over)
     if (lc.ctrl == CTRL_LEARNED && lc.tail == kTailGroupOnly) {
 #define GROUP_L(M, MO, G)                                                      \
     if (method == M && f.mode == MO && geom == G)                              \
-        return launch_group_t<M, MO, G, PTC_FP32, 0, CTRL_LEARNED>(f, sp, p, t_stop, kBlock, stream);
+        return lc.lanes == 3                                                   \
+                   ? launch_trio_t<M, MO, G, PTC_FP32, ROWS_NODES, CTRL_LEARNED>(f, sp, p, t_stop, kBlock, stream) \
+                   : launch_group_t<M, MO, G, PTC_FP32, 0, CTRL_LEARNED>(f, sp, p, t_stop, kBlock, stream);
         GROUP_L(PTC_RK5, PTC_MODE_VB, 0) GROUP_L(PTC_RK5, PTC_MODE_EB, 0)
         GROUP_L(PTC_DOP853, PTC_MODE_VB, 0) GROUP_L(PTC_DOP853, PTC_MODE_EB, 0)
         GROUP_L(PTC_RK5, PTC_MODE_VB, 1) GROUP_L(PTC_RK5, PTC_MODE_EB, 1)

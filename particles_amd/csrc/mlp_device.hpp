@@ -203,21 +203,24 @@ __device__ __forceinline__ float mlp_forward_wave(const MlpInfer &W, MlpWaveBuf 
     return W.b3 + (lane < 32 ? y[0] : y[1]);
 }
 
-// Forward pass for the ten six-lane groups of a wave (lanes 6 g .. 6 g + 5
-// hold the same features, those of sample g; lanes 60..63 belong to no
-// group): the ten samples fill one 32-sample column tile, so the layers take
-// half the matrix products of mlp_forward_wave and a quarter of its
-// accumulators, with the same operations per sample (identical outputs).
-// Every lane of group g receives sample g's output.  Every lane of the wave
-// must call it, with the wave converged.  Buffer rows 32..63 hold the
-// inputs of samples 0..31 (samples 10..31 zero), rows 0..31 the layer-1
-// activations.
+// Forward pass for the groups of G lanes of a wave (G = 6: ten groups,
+// lanes 60..63 in none; G = 3: 21 groups, lane 63 in none; lanes G g ..
+// G g + G - 1 hold the same features, those of sample g): the samples fill
+// one 32-sample column tile, so the layers take half the matrix products of
+// mlp_forward_wave and a quarter of its accumulators, with the same
+// operations per sample (identical outputs).  Every lane of group g
+// receives sample g's output.  Every lane of the wave must call it, with the
+// wave converged.  Buffer rows 32..63 hold the inputs of samples 0..31
+// (samples past the last group zero), rows 0..31 the layer-1 activations.
+template <int G = 6>
 __device__ __forceinline__ float mlp_forward_groups(const MlpInfer &W, MlpWaveBuf &B,
                                                     const float *feat)
 {
+    constexpr int kGroups = 64 / G;
+    static_assert(kGroups <= 32, "one 32-sample tile");
     const int lane = (int)__lane_id();
-    const int g = lane / 6;
-    if (lane < 60 && lane == 6 * g) {
+    const int g = lane / G;
+    if (lane < G * kGroups && lane == G * g) {
         bf16x8 x0, x1;
 #pragma unroll
         for (int i = 0; i < 8; i++) {
@@ -227,7 +230,7 @@ __device__ __forceinline__ float mlp_forward_groups(const MlpInfer &W, MlpWaveBu
         *(bf16x8 *)&B.h[32 + g][0] = x0;
         *(bf16x8 *)&B.h[32 + g][8] = x1;
     }
-    if (lane >= 42) {
+    if (lane >= 32 + kGroups) {
         bf16x8 z;
 #pragma unroll
         for (int i = 0; i < 8; i++) z[i] = __bf16(0.0f);

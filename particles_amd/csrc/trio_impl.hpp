@@ -36,7 +36,9 @@ constexpr int kTriosPerWave = 64 / kTrio;
 // eight corner rows of the cell it last gathered from (the cell cache, split
 // over the trio).  The corner lookup is group_corners with the trio's three
 // lanes as the three axes.
-template <int MODE, int GEOM, int PREC, int ROWS>
+// FEAT = 1 also keeps, from the last evaluation, the lane's two interpolated
+// field components and the in-cell fraction (step-size features).
+template <int MODE, int GEOM, int PREC, int ROWS, int FEAT = 0>
 struct TrioForce {
     static_assert(ROWS != ROWS_QUADS || PREC == PTC_FP32, "quad rows hold FP32 fields");
     using K = std::conditional_t<PREC == PTC_FP32, float, double>;
@@ -47,6 +49,8 @@ struct TrioForce {
     uint32_t key0 = 0xffffffffu, key7 = 0xffffffffu;
     uint32_t loads = 0;  // cell loads of the trio's particle (the same on each lane)
     K v[2][8];           // components a and a + 3 of the cached cell's corner rows
+    K fint0 = K(0), fint1 = K(0);          // FEAT: U or E component a, B component a
+    float xdl[3] = {0.0f, 0.0f, 0.0f};     // FEAT: the in-cell fraction
 
     __device__ __forceinline__ TrioForce(const DeviceFields &F_, const StepParams &P_, int b,
                                          int a_)
@@ -124,7 +128,7 @@ struct TrioForce {
     __device__ __forceinline__ void operator()(const double *s, K *out)
     {
         CornersT<K> c;
-        group_corners<K, GEOM>(P, s[0], base, a, c);
+        group_corners<K, GEOM>(P, s[0], base, a, c, FEAT ? xdl : nullptr);
         if (c.o[0] != key0 || c.o[7] != key7) {
             loads++;
             if constexpr (ROWS == ROWS_PAIRS) load_pairs(c.o);
@@ -140,6 +144,10 @@ struct TrioForce {
         for (int j = 1; j < 8; j++) {
             f0 = f0 + c.w[j] * v[0][j];
             f1 = f1 + c.w[j] * v[1][j];
+        }
+        if constexpr (FEAT == 1) {
+            fint0 = f0;
+            fint1 = f1;
         }
         const double u0 = __shfl(s[1], base), u1 = __shfl(s[1], base + 1),
                      u2 = __shfl(s[1], base + 2);
@@ -178,6 +186,32 @@ struct TrioForce {
         }
     }
 };
+
+// The step-size features (features_bc) of the trio's accepted state si, on
+// every lane of the trio, from the force's last evaluation, which was at si:
+// the interpolated fields and the in-cell fraction it kept, and |B| at the
+// eight corners from the B components the three lanes hold (the same values
+// and operations as group_features).
+template <int MODE, class TF>
+__device__ __forceinline__ void trio_features(const TF &force, const StepParams &P,
+                                              const double *si, int base, double h, float err,
+                                              float pi_ratio, bool rej, float *f)
+{
+    const double s[6] = {0.0, 0.0, 0.0, __shfl(si[1], base), __shfl(si[1], base + 1),
+                         __shfl(si[1], base + 2)};
+    float ub[6], bc[8];
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        ub[j] = float(__shfl(force.fint0, base + j));
+        ub[3 + j] = float(__shfl(force.fint1, base + j));
+    }
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+        const float b = float(force.v[1][j]);
+        bc[j] = corner_bmag(__shfl(b, base), __shfl(b, base + 1), __shfl(b, base + 2));
+    }
+    features_bc<MODE>(P, s, ub, bc, force.xdl, h, err, pi_ratio, rej, f);
+}
 
 // component j (0..5) of a value held as [0] (j < 3) or [1] (j >= 3) on lane
 // j % 3 of the trio
@@ -454,22 +488,32 @@ __device__ __forceinline__ void trio_hermite_out(double *st, int a, int base, do
 // wave, the FP32 forms would spill 212-236 B per lane.
 constexpr int kTrioWaves = 2;
 
-// The particle loop of integrate_group_kernel (static field, PI controller),
-// one particle per trio, particles taken from the second work-queue head
-// S.qcount[1]: it runs a whole launch or takes the particles the
-// lane-per-particle kernel hands over.  Particles whose status is not
-// running, or that were started and already reached the end of the segment,
-// are skipped.
-template <int METHOD, int MODE, int GEOM, int PREC, int ROWS>
+// The particle loop of integrate_group_kernel (static field), one particle
+// per trio, particles taken from the second work-queue head S.qcount[1]: it
+// runs a whole launch or takes the particles the lane-per-particle kernel
+// hands over.  Particles whose status is not running, or that were started
+// and already reached the end of the segment, are skipped.  CTRL_PI: the
+// reference's controller; CTRL_LEARNED (FP32 fields): after an accepted step
+// the network proposes the next step, as in the group kernel, evaluated once
+// per wave for its 21 particles (mlp_forward_groups<3>; the network and a
+// 64x72 BF16 buffer per wave in dynamic LDS).
+template <int METHOD, int MODE, int GEOM, int PREC, int ROWS, int CTRL = CTRL_PI>
 __global__ void __launch_bounds__(kBlock, kTrioWaves)
 integrate_trio_kernel(const DeviceFields F, const StepParams P, const DeviceParticles S,
                       double t_stop)
 {
+    static_assert(CTRL == CTRL_PI || (CTRL == CTRL_LEARNED && PREC == PTC_FP32),
+                  "the trio kernel's learned controller takes FP32 fields");
     __shared__ double s_logt[128][3];
     __shared__ unsigned long long s_expt[256];
     for (int i = threadIdx.x; i < 128 * 3; i += blockDim.x)
         (&s_logt[0][0])[i] = (&ptc_pow_log_tab[0][0])[i];
     for (int i = threadIdx.x; i < 256; i += blockDim.x) s_expt[i] = ptc_pow_exp_tab[i];
+    extern __shared__ __attribute__((aligned(16))) char s_dyn[];
+    if constexpr (CTRL == CTRL_LEARNED) {
+        for (int i = threadIdx.x; i < (int)(sizeof(MlpInfer) / 4); i += blockDim.x)
+            ((uint32_t *)s_dyn)[i] = ((const uint32_t *)&S.net->f)[i];
+    }
     __syncthreads();
     const PowTables T{s_logt, s_expt};
 
@@ -477,7 +521,7 @@ integrate_trio_kernel(const DeviceFields F, const StepParams P, const DevicePart
     const int base = (lane / kTrio) * kTrio, a = lane - base;
     const double t_end = t_stop < P.tmx ? t_stop : P.tmx;
     const int64_t N = S.np;
-    TrioForce<MODE, GEOM, PREC, ROWS> force(F, P, base, a);
+    TrioForce<MODE, GEOM, PREC, ROWS, CTRL == CTRL_LEARNED ? 1 : 0> force(F, P, base, a);
     using K = typename decltype(force)::K;
     // this lane's two state planes, chosen once (indexing the kernel
     // argument's pointer array by a per-lane value would copy it to scratch)
@@ -493,6 +537,11 @@ integrate_trio_kernel(const DeviceFields F, const StepParams P, const DevicePart
     double t = 0.0, dt = 0.0, errold = 1.0e-04, tnext = 0.0;
     int32_t n = 2, flags = 0, status = PTC_STATUS_RUNNING;
     int64_t c_all = 0, c_acc = 0, c_rej = 0, c_rhs = 0;
+    // learned controller: features are taken at the top of the iteration
+    // after an accepted step of size h_acc with error ratio err_acc
+    bool want_feat = false, rej_since = false, rej_before = false;
+    double h_acc = 0.0;
+    float err_acc = 0.0f, pi_ratio = 0.0f;
 
     for (;;) {
         // ---- refill trios without a particle (one atomic per wave) ----
@@ -516,6 +565,7 @@ integrate_trio_kernel(const DeviceFields F, const StepParams P, const DevicePart
                         active = true;
                         fresh = true;
                         need_rhs = true;
+                        want_feat = rej_since = false;
                         si[0] = xp[idx];
                         si[1] = up[idx];
                         dt = S.dt[idx];
@@ -539,6 +589,36 @@ integrate_trio_kernel(const DeviceFields F, const StepParams P, const DevicePart
             force(si, f1);
             c_rhs++;
             need_rhs = false;
+        }
+
+        if constexpr (CTRL == CTRL_LEARNED) {
+            // (the force's last evaluation was at si: the one above, or the
+            // end slope of the accepted step, which became si)
+            if (__any(want_feat)) {
+                float feat[kFeatures];
+                if (want_feat) {
+                    trio_features<MODE>(force, P, si, base, h_acc, err_acc, pi_ratio, rej_before,
+                                        feat);
+                } else {
+#pragma unroll
+                    for (int i = 0; i < kFeatures; i++) feat[i] = 0.0f;
+                }
+                const MlpInfer &W = *(const MlpInfer *)s_dyn;
+                MlpWaveBuf &wb = ((MlpWaveBuf *)(s_dyn + kNetLds))[threadIdx.x >> 6];
+                const float y = mlp_forward_groups<kTrio>(W, wb, feat);
+                if (want_feat && isfinite(y)) {
+                    // as the group kernel: growth at most facmax, never below
+                    // facmin times the PI proposal
+                    float lg = y + S.ctrl_shift;
+                    const float lfmn = logf(float(P.fcmn)), hi = logf(float(P.fcmx));
+                    const float lo = pi_ratio + lfmn > lfmn ? pi_ratio + lfmn : lfmn;
+                    lg = lg < lo ? lo : lg;
+                    lg = lg > hi ? hi : lg;
+                    const double dtl = h_acc * double(__expf(lg));
+                    dt = dtl < P.dtm ? dtl : P.dtm;
+                }
+            }
+            want_feat = false;
         }
 
         bool finish = false;
@@ -607,11 +687,15 @@ integrate_trio_kernel(const DeviceFields F, const StepParams P, const DevicePart
                             }
                             si[0] = sf[0];
                             si[1] = sf[1];
+                            if constexpr (CTRL != CTRL_PI)
+                                on_accept<CTRL>(S, idx, flags, c_rej, dt, del, 5.0f, rej_since,
+                                                want_feat, h_acc, err_acc, rej_before);
                             c_acc++;
                             dtn = dt * (P.fcmx < q ? P.fcmx : q);
                             need_rhs = true;
                         } else {
                             c_rej++;
+                            rej_since = true;
                             dtn = dt * (P.fcmn > q ? P.fcmn : q);
                         }
                     }
@@ -650,6 +734,10 @@ integrate_trio_kernel(const DeviceFields F, const StepParams P, const DevicePart
                             }
                             si[0] = sf[0];
                             si[1] = sf[1];
+                            if constexpr (CTRL != CTRL_PI)
+                                on_accept<CTRL>(S, idx, flags, c_rej, dt, err,
+                                                float(-1.0 / P.expo), rej_since, want_feat, h_acc,
+                                                err_acc, rej_before);
                             c_acc++;
                             double q = qe;
                             if (P.beta != 0.0) q = q * ptc_pow_with(errold, P.beta, T.log, T.exp);
@@ -660,6 +748,7 @@ integrate_trio_kernel(const DeviceFields F, const StepParams P, const DevicePart
                             flags &= ~FLAG_REJECTED;
                         } else {
                             c_rej++;
+                            rej_since = true;
                             dtn = dt * (P.fcmn > qe ? P.fcmn : qe);
                             flags |= FLAG_REJECTED;
                         }
@@ -668,6 +757,9 @@ integrate_trio_kernel(const DeviceFields F, const StepParams P, const DevicePart
                 if (!finish) {
                     c_all++;
                     dt = dtn < P.dtm ? dtn : P.dtm;
+                    if constexpr (CTRL != CTRL_PI) {
+                        if (want_feat) pi_ratio = float(log(dt / h_acc));
+                    }
                     if (!(t < t_end)) {
                         if (!(t < P.tmx)) status = PTC_STATUS_DONE;
                         finish = true;
@@ -693,30 +785,39 @@ integrate_trio_kernel(const DeviceFields F, const StepParams P, const DevicePart
             }
             active = false;
             need_rhs = false;
+            want_feat = false;
         }
     }
     // one count per trio (its lane 0); lane 63 never loads
     add_cell_loads(S, a == 0 && lane < kTrio * kTriosPerWave ? force.loads : 0u);
 }
 
-template <int METHOD, int MODE, int GEOM, int PREC, int ROWS>
+template <int METHOD, int MODE, int GEOM, int PREC, int ROWS, int CTRL = CTRL_PI>
 hipError_t launch_trio_t(const DeviceFields &f, const StepParams &sp, const DeviceParticles &p,
                          double t_stop, int block, hipStream_t stream)
 {
-    auto kern = integrate_trio_kernel<METHOD, MODE, GEOM, PREC, ROWS>;
+    auto kern = integrate_trio_kernel<METHOD, MODE, GEOM, PREC, ROWS, CTRL>;
+    const size_t lds =
+        CTRL == CTRL_LEARNED ? kNetLds + (size_t)(block / 64) * sizeof(MlpWaveBuf) : 0;
+    if (CTRL == CTRL_LEARNED && !p.net) return hipErrorInvalidValue;
+    if (lds > 65536) {
+        hipError_t e = hipFuncSetAttribute((const void *)kern,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
     int dev = 0, cus = 0, per_cu = 0;
     hipError_t e = hipGetDevice(&dev);
     if (e != hipSuccess) return e;
     e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     if (e != hipSuccess) return e;
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, block, 0);
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, block, lds);
     if (e != hipSuccess) return e;
     int64_t grid = (int64_t)cus * (per_cu > 0 ? per_cu : 1);
     const int64_t per_block = (int64_t)kTriosPerWave * (block / 64);
     const int64_t need = (p.np + per_block - 1) / per_block;
     if (grid > need) grid = need;
     if (grid < 1) grid = 1;
-    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(block), 0, stream, f, sp, p, t_stop);
+    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(block), lds, stream, f, sp, p, t_stop);
     return hipGetLastError();
 }
 

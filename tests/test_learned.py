@@ -200,7 +200,7 @@ def test_learned_mode_needs_a_trained_network_and_the_cell_cache():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("kernel", ["group", "0"])
+@pytest.mark.parametrize("kernel", ["group", "trio", "0"])
 @pytest.mark.parametrize("method", ["dop853", "rk5"])
 def test_trained_network_gains_over_pi(method, kernel):
     """Record with the PI controller on one particle set, train the network
@@ -208,14 +208,66 @@ def test_trained_network_gains_over_pi(method, kernel):
     far fewer steps and needs fewer RHS evaluations, at an error against a
     tight-tolerance FP64 run no larger than the PI controller's (mixed
     precision, tol 1e-8, turbulent 32^3 field, gamma in [1, 1e3]), in the
-    six-lanes-per-particle kernel ("group") and in one lane per particle
-    ("0"), the PI runs in the same kernel."""
+    six-lanes-per-particle kernel ("group"), in the three-lanes-per-particle
+    kernel ("trio") and in one lane per particle ("0"), the PI runs in the
+    same kernel."""
     import particles_amd as P
-    P.set_option("tail", kernel)
+    P.set_option("tail", "0" if kernel == "0" else "group")
+    P.set_option("group_lanes", "3" if kernel == "trio" else "6")
     try:
         _gains_over_pi(P, method)
     finally:
         P.set_option("tail", "auto")
+        P.set_option("group_lanes", "auto")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("method", ["dop853", "rk5"])
+def test_learned_trio_matches_group(method):
+    """The learned controller in the trio kernel (three lanes per particle,
+    21 samples per network tile) and in the group kernel (six lanes, ten
+    samples): the same features and network per particle from the same
+    values, the same FP32 arithmetic; every particle finishes, and the two
+    runs agree to the integration error (median relative deviation of
+    position and momentum after 0.5 T0 below 1e-4; the step sequences are
+    equal where both kernels round every feature alike)."""
+    import particles_amd as P
+    uu, bb = W.turbulent_vb(32, seed=11, nmodes=64)
+    p = W.params_vector(tmax=0.5 * W.T0, dtmax=1.0, atol=1e-8, rtol=1e-8)
+    tg = W.time_grid(1, p[2])
+    train = W.initial_state(5000, len(tg), seed=12, gamma_range=(1.0, 1e3))
+    test = W.initial_state(3000, len(tg), seed=13, gamma_range=(1.0, 1e3))
+    got = {}
+    try:
+        P.set_option("tail", "group")
+        with P.Fields(uu, bb, prec="fp32") as f:
+            with P.Particles(f, method, p, tg, train) as b:
+                b.record(1.0, 5_000_000)
+                b.advance()
+                r = b.records()
+            with P.Mlp(seed=1) as net:
+                net.normalize(r)
+                net.train(r, steps=500, batch=8192, lr=2e-3, alpha=1.0, seed=7)
+                for lanes in ("6", "3"):
+                    P.set_option("group_lanes", lanes)
+                    with P.Particles(f, method, p, tg, test) as b:
+                        b.use_mlp(net, -0.2)
+                        b.advance()
+                        got[lanes] = b.download()
+    finally:
+        P.set_option("tail", "auto")
+        P.set_option("group_lanes", "auto")
+    (s6, c6, st6), (s3, c3, st3) = got["6"], got["3"]
+    assert (st6 == 0).all() and (st3 == 0).all()
+    d = s3[:6, -1, :] - s6[:6, -1, :]
+    rel = np.maximum(np.sqrt((d[:3] ** 2).sum(0)) / np.maximum(np.sqrt((s6[:3, -1] ** 2).sum(0)), 1.0),
+                     np.sqrt((d[3:] ** 2).sum(0)) / np.maximum(np.sqrt((s6[3:6, -1] ** 2).sum(0)), 1.0))
+    same = float((c3 == c6).all(0).mean())
+    print(f"{method}: counters equal for {same:.3f} of the particles, deviation median "
+          f"{np.median(rel):.2e} max {rel.max():.2e}; accepted trio {c3[1].sum()} group "
+          f"{c6[1].sum()}")
+    assert np.median(rel) < 1e-4
+    assert abs(c3[1].sum() / c6[1].sum() - 1.0) < 0.05
 
 
 def _gains_over_pi(P, method):
