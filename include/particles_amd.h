@@ -412,9 +412,10 @@ int ptc_particles_use_mlp(ptc_particles *p, const ptc_mlp *net, double shift);
  *                 per wave: less than half the instructions per particle,
  *                 more particles in flight, each trajectory slower);
  *                 "auto" (default): three for FP32 DOP853 batches of at
- *                 least 2,000,000 particles, six elsewhere.  FP64 results
- *                 are identical for both; snapshot series and the learned
- *                 controller run six lanes.
+ *                 least 2,000,000 particles, six elsewhere.  Results are
+ *                 identical for both (FP64 and FP32); snapshot series run
+ *                 six lanes; the learned controller (FP32 fields) follows
+ *                 the option as the PI controller does.
  * Takes effect for the next create/advance. */
 int ptc_set_option(const char *key, const char *value);
 /* Reads an option back as text into buf (size len). */
