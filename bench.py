@@ -468,7 +468,7 @@ def main():
     ap.add_argument("--particles", type=int, default=0)
     ap.add_argument("--grid", type=int, default=0)
     ap.add_argument("--method", default="", choices=["", "rk5", "dop853"])
-    ap.add_argument("--layout", default=None, choices=["aos", "soa", "brick", "pairs"],
+    ap.add_argument("--layout", default=None, choices=["aos", "soa", "brick", "pairs", "columns"],
                     help="FP64 field storage (default: pairs for reconnection1024, 128-byte "
                          "rows of a node and its +x neighbour; aos node rows elsewhere)")
     ap.add_argument("--f32-layout", default=None, choices=["nodes", "pairs", "quads"],

@@ -194,7 +194,9 @@ int ptc_set_option(const char *key, const char *value)
         else if (!strcmp(value, "aos")) g_opt.layout = LAYOUT_AOS;
         else if (!strcmp(value, "brick")) g_opt.layout = LAYOUT_BRICK;
         else if (!strcmp(value, "pairs")) g_opt.layout = LAYOUT_PAIRS;
-        else return fail(PTC_EARG, "layout must be soa, aos, brick or pairs (got %s)", value);
+        else if (!strcmp(value, "columns")) g_opt.layout = LAYOUT_COLUMNS;
+        else return fail(PTC_EARG, "layout must be soa, aos, brick, pairs or columns (got %s)",
+                         value);
     } else if (!strcmp(key, "f32_layout")) {
         if (!strcmp(value, "nodes")) g_opt.f32_layout = LAYOUT_AOS;
         else if (!strcmp(value, "pairs")) g_opt.f32_layout = LAYOUT_PAIRS;
@@ -251,6 +253,7 @@ int ptc_get_option(const char *key, char *buf, int len)
         snprintf(buf, len, "%s", o.layout == LAYOUT_SOA     ? "soa"
                                  : o.layout == LAYOUT_AOS   ? "aos"
                                  : o.layout == LAYOUT_PAIRS ? "pairs"
+                                 : o.layout == LAYOUT_COLUMNS ? "columns"
                                                             : "brick");
     else if (!strcmp(key, "f32_layout"))
         snprintf(buf, len, "%s", o.f32_layout == LAYOUT_PAIRS   ? "pairs"
@@ -287,12 +290,13 @@ static int fields_from_device_soa(const int dm[3], double *soa, int mode, int pr
     if (prec == PTC_FP32) o.layout = o.f32_layout;  // FP32: node, pair or quad rows
     void *data = soa, *bricks = nullptr;
     hipError_t e = hipSuccess;
-    if (o.layout == LAYOUT_PAIRS || o.layout == LAYOUT_QUADS) {
+    if (rows_layout(o.layout)) {
         void *rows = nullptr;
         const size_t esz = prec == PTC_FP32 ? sizeof(float) : sizeof(double);
         e = hipMalloc(&rows, esz * (size_t)row_values(o.layout, prec) * (size_t)nodes);
         if (e == hipSuccess) {
             if (o.layout == LAYOUT_QUADS) e = launch_to_quads_f32(soa, dm, (float *)rows, nullptr);
+            else if (o.layout == LAYOUT_COLUMNS) e = launch_to_columns_f64(soa, dm, (double *)rows, nullptr);
             else if (prec == PTC_FP32) e = launch_to_pairs_f32(soa, dm, (float *)rows, nullptr);
             else e = launch_to_pairs_f64(soa, dm, (double *)rows, nullptr);
         }
@@ -519,12 +523,13 @@ int ptc_fields_convert(const ptc_fields *src, int prec, ptc_fields **out)
     const DeviceFields &s = src->d;
     if (s.nsnap > 0 || s.layout == LAYOUT_BRICK)
         return fail(PTC_EARG, "ptc_fields_convert: the source must be a static field in node, "
-                              "pair, quad or plane (soa) storage");
+                              "pair, quad, column or plane (soa) storage");
     Options o = options();
     const int layout = prec == PTC_FP32 ? o.f32_layout : o.layout;
     if (layout == LAYOUT_SOA || layout == LAYOUT_BRICK)
-        return fail(PTC_EARG, "ptc_fields_convert: the target layout must be node, pair or quad "
-                              "rows (layout aos or pairs; f32_layout nodes, pairs or quads)");
+        return fail(PTC_EARG, "ptc_fields_convert: the target layout must be node, pair, quad "
+                              "or column rows (layout aos, pairs or columns; f32_layout nodes, "
+                              "pairs or quads)");
     const size_t esz = prec == PTC_FP32 ? sizeof(float) : sizeof(double);
     void *rows = nullptr;
     HIP_TRY(hipMalloc(&rows, esz * (size_t)row_values(layout, prec) * (size_t)s.nodes));
@@ -607,8 +612,7 @@ static int choose_tail(int tail, const DeviceFields &f, int method, int64_t np, 
     // only, and never hands over; training records come from one lane per
     // particle
     const bool learned = ctrl == CTRL_LEARNED;
-    if (f.layout == LAYOUT_PAIRS || f.layout == LAYOUT_QUADS)
-        return kTailGroupOnly;  // read by the group kernel only
+    if (rows_layout(f.layout)) return kTailGroupOnly;  // read by the group kernel only
     if (learned && f.prec != PTC_FP32) return 0;
     if (tail != kTailAuto) return learned && tail > 0 ? 0 : tail;
     if (f.layout != LAYOUT_AOS || (ctrl != CTRL_PI && !learned)) return 0;
@@ -1290,7 +1294,7 @@ int ptc_probe(const ptc_fields *f, int op, int64_t n, const double *params, int 
     if (rc) return rc;
     if (op >= PTC_PROBE_FEATURES && f->d.layout != LAYOUT_AOS)
         return fail(PTC_EARG, "ptc_probe: op %d needs the aos layout", op);
-    if (f->d.layout == LAYOUT_PAIRS || f->d.layout == LAYOUT_QUADS)
+    if (rows_layout(f->d.layout))
         return fail(PTC_EARG, "ptc_probe: the probes read node rows (layout aos, f32_layout "
                               "nodes)");
     const int nin = op == PTC_PROBE_GATHER     ? 3

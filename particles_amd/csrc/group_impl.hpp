@@ -137,8 +137,10 @@ __device__ __forceinline__ bool group_finite(double x, int base)
 // stage point's position on that axis, the group exchanges the left and right indices
 // and the in-cell fraction, and every lane forms the eight offsets and
 // weights from them in corners()'s order (same operations, same results).
-// xdo (if given): the in-cell fraction of the three axes.
-template <class W, int GEOM>
+// xdo (if given): the in-cell fraction of the three axes.  ROWS =
+// ROWS_COLUMNS: the offsets are rows of column storage (z fastest: strides
+// dm3, dm1 dm3 and 1 along x, y and z) instead of nodes (x fastest).
+template <class W, int GEOM, int ROWS = ROWS_NODES>
 __device__ __forceinline__ void group_corners(const StepParams &P, double pos, int base,
                                               int c, CornersT<W> &cr, float *xdo = nullptr)
 {
@@ -168,10 +170,18 @@ __device__ __forceinline__ void group_corners(const StepParams &P, double pos, i
         xu[j] = 1.0 - xd[j];
         if (xdo) xdo[j] = float(xd[j]);
     }
-    const uint32_t sy = (uint32_t)P.dm[0], sz = (uint32_t)P.dm[0] * (uint32_t)P.dm[1];
-    const uint32_t yl = (uint32_t)li[1] * sy, yr = (uint32_t)ri[1] * sy;
-    const uint32_t zl = (uint32_t)li[2] * sz, zr = (uint32_t)ri[2] * sz;
-    const uint32_t x0 = (uint32_t)li[0], x1 = (uint32_t)ri[0];
+    uint32_t x0, x1, yl, yr, zl, zr;
+    if constexpr (ROWS == ROWS_COLUMNS) {
+        const uint32_t sx = (uint32_t)P.dm[2], sy = (uint32_t)P.dm[0] * (uint32_t)P.dm[2];
+        x0 = (uint32_t)li[0] * sx; x1 = (uint32_t)ri[0] * sx;
+        yl = (uint32_t)li[1] * sy; yr = (uint32_t)ri[1] * sy;
+        zl = (uint32_t)li[2]; zr = (uint32_t)ri[2];
+    } else {
+        const uint32_t sy = (uint32_t)P.dm[0], sz = (uint32_t)P.dm[0] * (uint32_t)P.dm[1];
+        yl = (uint32_t)li[1] * sy; yr = (uint32_t)ri[1] * sy;
+        zl = (uint32_t)li[2] * sz; zr = (uint32_t)ri[2] * sz;
+        x0 = (uint32_t)li[0]; x1 = (uint32_t)ri[0];
+    }
     cr.o[0] = x0 + yl + zl; cr.o[1] = x0 + yl + zr; cr.o[2] = x0 + yr + zl; cr.o[3] = x0 + yr + zr;
     cr.o[4] = x1 + yl + zl; cr.o[5] = x1 + yl + zr; cr.o[6] = x1 + yr + zl; cr.o[7] = x1 + yr + zr;
     const W u0 = W(xu[0]), u1 = W(xu[1]), u2 = W(xu[2]);
@@ -198,10 +208,16 @@ __device__ __forceinline__ void group_corners(const StepParams &P, double pos, i
 // own rows).  ROWS = ROWS_QUADS (static FP32 field in LAYOUT_QUADS): the two
 // (x-left, y-left) corner rows hold all four corners of their z plane, read
 // from them where the right x and y indices are the left ones' successors.
+// ROWS = ROWS_COLUMNS (static FP64 field in LAYOUT_COLUMNS): as quad rows,
+// the offsets rows of the column storage (group_corners), so the two rows
+// read are adjacent whenever the right z index is the left one's successor.
 template <int MODE, int GEOM, int PREC, int TIME = 0, int FEAT = 0, int ROWS = ROWS_NODES>
 struct GroupForce {
-    static_assert(ROWS == ROWS_NODES || (TIME == 0 && (ROWS == ROWS_PAIRS || PREC == PTC_FP32)),
-                  "pair rows hold a static field, quad rows a static FP32 field");
+    static_assert(ROWS == ROWS_NODES ||
+                      (TIME == 0 && (ROWS == ROWS_PAIRS || (ROWS == ROWS_QUADS && PREC == PTC_FP32) ||
+                                     (ROWS == ROWS_COLUMNS && PREC == PTC_FP64))),
+                  "pair rows hold a static field, quad rows a static FP32 field, column rows a "
+                  "static FP64 field");
     using K = std::conditional_t<PREC == PTC_FP32, float, double>;
     const DeviceFields &F;
     const StepParams &P;
@@ -323,6 +339,30 @@ struct GroupForce {
         }
     }
 
+    // the same from column rows (ROWS_COLUMNS): the (x-left, y-left) corner
+    // row of each z plane holds the lane's component of its four corners
+    // side by side (two 16-byte loads per plane), corner (x, y) at x + 2 y,
+    // when the right x and y indices are the left ones' periodic successors;
+    // else every corner comes from its own row
+    __device__ __forceinline__ void load_columns(const uint32_t *o, K *dst) const
+    {
+        const uint32_t sx = (uint32_t)P.dm[2], sy = (uint32_t)P.dm[0] * (uint32_t)P.dm[2];
+        const uint32_t dx = o[4] - o[0], dy = o[2] - o[0];
+        const bool xs = dx == sx || dx == (uint32_t)((int64_t)(1 - P.dm[0]) * P.dm[2]);
+        const bool ys = dy == sy || dy == (uint32_t)((int64_t)(1 - P.dm[1]) * sy);
+        const double2 *__restrict__ d2 = (const double2 *)F.data + 2 * comp;
+        // corner j: x = j >> 2, y = (j >> 1) & 1, z = j & 1
+        const double2 a0 = d2[(size_t)o[0] * 12], a1 = d2[(size_t)o[0] * 12 + 1];
+        const double2 b0 = d2[(size_t)o[1] * 12], b1 = d2[(size_t)o[1] * 12 + 1];
+        dst[0] = a0.x; dst[1] = b0.x; dst[2] = a1.x; dst[3] = b1.x;
+        dst[4] = a0.y; dst[5] = b0.y; dst[6] = a1.y; dst[7] = b1.y;
+        if (!(xs && ys)) {
+            const double *__restrict__ d = (const double *)F.data + 4 * comp;
+#pragma unroll
+            for (int j = 2; j < 8; j++) dst[j] = d[(size_t)o[j] * 24];
+        }
+    }
+
     // component comp of f(s), s given by its component comp on each lane
     __device__ __forceinline__ K operator()(double s)
     {
@@ -330,7 +370,7 @@ struct GroupForce {
         const double pos = __shfl(s, L.own);
         const double u0 = gsh(s, base, 3), u1 = gsh(s, base, 4), u2 = gsh(s, base, 5);
         CornersT<K> c;
-        group_corners<K, GEOM>(P, pos, base, comp, c, FEAT ? xdl : nullptr);
+        group_corners<K, GEOM, ROWS>(P, pos, base, comp, c, FEAT ? xdl : nullptr);
         K f;
         if constexpr (TIME == 1) {
             // the two bracketing snapshots' interpolants blended in time,
@@ -362,6 +402,7 @@ struct GroupForce {
                 loads++;
                 if constexpr (ROWS == ROWS_PAIRS) load_pairs(c.o, v);
                 else if constexpr (ROWS == ROWS_QUADS) load_quads(c.o, v);
+                else if constexpr (ROWS == ROWS_COLUMNS) load_columns(c.o, v);
                 else load(c.o, 0, v);
                 key0 = c.o[0];
                 key7 = c.o[7];
@@ -580,13 +621,16 @@ __device__ __forceinline__ void group_features(const GF &force, const StepParams
 // 512^3 RK5 +22 %).  FP64 static fields in pair rows with the PI
 // controller: three as well (168 registers; RK5 spills 76 B, DOP853 124 B,
 // mostly loop-invariant constants reloaded once per attempt), 6.5 % faster
-// than two at 1024^3 (DOP853, 4,000,000 particles).  The other forms need 180-230 registers
+// than two at 1024^3 (DOP853, 4,000,000 particles); column rows alike.  The other forms need 180-230 registers
 // and run two (FP64 node rows forced to three measured within 1.3 % of two;
 // keeping the FP64 cell cache in LDS instead of registers, to cut the
 // spills, was 5 % slower).
 constexpr int group_waves(int prec, int time, int ctrl, int rows)
 {
-    return time == 0 && ctrl == CTRL_PI && (prec == PTC_FP32 || rows == ROWS_PAIRS) ? 3 : 1;
+    return time == 0 && ctrl == CTRL_PI &&
+                   (prec == PTC_FP32 || rows == ROWS_PAIRS || rows == ROWS_COLUMNS)
+               ? 3
+               : 1;
 }
 
 // The same particle loop as integrate_kernel (static field or snapshot

@@ -197,21 +197,46 @@ __global__ void to_quads_f32_kernel(const double *__restrict__ soa, int dm0, int
     }
 }
 
-// value comp (0..5) of node n of a stored static field, any layout
+// SOA planes (FP64) -> FP64 column rows (192 bytes, stored z fastest):
+// value c of node a of n = (i, j, k), (i+1, j, k), (i, j+1, k), (i+1, j+1, k)
+// (periodic), a = 0..3, at position 4 c + a of row (i + dm0 j) dm2 + k.
+// One thread per row, rows in storage order.
+__global__ void to_columns_f64_kernel(const double *__restrict__ soa, int dm0, int dm1, int dm2,
+                                      int64_t nodes, double *__restrict__ out)
+{
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < nodes;
+         r += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t ij = r / dm2;
+        const int k = (int)(r - ij * dm2);
+        const int i = (int)(ij % dm0), j = (int)(ij / dm0);
+        const int64_t base = (int64_t)k * dm0 * dm1;  // node (0, 0, k)
+        const int i1 = i + 1 == dm0 ? 0 : i + 1, j1 = j + 1 == dm1 ? 0 : j + 1;
+        const int64_t q[4] = {base + i + (int64_t)j * dm0, base + i1 + (int64_t)j * dm0,
+                              base + i + (int64_t)j1 * dm0, base + i1 + (int64_t)j1 * dm0};
+        double *__restrict__ row = out + 24 * r;
+#pragma unroll
+        for (int c = 0; c < 6; c++)
+#pragma unroll
+            for (int a = 0; a < 4; a++) row[4 * c + a] = soa[c * nodes + q[a]];
+    }
+}
+
+// value comp (0..5) of node n (x fastest) of a stored static field, any layout
 __device__ __forceinline__ double stored_value(const DeviceFields &F, int64_t n, int comp)
 {
     const int64_t rv = row_values(F.layout, F.prec);
     const int pos = row_nodes(F.layout) * comp;
     if (F.prec == PTC_FP32) return double(((const float *)F.data)[n * rv + pos]);
     if (F.layout == LAYOUT_SOA) return ((const double *)F.data)[comp * F.nodes + n];
-    return ((const double *)F.data)[n * rv + pos];
+    return ((const double *)F.data)[row_of(F.layout, F.dm, n) * rv + pos];
 }
 
 // Rows of a new static field (layout, prec) from a stored one (any layout
 // and precision), node by node: node rows (FP64: six values; FP32: six, |B|
 // from the stored values in FP64, 0), pair rows (the node, its +x
-// neighbour, zeros) or FP32 quad rows (the node, +x, +y, +x+y, zeros; value
-// c of the row's node a at row_nodes * c + a), the neighbours periodic.  The
+// neighbour, zeros), FP32 quad rows (the node, +x, +y, +x+y, zeros; value
+// c of the row's node a at row_nodes * c + a) or FP64 column rows (as quad
+// rows, no zeros, row row_of(n)), the neighbours periodic.  The
 // same values as building the new form from the stored field's planes.
 __global__ void convert_rows_kernel(const DeviceFields F, void *__restrict__ out, int layout,
                                     int prec)
@@ -228,13 +253,14 @@ __global__ void convert_rows_kernel(const DeviceFields F, void *__restrict__ out
         const int64_t q[4] = {n, base + i1 + (int64_t)j * dm0, base + i + (int64_t)j1 * dm0,
                               base + i1 + (int64_t)j1 * dm0};
         const int nq = row_nodes(layout);
+        const int64_t ro = row_of(layout, F.dm, n);  // the new row of node n
         for (int a = 0; a < 4; a++) {
             if (a >= nq) break;
 #pragma unroll
             for (int c = 0; c < 6; c++) {
                 const double v = stored_value(F, q[a], c);
-                if (prec == PTC_FP32) ((float *)out)[n * rv + nq * c + a] = float(v);
-                else ((double *)out)[n * rv + nq * c + a] = v;
+                if (prec == PTC_FP32) ((float *)out)[ro * rv + nq * c + a] = float(v);
+                else ((double *)out)[ro * rv + nq * c + a] = v;
             }
         }
         if (prec == PTC_FP32 && layout == LAYOUT_AOS) {
@@ -254,13 +280,16 @@ __global__ void convert_rows_kernel(const DeviceFields F, void *__restrict__ out
 // One component of a stored field back to a plane of doubles (the
 // reference's uu(:,:,:,c) / bb(:,:,:,c) order): from the six SOA planes,
 // FP64 node rows, FP32 node rows (eight floats), FP32 pair rows (sixteen
-// floats), FP32 quad rows (32 floats) or FP64 pair rows (16 doubles); in
-// pair and quad rows the row's own node's value comp at row_nodes * comp.
-__global__ void extract_plane_kernel(const void *__restrict__ data, int layout, int prec,
-                                     int64_t nodes, int comp, double *__restrict__ out)
+// floats), FP32 quad rows (32 floats), FP64 pair rows (16 doubles) or FP64
+// column rows (24 doubles, row row_of(i)); in pair, quad and column rows the
+// row's own node's value comp at row_nodes * comp.
+__global__ void extract_plane_kernel(const void *__restrict__ data, int layout, int prec, int dm0,
+                                     int dm1, int dm2, int64_t nodes, int comp,
+                                     double *__restrict__ out)
 {
     const int64_t rv = row_values(layout, prec);
     const int pos = row_nodes(layout) * comp;
+    const int dm[3] = {dm0, dm1, dm2};
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nodes;
          i += (int64_t)gridDim.x * blockDim.x) {
         double v;
@@ -269,7 +298,7 @@ __global__ void extract_plane_kernel(const void *__restrict__ data, int layout, 
         else if (layout == LAYOUT_SOA)
             v = ((const double *)data)[comp * nodes + i];
         else
-            v = ((const double *)data)[i * rv + pos];
+            v = ((const double *)data)[row_of(layout, dm, i) * rv + pos];
         out[i] = v;
     }
 }
@@ -287,7 +316,8 @@ hipError_t launch_extract_plane(const DeviceFields &f, int snap, int comp, doubl
     int64_t blocks = (f.nodes + 255) / 256;
     if (blocks > 65536) blocks = 65536;
     hipLaunchKernelGGL(extract_plane_kernel, dim3((unsigned)blocks), dim3(256), 0, stream,
-                       (const void *)data, f.layout, f.prec, f.nodes, comp, out);
+                       (const void *)data, f.layout, f.prec, f.dm[0], f.dm[1], f.dm[2], f.nodes,
+                       comp, out);
     return hipGetLastError();
 }
 
@@ -303,21 +333,25 @@ hipError_t launch_integrate(int method, const DeviceFields &f, const StepParams 
                             hipStream_t stream)
 {
     if (f.prec == PTC_FP32) return launch_integrate_mixed(method, f, sp, p, t_stop, lc, stream);
-    if (f.layout == LAYOUT_PAIRS) {
-        // FP64 pair rows: static field, PI controller, the trio or group kernel only
+    if (f.layout == LAYOUT_PAIRS || f.layout == LAYOUT_COLUMNS) {
+        // FP64 pair or column rows: static field, PI controller, the trio or
+        // group kernel only
         if (f.nsnap > 0 || lc.ctrl != CTRL_PI || lc.tail != kTailGroupOnly)
             return hipErrorInvalidValue;
         const int g = sp.pow2 ? 1 : 0;
-#define GROUP_P(M, MO, G)                                                      \
-    if (method == M && f.mode == MO && g == G)                                 \
+        const int rows = f.layout == LAYOUT_COLUMNS ? ROWS_COLUMNS : ROWS_PAIRS;
+#define GROUP_R(M, MO, G, R)                                                   \
+    if (method == M && f.mode == MO && g == G && rows == R)                    \
         return lc.lanes == 3                                                   \
-                   ? launch_trio_t<M, MO, G, PTC_FP64, ROWS_PAIRS>(f, sp, p, t_stop, kBlock, stream) \
-                   : launch_group_t<M, MO, G, PTC_FP64, 0, CTRL_PI, ROWS_PAIRS>(f, sp, p, t_stop, kBlock, stream);
+                   ? launch_trio_t<M, MO, G, PTC_FP64, R>(f, sp, p, t_stop, kBlock, stream) \
+                   : launch_group_t<M, MO, G, PTC_FP64, 0, CTRL_PI, R>(f, sp, p, t_stop, kBlock, stream);
+#define GROUP_P(M, MO, G) GROUP_R(M, MO, G, ROWS_PAIRS) GROUP_R(M, MO, G, ROWS_COLUMNS)
         GROUP_P(PTC_RK5, PTC_MODE_VB, 0) GROUP_P(PTC_RK5, PTC_MODE_EB, 0)
         GROUP_P(PTC_DOP853, PTC_MODE_VB, 0) GROUP_P(PTC_DOP853, PTC_MODE_EB, 0)
         GROUP_P(PTC_RK5, PTC_MODE_VB, 1) GROUP_P(PTC_RK5, PTC_MODE_EB, 1)
         GROUP_P(PTC_DOP853, PTC_MODE_VB, 1) GROUP_P(PTC_DOP853, PTC_MODE_EB, 1)
 #undef GROUP_P
+#undef GROUP_R
         return hipErrorInvalidValue;
     }
     if (f.nsnap > 0) {
@@ -435,8 +469,7 @@ hipError_t launch_probe(const DeviceFields &f, const StepParams &sp, int op, int
     if (n <= 0) return hipSuccess;
     if (op < 0 || op > 7) return hipErrorInvalidValue;
     if (op >= 5 && f.layout != LAYOUT_AOS) return hipErrorInvalidValue;
-    if (f.layout == LAYOUT_PAIRS || f.layout == LAYOUT_QUADS)
-        return hipErrorInvalidValue;  // the probes read node rows
+    if (rows_layout(f.layout)) return hipErrorInvalidValue;  // the probes read node rows
     if (f.prec == PTC_FP32) return launch_probe_mixed(f, sp, op, n, in, out, stream);
     if (f.mode == PTC_MODE_VB && f.layout == LAYOUT_SOA)
         return launch_probe_t<PTC_MODE_VB, LAYOUT_SOA>(f, sp, op, n, in, out, stream);
@@ -512,7 +545,8 @@ hipError_t launch_convert_rows(const DeviceFields &f, void *out, int layout, int
                                hipStream_t stream)
 {
     if (f.nsnap > 0 || f.layout == LAYOUT_BRICK || layout == LAYOUT_SOA ||
-        layout == LAYOUT_BRICK || (layout == LAYOUT_QUADS && prec != PTC_FP32))
+        layout == LAYOUT_BRICK || (layout == LAYOUT_QUADS && prec != PTC_FP32) ||
+        (layout == LAYOUT_COLUMNS && prec != PTC_FP64))
         return hipErrorInvalidValue;
     int64_t blocks = (f.nodes + 255) / 256;
     if (blocks > 65536) blocks = 65536;
@@ -538,6 +572,16 @@ hipError_t launch_to_quads_f32(const double *soa, const int dm[3], float *out, h
     if (blocks > 65536) blocks = 65536;
     hipLaunchKernelGGL(to_quads_f32_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, soa,
                        dm[0], dm[1], nodes, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_to_columns_f64(const double *soa, const int dm[3], double *out, hipStream_t stream)
+{
+    const int64_t nodes = (int64_t)dm[0] * dm[1] * dm[2];
+    int64_t blocks = (nodes + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(to_columns_f64_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, soa,
+                       dm[0], dm[1], dm[2], nodes, out);
     return hipGetLastError();
 }
 

@@ -10,10 +10,11 @@ namespace ptc {
 struct MlpPacked;
 
 enum Layout : int {
-    LAYOUT_SOA = 0, LAYOUT_AOS = 1, LAYOUT_BRICK = 2, LAYOUT_PAIRS = 3, LAYOUT_QUADS = 4
+    LAYOUT_SOA = 0, LAYOUT_AOS = 1, LAYOUT_BRICK = 2, LAYOUT_PAIRS = 3, LAYOUT_QUADS = 4,
+    LAYOUT_COLUMNS = 5
 };
 // the group kernel's row forms (its ROWS template argument)
-enum : int { ROWS_NODES = 0, ROWS_PAIRS = 1, ROWS_QUADS = 2 };
+enum : int { ROWS_NODES = 0, ROWS_PAIRS = 1, ROWS_QUADS = 2, ROWS_COLUMNS = 3 };
 
 // Device copy of one field snapshot.
 //   SOA: six planes of dm1*dm2*dm3 values (U/E x,y,z then B x,y,z), each x
@@ -28,6 +29,12 @@ enum : int { ROWS_NODES = 0, ROWS_PAIRS = 1, ROWS_QUADS = 2 };
 //        of the node, of its +x, +y and +x+y neighbours (periodic), then 8
 //        zeros; a cell's eight corners are two aligned 128-byte rows (the
 //        group kernel).
+//   COLUMNS (FP64 only): per node a 192-byte row of 24 doubles, the six
+//        values of the node, of its +x, +y and +x+y neighbours (periodic),
+//        value c of node a at 4 c + a, no padding; the rows stored z
+//        fastest (row of node (i, j, k) at (i + dm1 j) dm3 + k), so the two
+//        rows of a cell, planes k and k + 1, are one 384-byte span: three
+//        128-byte lines when k is even, four when it is odd.
 //   BRICK: AOS nodes plus, per cell, the eight corner rows of that cell in
 //        the reference's corner order (48 values, 384 bytes, three 128-byte
 //        lines) -- eight copies of every node, so a gather reads one
@@ -151,10 +158,15 @@ hipError_t launch_to_aos(const double *soa, int64_t nodes, void *aos, int prec, 
 hipError_t launch_to_pairs_f32(const double *soa, const int dm[3], float *out, hipStream_t stream);
 hipError_t launch_to_pairs_f64(const double *soa, const int dm[3], double *out, hipStream_t stream);
 hipError_t launch_to_quads_f32(const double *soa, const int dm[3], float *out, hipStream_t stream);
+hipError_t launch_to_columns_f64(const double *soa, const int dm[3], double *out, hipStream_t stream);
 // values per node row of a stored layout (node rows: 6 FP64 or 8 FP32)
 constexpr int row_values(int layout, int prec)
 {
-    return layout == LAYOUT_QUADS ? 32 : layout == LAYOUT_PAIRS ? 16 : prec == 1 ? 8 : 6;
+    return layout == LAYOUT_QUADS     ? 32
+           : layout == LAYOUT_COLUMNS ? 24
+           : layout == LAYOUT_PAIRS   ? 16
+           : prec == 1                ? 8
+                                      : 6;
 }
 // nodes per row: a pair row holds node n and its +x neighbour, a quad row
 // node n and its +x, +y and +x+y neighbours; value c of the row's node a
@@ -163,11 +175,25 @@ constexpr int row_values(int layout, int prec)
 // row's own node's value c at row_nodes * c
 constexpr int row_nodes(int layout)
 {
-    return layout == LAYOUT_QUADS ? 4 : layout == LAYOUT_PAIRS ? 2 : 1;
+    return layout == LAYOUT_QUADS || layout == LAYOUT_COLUMNS ? 4 : layout == LAYOUT_PAIRS ? 2 : 1;
+}
+// the row of node n (x fastest) in a stored layout: n itself, except in
+// column rows, which are stored z fastest
+__host__ __device__ inline int64_t row_of(int layout, const int dm[3], int64_t n)
+{
+    if (layout != LAYOUT_COLUMNS) return n;
+    const int64_t plane = (int64_t)dm[0] * dm[1];
+    const int64_t k = n / plane, ij = n - k * plane;
+    return ij * dm[2] + k;
+}
+// layouts whose rows only the group and trio kernels read (static fields)
+constexpr bool rows_layout(int layout)
+{
+    return layout == LAYOUT_PAIRS || layout == LAYOUT_QUADS || layout == LAYOUT_COLUMNS;
 }
 // AOS nodes -> per-cell corner bricks.
 hipError_t launch_to_bricks(const DeviceFields &f, hipStream_t stream);
-// Rows of a static field in another layout (node, pair or quad rows) and
+// Rows of a static field in another layout (node, pair, quad or column rows) and
 // precision from a stored static field (not bricks); out holds
 // row_values(layout, prec) values per node.
 hipError_t launch_convert_rows(const DeviceFields &f, void *out, int layout, int prec,

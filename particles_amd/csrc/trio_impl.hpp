@@ -41,6 +41,7 @@ constexpr int kTriosPerWave = 64 / kTrio;
 template <int MODE, int GEOM, int PREC, int ROWS, int FEAT = 0>
 struct TrioForce {
     static_assert(ROWS != ROWS_QUADS || PREC == PTC_FP32, "quad rows hold FP32 fields");
+    static_assert(ROWS != ROWS_COLUMNS || PREC == PTC_FP64, "column rows hold FP64 fields");
     using K = std::conditional_t<PREC == PTC_FP32, float, double>;
     const DeviceFields &F;
     const StepParams &P;
@@ -123,16 +124,43 @@ struct TrioForce {
         }
     }
 
+    // column rows: as GroupForce::load_columns, for both components
+    __device__ __forceinline__ void load_columns(const uint32_t *o)
+    {
+        const uint32_t sx = (uint32_t)P.dm[2], sy = (uint32_t)P.dm[0] * (uint32_t)P.dm[2];
+        const uint32_t dx = o[4] - o[0], dy = o[2] - o[0];
+        const bool xs = dx == sx || dx == (uint32_t)((int64_t)(1 - P.dm[0]) * P.dm[2]);
+        const bool ys = dy == sy || dy == (uint32_t)((int64_t)(1 - P.dm[1]) * sy);
+        const double2 *__restrict__ d2 = (const double2 *)F.data;
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const int c = a + 3 * h;
+            const double2 a0 = d2[(size_t)o[0] * 12 + 2 * c], a1 = d2[(size_t)o[0] * 12 + 2 * c + 1];
+            const double2 b0 = d2[(size_t)o[1] * 12 + 2 * c], b1 = d2[(size_t)o[1] * 12 + 2 * c + 1];
+            v[h][0] = a0.x; v[h][1] = b0.x; v[h][2] = a1.x; v[h][3] = b1.x;
+            v[h][4] = a0.y; v[h][5] = b0.y; v[h][6] = a1.y; v[h][7] = b1.y;
+        }
+        if (!(xs && ys)) {
+            const double *__restrict__ d = (const double *)F.data;
+#pragma unroll
+            for (int h = 0; h < 2; h++)
+#pragma unroll
+                for (int j = 2; j < 8; j++)
+                    v[h][j] = d[(size_t)o[j] * 24 + 4 * (a + 3 * h)];
+        }
+    }
+
     // components a and a + 3 of f(s), s given by its components a and a + 3
     // on each lane
     __device__ __forceinline__ void operator()(const double *s, K *out)
     {
         CornersT<K> c;
-        group_corners<K, GEOM>(P, s[0], base, a, c, FEAT ? xdl : nullptr);
+        group_corners<K, GEOM, ROWS>(P, s[0], base, a, c, FEAT ? xdl : nullptr);
         if (c.o[0] != key0 || c.o[7] != key7) {
             loads++;
             if constexpr (ROWS == ROWS_PAIRS) load_pairs(c.o);
             else if constexpr (ROWS == ROWS_QUADS) load_quads(c.o);
+            else if constexpr (ROWS == ROWS_COLUMNS) load_columns(c.o);
             else load_nodes(c.o);
             key0 = c.o[0];
             key7 = c.o[7];

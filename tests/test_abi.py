@@ -68,6 +68,12 @@ def test_options_roundtrip_and_validation():
         assert P.get_option("f32_layout") == "quads"
         P.set_option("layout", "pairs")
         assert P.get_option("layout") == "pairs"
+        P.set_option("layout", "columns")
+        assert P.get_option("layout") == "columns"
+        for bad in ("column", "quads", ""):
+            with pytest.raises(P.PtcError):
+                P.set_option("layout", bad)
+        assert P.get_option("layout") == "columns"
         assert P.get_option("group_lanes") == "auto"
         P.set_option("group_lanes", "3")
         assert P.get_option("group_lanes") == "3"

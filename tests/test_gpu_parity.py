@@ -697,8 +697,8 @@ def test_handover_counts_retired_particles(oracle, lanes):
     assert np.array_equal(c2, co) and np.array_equal(s2, so, equal_nan=True)
 
 
-@pytest.mark.parametrize("form", ["aos", "soa", "brick", "pairs", "fp32 nodes", "fp32 pairs",
-                                  "fp32 quads", "series"])
+@pytest.mark.parametrize("form", ["aos", "soa", "brick", "pairs", "columns", "fp32 nodes",
+                                  "fp32 pairs", "fp32 quads", "series"])
 def test_fields_download_returns_stored_values(form):
     """ptc_fields_download gives back what each storage form holds: FP64
     layouts exactly the uploaded arrays, FP32 ones the arrays rounded to
@@ -709,7 +709,7 @@ def test_fields_download_returns_stored_values(form):
     bb = np.asfortranarray(rng.normal(size=dm + (3,)))
     old = P.get_option("layout")
     try:
-        if form in ("aos", "soa", "brick", "pairs"):
+        if form in ("aos", "soa", "brick", "pairs", "columns"):
             P.set_option("layout", form)
             with P.Fields(uu, bb) as f:
                 u2, b2 = f.download()
@@ -731,8 +731,8 @@ def test_fields_download_returns_stored_values(form):
 
 
 @pytest.mark.parametrize("target", ["fp32 nodes", "fp32 pairs", "fp32 quads", "fp64 pairs",
-                                    "fp64 aos"])
-@pytest.mark.parametrize("source", ["aos", "pairs", "soa"])
+                                    "fp64 columns", "fp64 aos"])
+@pytest.mark.parametrize("source", ["aos", "pairs", "columns", "soa"])
 def test_fields_convert_equals_direct_creation(source, target):
     """ptc_fields_convert builds, on the device, the same stored values as
     creating the target form from the host arrays (FP32: the values rounded
@@ -813,7 +813,7 @@ def _fields_fp32(uu, bb, layout, **kw):
 def _fields_rows(uu, bb, form, **kw):
     """Fields in one of the group kernel's row forms: "nodes" / "fp64
     nodes" (node rows), "pairs" (FP32 pair rows), "quads" (FP32 quad rows),
-    "fp64 pairs"."""
+    "fp64 pairs", "fp64 columns" (FP64 quad rows stored z fastest)."""
     if form.startswith("fp64"):
         P.set_option("layout", form.split()[1].replace("nodes", "aos"))
         try:
@@ -823,20 +823,21 @@ def _fields_rows(uu, bb, form, **kw):
     return _fields_fp32(uu, bb, form, **kw)
 
 
-@pytest.mark.parametrize("form", ["pairs", "quads", "fp64 pairs"])
+@pytest.mark.parametrize("form", ["pairs", "quads", "fp64 pairs", "fp64 columns"])
 @pytest.mark.parametrize("method", ["rk5", "dop853"])
 def test_pair_rows_match_node_rows(method, form):
     """Pair rows (a node and its +x neighbour in one row: 64 bytes in FP32,
-    128 in FP64) and FP32 quad rows (a node and its +x, +y, +x+y
-    neighbours in one 128-byte row) hold the same values as node rows, and
-    the group kernel reads a right corner from the left corner's row only
-    where the right index is the left one's periodic successor: snapshots
-    and counters identical to node rows with the group kernel (FP64: to the
-    oracle), in the unit box (power-of-two geometry), on a 12 x 10 x 8 grid
-    in a shifted box, in EB mode, with particles started exactly on x and y
-    cell faces (where the separately rounded right index can differ) and
-    across the periodic x and y boundaries; every "tail" setting runs the
-    group kernel on these rows."""
+    128 in FP64), FP32 quad rows (a node and its +x, +y, +x+y neighbours
+    in one 128-byte row) and FP64 column rows (the same four nodes in a
+    192-byte row, rows stored z fastest) hold the same values as node rows,
+    and the group kernel reads a right corner from the left corner's row
+    only where the right index is the left one's periodic successor:
+    snapshots and counters identical to node rows with the group kernel
+    (FP64: to the oracle), in the unit box (power-of-two geometry), on a
+    12 x 10 x 8 grid in a shifted box, in EB mode, with particles started
+    exactly on x, y and z cell faces (where the separately rounded right
+    index can differ) and across the periodic boundaries; every "tail"
+    setting runs the group kernel on these rows."""
     uu, bb = W.turbulent_vb(32, seed=51, nmodes=64)
     p = W.params_vector(tmax=1 * W.T0, dtmax=1.0, atol=1e-8, rtol=1e-8)
     tg = W.time_grid(3, p[2])
@@ -847,6 +848,9 @@ def test_pair_rows_match_node_rows(method, form):
     st0[1, 0, 260:460] = rng.integers(0, 32, 200) / 32.0  # on y faces
     st0[1, 0, 460:520] = 1.0 - 1e-12  # last cell in y
     st0[:2, 0, 520:560] = 1.0 - 1e-12  # last cell in x and y
+    st0[2, 0, 560:760] = rng.integers(0, 32, 200) / 32.0  # on z faces
+    st0[2, 0, 760:800] = 1.0 - 1e-12  # last cell in z
+    st0[:3, 0, 800:840] = 1.0 - 1e-12  # last cell in x, y and z
     cases = [("unit box", uu, bb, {}, p, st0)]
     ug, bg = W.turbulent_vb(32, seed=54, nmodes=64)
     ug, bg = np.asfortranarray(ug[:12, :10, :8]), np.asfortranarray(bg[:12, :10, :8])
@@ -856,6 +860,7 @@ def test_pair_rows_match_node_rows(method, form):
                          xl=(-1.0, 0.5, 2.0), xs=(2.5, 3.0, 1.75))
     sg[0, 0, :200] = -1.0 + 2.5 * rng.integers(0, 12, 200) / 12.0
     sg[1, 0, 200:400] = 0.5 + 3.0 * rng.integers(0, 10, 200) / 10.0
+    sg[2, 0, 400:600] = 2.0 + 1.75 * rng.integers(0, 8, 200) / 8.0
     cases.append(("12x10x8 shifted box", ug, bg, {}, g, sg))
     ee, be = W.reconnection_eb(32, seed=56, nmodes=64)
     cases.append(("EB", ee, be, {"mode": "eb"}, p, st0))
@@ -876,7 +881,8 @@ def test_pair_rows_match_node_rows(method, form):
         P.set_option("tail", "auto")
 
 
-@pytest.mark.parametrize("form", ["fp64 nodes", "fp64 pairs", "nodes", "pairs", "quads"])
+@pytest.mark.parametrize("form", ["fp64 nodes", "fp64 pairs", "fp64 columns", "nodes", "pairs",
+                                  "quads"])
 @pytest.mark.parametrize("method", ["rk5", "dop853"])
 def test_trio_kernel_matches_group_kernel(method, form):
     """The trio kernel (three lanes per particle, group_lanes 3) against
@@ -889,7 +895,7 @@ def test_trio_kernel_matches_group_kernel(method, form):
     expressions, which both kernels write alike).  VB in the unit box, EB in a shifted box with
     particles on x and y cell faces and in the last cells, so that both
     the pair and quad rows' successor rows and the per-corner fallback are
-    read."""
+    read (column rows: the z-right rows as well)."""
     uu, bb = W.turbulent_vb(24, seed=71, nmodes=48)
     p = W.params_vector(tmax=1 * W.T0, dtmax=1.0, atol=1e-8, rtol=1e-8)
     tg = W.time_grid(3, p[2])
@@ -898,6 +904,8 @@ def test_trio_kernel_matches_group_kernel(method, form):
     st0[0, 0, :100] = rng.integers(0, 24, 100) / 24.0
     st0[1, 0, 100:200] = rng.integers(0, 24, 100) / 24.0
     st0[:2, 0, 200:240] = 1.0 - 1e-12
+    st0[2, 0, 240:340] = rng.integers(0, 24, 100) / 24.0
+    st0[:3, 0, 340:360] = 1.0 - 1e-12
     ee, be = W.reconnection_eb(24, seed=74, nmodes=48)
     g = W.params_vector(vun=0.5, tmax=1 * W.T0, dtmax=1.0, atol=1e-8, rtol=1e-8,
                         xl=(-1.0, 0.5, 2.0), xs=(2.5, 3.0, 1.75))
@@ -925,13 +933,13 @@ def test_trio_kernel_matches_group_kernel(method, form):
         P.set_option("group_lanes", "auto")
 
 
-@pytest.mark.parametrize("form", ["pairs", "quads", "fp64 pairs"])
+@pytest.mark.parametrize("form", ["pairs", "quads", "fp64 pairs", "fp64 columns"])
 def test_pair_rows_refused_where_node_rows_are_read(form):
     """Probes, record mode and snapshot series read node rows: they refuse
-    pair and quad rows with an error; the learned controller runs on FP32
-    pair and quad rows (group kernel) with the results it gives on node
-    rows, and refuses FP64 pair rows (it runs FP64 fields one lane per
-    particle)."""
+    pair, quad and column rows with an error; the learned controller runs
+    on FP32 pair and quad rows (group kernel) with the results it gives on
+    node rows, and refuses FP64 pair and column rows (it runs FP64 fields
+    one lane per particle)."""
     from oracle import mlp_oracle as M
     uu, bb = W.turbulent_vb(16, seed=57, nmodes=32)
     p = W.params_vector(tmax=0.5 * W.T0, dtmax=1.0, atol=1e-8, rtol=1e-8)
