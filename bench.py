@@ -62,6 +62,14 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 BYTES_PER_LOAD = {"fp64": 8 * 6 * 8, "fp32": 8 * 6 * 4}
+# stored bytes per node of each field form, and the most the FP64 field and
+# an FP32 copy may hold together on the device (MI355X: 288 GB) before the
+# FP64 field is released for the FP32 runs
+NODE_BYTES = {("fp64", "aos"): 48, ("fp64", "soa"): 48, ("fp64", "brick"): 432,
+              ("fp64", "pairs"): 128, ("fp64", "columns"): 192,
+              ("fp32", "nodes"): 32, ("fp32", "pairs"): 64, ("fp32", "quads"): 128,
+              ("fp32", "columns"): 128}
+FIELDS_TOGETHER_BYTES = 200 << 30
 # FP64 operations per RHS evaluation (cell lookup ~30, weights 16, corner
 # sums 96, Lorentz force ~30) and per attempted step beyond the RHS (stage
 # sums, solution, error estimate and control): Cash-Karp 6 stages, DOP853 12
@@ -235,14 +243,21 @@ def traffic_from_profile(cfg):
 
 
 def _norm(cfg, key, defaults):
-    """A configuration value for matching; the tail setting only selects
-    kernels for node rows ("aos"), elsewhere it is moot."""
-    if key == "tail" and cfg.get("layout") != "aos":
+    """A configuration value for matching: the FP64 storage is moot for
+    FP32 runs and the FP32 storage for FP64 runs; the tail setting only
+    selects kernels for node rows (layout aos, f32_layout nodes), elsewhere
+    it is moot."""
+    fp32 = cfg.get("field_precision") == "fp32"
+    node_rows = (cfg.get("f32_layout", defaults["f32_layout"]) == "nodes" if fp32
+                 else cfg.get("layout") == "aos")
+    if key == "layout" and fp32:
         return None
-    if key == "f32_layout" and cfg.get("field_precision") != "fp32":
-        return None  # FP64 fields ignore it
+    if key == "f32_layout" and not fp32:
+        return None
+    if key == "tail" and not node_rows:
+        return None
     if key == "group_lanes" and (cfg.get("field_snapshots", 0) > 0 or
-                                 (cfg.get("layout") == "aos" and str(cfg.get("tail")) == "0")):
+                                 (node_rows and str(cfg.get("tail")) == "0")):
         return None  # snapshot series and one-lane runs have no trio form
     return cfg.get(key, defaults.get(key))
 
@@ -320,14 +335,15 @@ def summary(r, npart, steps, prec, method, plain_tps=None, mixed_tps=None, traff
     return out
 
 
-def _fp32_field(fields64, make_fields, layout):
+def _fp32_field(fields64, make_fields, layout, rebuild=False):
     """The FP32 field in `layout`: converted on the device from the FP64
     field where its storage allows (no second synthesis, no plane copy),
-    else built again."""
+    else, or with `rebuild`, built again (the same values: the same doubles
+    rounded to float)."""
     import particles_amd as P
     P.set_option("f32_layout", layout)
     try:
-        if fields64.prec == "fp64" and P.get_option("layout") != "brick":
+        if not rebuild and fields64.prec == "fp64" and P.get_option("layout") != "brick":
             return fields64.convert("fp32")
         return make_fields("fp32")
     finally:
@@ -402,7 +418,16 @@ def variants(args, fields64, make_fields, p, tg, st0, method, plain, cfg):
                           f"{args.train_horizon:g} T0, PI controller, FP32 node rows")
     ref_cal = reference_states(fields64, method, p, st_train, horizon)
 
-    f32 = _fp32_field(fields64, make_fields, args.f32_layout)
+    # the FP32 run field; where it would not fit beside the FP64 field
+    # (1024^3: 192 GiB of column rows and 128 GiB of quad rows) the FP64
+    # field is released first, its runs being done
+    nodes = cfg["grid"] ** 3
+    together = nodes * (NODE_BYTES.get(("fp64", args.layout), 48) +
+                        NODE_BYTES.get(("fp32", args.f32_layout), 32))
+    release = together > FIELDS_TOGETHER_BYTES
+    if release:
+        fields64.close()
+    f32 = _fp32_field(fields64, make_fields, args.f32_layout, rebuild=release)
     r = run_batch(f32, method, p, tg, st0, args.steps, args.warmup, label="mixed")
     out["mixed"] = summary(r, npart, args.steps, "fp32", method, plain_tps,
                            traffic=traffic_from_profile(dict(cfg, field_precision="fp32")))
@@ -469,13 +494,15 @@ def main():
     ap.add_argument("--grid", type=int, default=0)
     ap.add_argument("--method", default="", choices=["", "rk5", "dop853"])
     ap.add_argument("--layout", default=None, choices=["aos", "soa", "brick", "pairs", "columns"],
-                    help="FP64 field storage (default: pairs for reconnection1024, 128-byte "
-                         "rows of a node and its +x neighbour; aos node rows elsewhere)")
-    ap.add_argument("--f32-layout", default=None, choices=["nodes", "pairs", "quads"],
+                    help="FP64 field storage (default: columns, 192-byte rows of a node and "
+                         "its +x, +y, +x+y neighbours stored z fastest; aos node rows for "
+                         "turbulent64 and snapshot series)")
+    ap.add_argument("--f32-layout", default=None, choices=["nodes", "pairs", "quads", "columns"],
                     help="FP32 fields (--prec fp32 and the mixed / learned-h variants): "
-                         "32-byte node rows, 64-byte rows of a node and its +x neighbour, or "
-                         "128-byte rows of a node and its +x, +y, +x+y neighbours (default: "
-                         "quads for reconnection1024, nodes elsewhere)")
+                         "32-byte node rows, 64-byte rows of a node and its +x neighbour, "
+                         "128-byte rows of a node and its +x, +y, +x+y neighbours (quads), or "
+                         "those stored z fastest (columns; the default for the reconnection "
+                         "workloads, nodes elsewhere)")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--cell-cache", type=int, default=1, choices=[0, 1])
     ap.add_argument("--tail", default="auto",
@@ -526,11 +553,14 @@ def main():
                     help="only copy a 1 GiB float64 device tensor three times (the known-size "
                          "copy that scales the rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
-    big = args.workload == "reconnection1024"
+    # column rows (the trio kernel) wherever they measured fastest: every
+    # static FP64 field but the 64^3 one, and FP32 on the reconnection grids
+    # (node rows with the lane-per-particle kernel stay ahead on 256^3)
     if args.layout is None:
-        args.layout = "pairs" if big else "aos"
+        args.layout = ("aos" if args.snapshots > 0 or args.workload == "turbulent64"
+                       else "columns")
     if args.f32_layout is None:
-        args.f32_layout = "quads" if big else "nodes"
+        args.f32_layout = "columns" if args.workload.startswith("reconnection") else "nodes"
     if args.copy_bandwidth:
         import torch
         a = torch.ones(1 << 29, dtype=torch.float64, device="cuda")
@@ -639,12 +669,7 @@ def main():
         print("host buffers ...", file=sys.stderr, flush=True)
         hostbuf = host_buffer_rate(fields, method, p, st0, args.host_buffers)
 
-    var = None
-    if not args.no_variants and args.prec == "fp64" and args.snapshots == 0:
-        print("variants ...", file=sys.stderr, flush=True)
-        var = variants(args, fields, make_fields, p, tg, st0, method, dict(acc=acc, wall=wall),
-                       cfg)
-
+    # the CPU baselines first: the variants may release the FP64 field
     cpu = cpu_oracle = None
     if not args.no_cpu and args.snapshots == 0:
         print("cpu baseline ...", file=sys.stderr, flush=True)
@@ -661,6 +686,12 @@ def main():
             vpl[:, :, :, 1] = -0.1 * W.sheet_profile(np.arange(dm) / dm, 0.02)[None, :, None]
         cpu, cpu_oracle = cpu_baselines(f1h, bbh, vpl, p, st0, method, args.cpu_seconds, eb)
         del f1h, bbh, vpl
+
+    var = None
+    if not args.no_variants and args.prec == "fp64" and args.snapshots == 0:
+        print("variants ...", file=sys.stderr, flush=True)
+        var = variants(args, fields, make_fields, p, tg, st0, method, dict(acc=acc, wall=wall),
+                       cfg)
 
     out = {
         "metric": "accepted particle-steps/s on 1 MI355X at tol=1e-8, plain vs learned-h",

@@ -109,7 +109,7 @@ struct Options {
     int sort = 1;   // re-order the work queue every `sort` advances (0 = never)
     int order = 3;  // the order then: 1 spatial (Morton), 2 cost (most work first), 3 auto
     int tail = kTailAuto;  // LaunchConfig::tail, or chosen per batch
-    int f32_layout = LAYOUT_AOS;  // FP32 fields: node rows, LAYOUT_PAIRS or LAYOUT_QUADS
+    int f32_layout = LAYOUT_AOS;  // FP32 fields: node rows, LAYOUT_PAIRS, _QUADS or _COLUMNS
     int lanes = 0;                // LaunchConfig::lanes, or 0: chosen per batch (choose_lanes)
 };
 
@@ -201,7 +201,9 @@ int ptc_set_option(const char *key, const char *value)
         if (!strcmp(value, "nodes")) g_opt.f32_layout = LAYOUT_AOS;
         else if (!strcmp(value, "pairs")) g_opt.f32_layout = LAYOUT_PAIRS;
         else if (!strcmp(value, "quads")) g_opt.f32_layout = LAYOUT_QUADS;
-        else return fail(PTC_EARG, "f32_layout must be nodes, pairs or quads (got %s)", value);
+        else if (!strcmp(value, "columns")) g_opt.f32_layout = LAYOUT_COLUMNS;
+        else return fail(PTC_EARG, "f32_layout must be nodes, pairs, quads or columns (got %s)",
+                         value);
     } else if (!strcmp(key, "max_steps")) {
         long long v = strtoll(value, &end, 10);
         if (*end || v < 0) return fail(PTC_EARG, "bad max_steps %s", value);
@@ -256,9 +258,10 @@ int ptc_get_option(const char *key, char *buf, int len)
                                  : o.layout == LAYOUT_COLUMNS ? "columns"
                                                             : "brick");
     else if (!strcmp(key, "f32_layout"))
-        snprintf(buf, len, "%s", o.f32_layout == LAYOUT_PAIRS   ? "pairs"
-                                 : o.f32_layout == LAYOUT_QUADS ? "quads"
-                                                                : "nodes");
+        snprintf(buf, len, "%s", o.f32_layout == LAYOUT_PAIRS     ? "pairs"
+                                 : o.f32_layout == LAYOUT_QUADS   ? "quads"
+                                 : o.f32_layout == LAYOUT_COLUMNS ? "columns"
+                                                                  : "nodes");
     else if (!strcmp(key, "max_steps")) snprintf(buf, len, "%lld", (long long)o.max_steps);
     else if (!strcmp(key, "cell_cache")) snprintf(buf, len, "%d", o.cell_cache);
     else if (!strcmp(key, "sort")) snprintf(buf, len, "%d", o.sort);
@@ -287,7 +290,7 @@ static int fields_from_device_soa(const int dm[3], double *soa, int mode, int pr
 {
     const int64_t nodes = (int64_t)dm[0] * dm[1] * dm[2];
     Options o = options();
-    if (prec == PTC_FP32) o.layout = o.f32_layout;  // FP32: node, pair or quad rows
+    if (prec == PTC_FP32) o.layout = o.f32_layout;  // FP32: node, pair, quad or column rows
     void *data = soa, *bricks = nullptr;
     hipError_t e = hipSuccess;
     if (rows_layout(o.layout)) {
@@ -296,6 +299,8 @@ static int fields_from_device_soa(const int dm[3], double *soa, int mode, int pr
         e = hipMalloc(&rows, esz * (size_t)row_values(o.layout, prec) * (size_t)nodes);
         if (e == hipSuccess) {
             if (o.layout == LAYOUT_QUADS) e = launch_to_quads_f32(soa, dm, (float *)rows, nullptr);
+            else if (o.layout == LAYOUT_COLUMNS && prec == PTC_FP32)
+                e = launch_to_columns_f32(soa, dm, (float *)rows, nullptr);
             else if (o.layout == LAYOUT_COLUMNS) e = launch_to_columns_f64(soa, dm, (double *)rows, nullptr);
             else if (prec == PTC_FP32) e = launch_to_pairs_f32(soa, dm, (float *)rows, nullptr);
             else e = launch_to_pairs_f64(soa, dm, (double *)rows, nullptr);
@@ -529,7 +534,7 @@ int ptc_fields_convert(const ptc_fields *src, int prec, ptc_fields **out)
     if (layout == LAYOUT_SOA || layout == LAYOUT_BRICK)
         return fail(PTC_EARG, "ptc_fields_convert: the target layout must be node, pair, quad "
                               "or column rows (layout aos, pairs or columns; f32_layout nodes, "
-                              "pairs or quads)");
+                              "pairs, quads or columns)");
     const size_t esz = prec == PTC_FP32 ? sizeof(float) : sizeof(double);
     void *rows = nullptr;
     HIP_TRY(hipMalloc(&rows, esz * (size_t)row_values(layout, prec) * (size_t)s.nodes));
@@ -647,15 +652,20 @@ static int resolve_tail(int tail, const ptc_particles *p)
 
 // Lanes per particle of the kernel that runs a static field with the PI
 // controller where particles share a wave's lanes ("group_lanes" auto,
-// measured on the MI355X, both giving the same FP64 results): three (the
-// trio kernel) for FP32 DOP853 batches of at least 2,000,000 particles
-// (1024^3, 4,000,000 particles: 408 against 472 ms per gyro-period), six
-// elsewhere (the trio kernel measured 6-13 % slower on 512^3 RK5 and FP32
-// DOP853 batches of 1,000,000 particles and on FP64 at 1024^3, whose
-// 256-register form spills).
+// measured on the MI355X, both giving the same results): three (the trio
+// kernel) for column rows (FP64: 1.24-1.30x the six-lane kernel on 256^3
+// DOP853 with 100,000 particles, 512^3 RK5 with 1,000,000, 1024^3 DOP853
+// and RK5 with 1,000,000 and 4,000,000; FP32: 1.32x on 256^3 DOP853, 1.25x
+// on 512^3 RK5, 1.36x on 1024^3 DOP853) and for FP32 DOP853 batches of at
+// least 2,000,000 particles on other rows (1024^3, 4,000,000 particles on
+// quad rows: 408 against 472 ms per gyro-period); six elsewhere (the trio
+// kernel measured 6-13 % slower on FP32 quad rows with 512^3 RK5 and with
+// DOP853 batches of 1,000,000 particles, and 4-7 % slower on FP64 pair rows
+// at 1024^3).
 static int choose_lanes(int lanes, const DeviceFields &f, int method, int64_t np)
 {
     if (lanes != 0) return lanes;
+    if (f.layout == LAYOUT_COLUMNS) return 3;
     return f.prec == PTC_FP32 && method == PTC_DOP853 && np >= 2000000 ? 3 : 6;
 }
 
@@ -1402,11 +1412,10 @@ int ptc_particles_use_mlp(ptc_particles *p, const ptc_mlp *net, double shift)
         return PTC_OK;
     }
     const DeviceFields &fd = p->fields->d;
-    const bool rows_ok = fd.layout == LAYOUT_AOS ||
-                         (fd.prec == PTC_FP32 && (fd.layout == LAYOUT_PAIRS || fd.layout == LAYOUT_QUADS));
+    const bool rows_ok = fd.layout == LAYOUT_AOS || (fd.prec == PTC_FP32 && rows_layout(fd.layout));
     if (!rows_ok || fd.nsnap > 0)
         return fail(PTC_ESTATE, "ptc_particles_use_mlp: needs a static field in the aos layout "
-                                "(or FP32 pair or quad rows)");
+                                "(or FP32 pair, quad or column rows)");
     if (!net->trained)
         return fail(PTC_ESTATE, "ptc_particles_use_mlp: the network still has its random "
                                 "initialisation (train it or set its parameters)");

@@ -208,16 +208,15 @@ __device__ __forceinline__ void group_corners(const StepParams &P, double pos, i
 // own rows).  ROWS = ROWS_QUADS (static FP32 field in LAYOUT_QUADS): the two
 // (x-left, y-left) corner rows hold all four corners of their z plane, read
 // from them where the right x and y indices are the left ones' successors.
-// ROWS = ROWS_COLUMNS (static FP64 field in LAYOUT_COLUMNS): as quad rows,
-// the offsets rows of the column storage (group_corners), so the two rows
-// read are adjacent whenever the right z index is the left one's successor.
+// ROWS = ROWS_COLUMNS (static field in LAYOUT_COLUMNS): as quad rows, the
+// offsets rows of the column storage (group_corners), so the two rows read
+// are adjacent whenever the right z index is the left one's successor.
 template <int MODE, int GEOM, int PREC, int TIME = 0, int FEAT = 0, int ROWS = ROWS_NODES>
 struct GroupForce {
     static_assert(ROWS == ROWS_NODES ||
-                      (TIME == 0 && (ROWS == ROWS_PAIRS || (ROWS == ROWS_QUADS && PREC == PTC_FP32) ||
-                                     (ROWS == ROWS_COLUMNS && PREC == PTC_FP64))),
-                  "pair rows hold a static field, quad rows a static FP32 field, column rows a "
-                  "static FP64 field");
+                      (TIME == 0 && (ROWS == ROWS_PAIRS || ROWS == ROWS_COLUMNS ||
+                                     (ROWS == ROWS_QUADS && PREC == PTC_FP32))),
+                  "pair and column rows hold a static field, quad rows a static FP32 field");
     using K = std::conditional_t<PREC == PTC_FP32, float, double>;
     const DeviceFields &F;
     const StepParams &P;
@@ -316,17 +315,20 @@ struct GroupForce {
         }
     }
 
-    // the same from quad rows (ROWS_QUADS): the (x-left, y-left) corner row
-    // of each z plane holds the lane's component of its four corners side by
-    // side (one 16-byte load per plane), corner (x, y) at x + 2 y, when the
-    // right x and y indices are the left ones' periodic successors; else
-    // every corner comes from its own row
+    // the same from FP32 quad rows (ROWS_QUADS, or ROWS_COLUMNS in FP32: the
+    // rows in column storage): the (x-left, y-left) corner row of each z
+    // plane holds the lane's component of its four corners side by side (one
+    // 16-byte load per plane), corner (x, y) at x + 2 y, when the right x and
+    // y indices are the left ones' periodic successors; else every corner
+    // comes from its own row
     __device__ __forceinline__ void load_quads(const uint32_t *o, K *dst) const
     {
+        // row strides along x and y (node order, or column storage)
+        const uint32_t sx = ROWS == ROWS_COLUMNS ? (uint32_t)P.dm[2] : 1u;
+        const uint32_t sy = (uint32_t)P.dm[0] * sx;
         const uint32_t dx = o[4] - o[0], dy = o[2] - o[0];
-        const bool xs = dx == 1u || dx == (uint32_t)(1 - P.dm[0]);
-        const bool ys = dy == (uint32_t)P.dm[0] ||
-                        dy == (uint32_t)((int64_t)P.dm[0] * (1 - P.dm[1]));
+        const bool xs = dx == sx || dx == (uint32_t)((int64_t)(1 - P.dm[0]) * sx);
+        const bool ys = dy == sy || dy == (uint32_t)((int64_t)(1 - P.dm[1]) * sy);
         const float4 *__restrict__ d4 = (const float4 *)F.data + comp;
         // corner j: x = j >> 2, y = (j >> 1) & 1, z = j & 1
         const float4 q0 = d4[(size_t)o[0] * 8], q1 = d4[(size_t)o[1] * 8];
@@ -402,6 +404,7 @@ struct GroupForce {
                 loads++;
                 if constexpr (ROWS == ROWS_PAIRS) load_pairs(c.o, v);
                 else if constexpr (ROWS == ROWS_QUADS) load_quads(c.o, v);
+                else if constexpr (ROWS == ROWS_COLUMNS && PREC == PTC_FP32) load_quads(c.o, v);
                 else if constexpr (ROWS == ROWS_COLUMNS) load_columns(c.o, v);
                 else load(c.o, 0, v);
                 key0 = c.o[0];

@@ -221,22 +221,48 @@ __global__ void to_columns_f64_kernel(const double *__restrict__ soa, int dm0, i
     }
 }
 
+// SOA planes (FP64) -> FP32 column rows (128 bytes, stored z fastest): the
+// FP32 quad row of node (i, j, k) (value c of node a at 4 c + a, then eight
+// zeros) at row (i + dm0 j) dm2 + k.  One thread per row, in storage order.
+__global__ void to_columns_f32_kernel(const double *__restrict__ soa, int dm0, int dm1, int dm2,
+                                      int64_t nodes, float *__restrict__ out)
+{
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < nodes;
+         r += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t ij = r / dm2;
+        const int k = (int)(r - ij * dm2);
+        const int i = (int)(ij % dm0), j = (int)(ij / dm0);
+        const int64_t base = (int64_t)k * dm0 * dm1;  // node (0, 0, k)
+        const int i1 = i + 1 == dm0 ? 0 : i + 1, j1 = j + 1 == dm1 ? 0 : j + 1;
+        const int64_t q[4] = {base + i + (int64_t)j * dm0, base + i1 + (int64_t)j * dm0,
+                              base + i + (int64_t)j1 * dm0, base + i1 + (int64_t)j1 * dm0};
+        float *__restrict__ row = out + 32 * r;
+#pragma unroll
+        for (int c = 0; c < 6; c++)
+#pragma unroll
+            for (int a = 0; a < 4; a++) row[4 * c + a] = float(soa[c * nodes + q[a]]);
+#pragma unroll
+        for (int c = 24; c < 32; c++) row[c] = 0.0f;
+    }
+}
+
 // value comp (0..5) of node n (x fastest) of a stored static field, any layout
 __device__ __forceinline__ double stored_value(const DeviceFields &F, int64_t n, int comp)
 {
     const int64_t rv = row_values(F.layout, F.prec);
     const int pos = row_nodes(F.layout) * comp;
-    if (F.prec == PTC_FP32) return double(((const float *)F.data)[n * rv + pos]);
+    const int64_t r = row_of(F.layout, F.dm, n);
+    if (F.prec == PTC_FP32) return double(((const float *)F.data)[r * rv + pos]);
     if (F.layout == LAYOUT_SOA) return ((const double *)F.data)[comp * F.nodes + n];
-    return ((const double *)F.data)[row_of(F.layout, F.dm, n) * rv + pos];
+    return ((const double *)F.data)[r * rv + pos];
 }
 
 // Rows of a new static field (layout, prec) from a stored one (any layout
 // and precision), node by node: node rows (FP64: six values; FP32: six, |B|
 // from the stored values in FP64, 0), pair rows (the node, its +x
 // neighbour, zeros), FP32 quad rows (the node, +x, +y, +x+y, zeros; value
-// c of the row's node a at row_nodes * c + a) or FP64 column rows (as quad
-// rows, no zeros, row row_of(n)), the neighbours periodic.  The
+// c of the row's node a at row_nodes * c + a) or column rows (quad rows in
+// row row_of(n); FP64 without zeros), the neighbours periodic.  The
 // same values as building the new form from the stored field's planes.
 __global__ void convert_rows_kernel(const DeviceFields F, void *__restrict__ out, int layout,
                                     int prec)
@@ -270,8 +296,8 @@ __global__ void convert_rows_kernel(const DeviceFields F, void *__restrict__ out
             ((float *)out)[n * rv + 7] = 0.0f;
         } else {
             for (int c = 6 * nq; c < rv; c++) {
-                if (prec == PTC_FP32) ((float *)out)[n * rv + c] = 0.0f;
-                else ((double *)out)[n * rv + c] = 0.0;
+                if (prec == PTC_FP32) ((float *)out)[ro * rv + c] = 0.0f;
+                else ((double *)out)[ro * rv + c] = 0.0;
             }
         }
     }
@@ -280,9 +306,9 @@ __global__ void convert_rows_kernel(const DeviceFields F, void *__restrict__ out
 // One component of a stored field back to a plane of doubles (the
 // reference's uu(:,:,:,c) / bb(:,:,:,c) order): from the six SOA planes,
 // FP64 node rows, FP32 node rows (eight floats), FP32 pair rows (sixteen
-// floats), FP32 quad rows (32 floats), FP64 pair rows (16 doubles) or FP64
-// column rows (24 doubles, row row_of(i)); in pair, quad and column rows the
-// row's own node's value comp at row_nodes * comp.
+// floats), FP32 quad rows (32 floats), FP64 pair rows (16 doubles) or
+// column rows (row row_of(i): 32 floats or 24 doubles); in pair, quad and
+// column rows the row's own node's value comp at row_nodes * comp.
 __global__ void extract_plane_kernel(const void *__restrict__ data, int layout, int prec, int dm0,
                                      int dm1, int dm2, int64_t nodes, int comp,
                                      double *__restrict__ out)
@@ -293,12 +319,13 @@ __global__ void extract_plane_kernel(const void *__restrict__ data, int layout, 
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nodes;
          i += (int64_t)gridDim.x * blockDim.x) {
         double v;
+        const int64_t r = row_of(layout, dm, i);
         if (prec == PTC_FP32)
-            v = double(((const float *)data)[i * rv + pos]);
+            v = double(((const float *)data)[r * rv + pos]);
         else if (layout == LAYOUT_SOA)
             v = ((const double *)data)[comp * nodes + i];
         else
-            v = ((const double *)data)[row_of(layout, dm, i) * rv + pos];
+            v = ((const double *)data)[r * rv + pos];
         out[i] = v;
     }
 }
@@ -545,8 +572,7 @@ hipError_t launch_convert_rows(const DeviceFields &f, void *out, int layout, int
                                hipStream_t stream)
 {
     if (f.nsnap > 0 || f.layout == LAYOUT_BRICK || layout == LAYOUT_SOA ||
-        layout == LAYOUT_BRICK || (layout == LAYOUT_QUADS && prec != PTC_FP32) ||
-        (layout == LAYOUT_COLUMNS && prec != PTC_FP64))
+        layout == LAYOUT_BRICK || (layout == LAYOUT_QUADS && prec != PTC_FP32))
         return hipErrorInvalidValue;
     int64_t blocks = (f.nodes + 255) / 256;
     if (blocks > 65536) blocks = 65536;
@@ -572,6 +598,16 @@ hipError_t launch_to_quads_f32(const double *soa, const int dm[3], float *out, h
     if (blocks > 65536) blocks = 65536;
     hipLaunchKernelGGL(to_quads_f32_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, soa,
                        dm[0], dm[1], nodes, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_to_columns_f32(const double *soa, const int dm[3], float *out, hipStream_t stream)
+{
+    const int64_t nodes = (int64_t)dm[0] * dm[1] * dm[2];
+    int64_t blocks = (nodes + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(to_columns_f32_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, soa,
+                       dm[0], dm[1], dm[2], nodes, out);
     return hipGetLastError();
 }
 

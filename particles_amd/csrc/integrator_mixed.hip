@@ -5,8 +5,8 @@
 // mixed path is not bitwise comparable to the reference anyway, fused
 // multiply-adds in the FP64 stage sums and the FP32 gathers both cut
 // instructions and round less, and contracting only what one expression
-// holds makes every instantiation (node, pair or quad rows; trio or group
-// kernel) round the same way.
+// holds makes every instantiation (node, pair, quad or column rows; trio or
+// group kernel) round the same way.
 #include "trio_impl.hpp"
 
 namespace ptc {
@@ -15,20 +15,22 @@ hipError_t launch_integrate_mixed(int method, const DeviceFields &f, const StepP
                                   const DeviceParticles &p, double t_stop, const LaunchConfig &lc,
                                   hipStream_t stream)
 {
-    if (f.prec != PTC_FP32 ||
-        (f.layout != LAYOUT_AOS && f.layout != LAYOUT_PAIRS && f.layout != LAYOUT_QUADS))
+    if (f.prec != PTC_FP32 || (f.layout != LAYOUT_AOS && !rows_layout(f.layout)))
         return hipErrorInvalidValue;
     const int geom = sp.pow2 ? 1 : 0;
-    if (f.layout == LAYOUT_PAIRS || f.layout == LAYOUT_QUADS) {
-        // pair or quad rows: static field, the group kernel only
+    if (rows_layout(f.layout)) {
+        // pair, quad or column rows: static field, the group kernel only
         if (f.nsnap > 0 || lc.tail != kTailGroupOnly) return hipErrorInvalidValue;
-        const int rows = f.layout == LAYOUT_QUADS ? ROWS_QUADS : ROWS_PAIRS;
+        const int rows = f.layout == LAYOUT_QUADS     ? ROWS_QUADS
+                         : f.layout == LAYOUT_COLUMNS ? ROWS_COLUMNS
+                                                      : ROWS_PAIRS;
 #define GROUP_R(M, MO, G, CT, R)                                               \
     if (method == M && f.mode == MO && geom == G && lc.ctrl == CT && rows == R) \
         return lc.lanes == 3                                                   \
                    ? launch_trio_t<M, MO, G, PTC_FP32, R, CT>(f, sp, p, t_stop, kBlock, stream) \
                    : launch_group_t<M, MO, G, PTC_FP32, 0, CT, R>(f, sp, p, t_stop, kBlock, stream);
-#define GROUP_P(M, MO, G, CT) GROUP_R(M, MO, G, CT, ROWS_PAIRS) GROUP_R(M, MO, G, CT, ROWS_QUADS)
+#define GROUP_P(M, MO, G, CT) \
+    GROUP_R(M, MO, G, CT, ROWS_PAIRS) GROUP_R(M, MO, G, CT, ROWS_QUADS) GROUP_R(M, MO, G, CT, ROWS_COLUMNS)
 #define GROUP_P_ALL(CT)                                                        \
         GROUP_P(PTC_RK5, PTC_MODE_VB, 0, CT) GROUP_P(PTC_RK5, PTC_MODE_EB, 0, CT) \
         GROUP_P(PTC_DOP853, PTC_MODE_VB, 0, CT) GROUP_P(PTC_DOP853, PTC_MODE_EB, 0, CT) \

@@ -29,12 +29,14 @@ enum : int { ROWS_NODES = 0, ROWS_PAIRS = 1, ROWS_QUADS = 2, ROWS_COLUMNS = 3 };
 //        of the node, of its +x, +y and +x+y neighbours (periodic), then 8
 //        zeros; a cell's eight corners are two aligned 128-byte rows (the
 //        group kernel).
-//   COLUMNS (FP64 only): per node a 192-byte row of 24 doubles, the six
+//   COLUMNS: quad rows stored z fastest (row of node (i, j, k) at
+//        (i + dm1 j) dm3 + k), so the two rows of a cell, planes k and
+//        k + 1, are adjacent.  FP64: a 192-byte row of 24 doubles, the six
 //        values of the node, of its +x, +y and +x+y neighbours (periodic),
-//        value c of node a at 4 c + a, no padding; the rows stored z
-//        fastest (row of node (i, j, k) at (i + dm1 j) dm3 + k), so the two
-//        rows of a cell, planes k and k + 1, are one 384-byte span: three
-//        128-byte lines when k is even, four when it is odd.
+//        value c of node a at 4 c + a, no padding; a cell's rows are one
+//        384-byte span, three 128-byte lines when k is even, four when it
+//        is odd.  FP32: the 128-byte FP32 quad row (24 floats, 8 zeros); a
+//        cell's rows are two adjacent lines.
 //   BRICK: AOS nodes plus, per cell, the eight corner rows of that cell in
 //        the reference's corner order (48 values, 384 bytes, three 128-byte
 //        lines) -- eight copies of every node, so a gather reads one
@@ -159,11 +161,12 @@ hipError_t launch_to_pairs_f32(const double *soa, const int dm[3], float *out, h
 hipError_t launch_to_pairs_f64(const double *soa, const int dm[3], double *out, hipStream_t stream);
 hipError_t launch_to_quads_f32(const double *soa, const int dm[3], float *out, hipStream_t stream);
 hipError_t launch_to_columns_f64(const double *soa, const int dm[3], double *out, hipStream_t stream);
+hipError_t launch_to_columns_f32(const double *soa, const int dm[3], float *out, hipStream_t stream);
 // values per node row of a stored layout (node rows: 6 FP64 or 8 FP32)
 constexpr int row_values(int layout, int prec)
 {
     return layout == LAYOUT_QUADS     ? 32
-           : layout == LAYOUT_COLUMNS ? 24
+           : layout == LAYOUT_COLUMNS ? (prec == 1 ? 32 : 24)
            : layout == LAYOUT_PAIRS   ? 16
            : prec == 1                ? 8
                                       : 6;

@@ -41,7 +41,6 @@ constexpr int kTriosPerWave = 64 / kTrio;
 template <int MODE, int GEOM, int PREC, int ROWS, int FEAT = 0>
 struct TrioForce {
     static_assert(ROWS != ROWS_QUADS || PREC == PTC_FP32, "quad rows hold FP32 fields");
-    static_assert(ROWS != ROWS_COLUMNS || PREC == PTC_FP64, "column rows hold FP64 fields");
     using K = std::conditional_t<PREC == PTC_FP32, float, double>;
     const DeviceFields &F;
     const StepParams &P;
@@ -99,13 +98,15 @@ struct TrioForce {
         }
     }
 
-    // quad rows: as GroupForce::load_quads, for both components
+    // FP32 quad rows (node order, or column storage): as
+    // GroupForce::load_quads, for both components
     __device__ __forceinline__ void load_quads(const uint32_t *o)
     {
+        const uint32_t sx = ROWS == ROWS_COLUMNS ? (uint32_t)P.dm[2] : 1u;
+        const uint32_t sy = (uint32_t)P.dm[0] * sx;
         const uint32_t dx = o[4] - o[0], dy = o[2] - o[0];
-        const bool xs = dx == 1u || dx == (uint32_t)(1 - P.dm[0]);
-        const bool ys = dy == (uint32_t)P.dm[0] ||
-                        dy == (uint32_t)((int64_t)P.dm[0] * (1 - P.dm[1]));
+        const bool xs = dx == sx || dx == (uint32_t)((int64_t)(1 - P.dm[0]) * sx);
+        const bool ys = dy == sy || dy == (uint32_t)((int64_t)(1 - P.dm[1]) * sy);
         const float4 *__restrict__ d4 = (const float4 *)F.data;
 #pragma unroll
         for (int h = 0; h < 2; h++) {
@@ -160,6 +161,7 @@ struct TrioForce {
             loads++;
             if constexpr (ROWS == ROWS_PAIRS) load_pairs(c.o);
             else if constexpr (ROWS == ROWS_QUADS) load_quads(c.o);
+            else if constexpr (ROWS == ROWS_COLUMNS && PREC == PTC_FP32) load_quads(c.o);
             else if constexpr (ROWS == ROWS_COLUMNS) load_columns(c.o);
             else load_nodes(c.o);
             key0 = c.o[0];

@@ -66,6 +66,8 @@ def test_options_roundtrip_and_validation():
         assert P.get_option("f32_layout") == "pairs"
         P.set_option("f32_layout", "quads")
         assert P.get_option("f32_layout") == "quads"
+        P.set_option("f32_layout", "columns")
+        assert P.get_option("f32_layout") == "columns"
         P.set_option("layout", "pairs")
         assert P.get_option("layout") == "pairs"
         P.set_option("layout", "columns")

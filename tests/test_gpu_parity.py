@@ -698,7 +698,7 @@ def test_handover_counts_retired_particles(oracle, lanes):
 
 
 @pytest.mark.parametrize("form", ["aos", "soa", "brick", "pairs", "columns", "fp32 nodes",
-                                  "fp32 pairs", "fp32 quads", "series"])
+                                  "fp32 pairs", "fp32 quads", "fp32 columns", "series"])
 def test_fields_download_returns_stored_values(form):
     """ptc_fields_download gives back what each storage form holds: FP64
     layouts exactly the uploaded arrays, FP32 ones the arrays rounded to
@@ -730,8 +730,8 @@ def test_fields_download_returns_stored_values(form):
         P.set_option("layout", old)
 
 
-@pytest.mark.parametrize("target", ["fp32 nodes", "fp32 pairs", "fp32 quads", "fp64 pairs",
-                                    "fp64 columns", "fp64 aos"])
+@pytest.mark.parametrize("target", ["fp32 nodes", "fp32 pairs", "fp32 quads", "fp32 columns",
+                                    "fp64 pairs", "fp64 columns", "fp64 aos"])
 @pytest.mark.parametrize("source", ["aos", "pairs", "columns", "soa"])
 def test_fields_convert_equals_direct_creation(source, target):
     """ptc_fields_convert builds, on the device, the same stored values as
@@ -813,7 +813,8 @@ def _fields_fp32(uu, bb, layout, **kw):
 def _fields_rows(uu, bb, form, **kw):
     """Fields in one of the group kernel's row forms: "nodes" / "fp64
     nodes" (node rows), "pairs" (FP32 pair rows), "quads" (FP32 quad rows),
-    "fp64 pairs", "fp64 columns" (FP64 quad rows stored z fastest)."""
+    "columns" (FP32 quad rows stored z fastest), "fp64 pairs", "fp64 columns"
+    (FP64 quad rows stored z fastest)."""
     if form.startswith("fp64"):
         P.set_option("layout", form.split()[1].replace("nodes", "aos"))
         try:
@@ -823,13 +824,13 @@ def _fields_rows(uu, bb, form, **kw):
     return _fields_fp32(uu, bb, form, **kw)
 
 
-@pytest.mark.parametrize("form", ["pairs", "quads", "fp64 pairs", "fp64 columns"])
+@pytest.mark.parametrize("form", ["pairs", "quads", "columns", "fp64 pairs", "fp64 columns"])
 @pytest.mark.parametrize("method", ["rk5", "dop853"])
 def test_pair_rows_match_node_rows(method, form):
     """Pair rows (a node and its +x neighbour in one row: 64 bytes in FP32,
     128 in FP64), FP32 quad rows (a node and its +x, +y, +x+y neighbours
-    in one 128-byte row) and FP64 column rows (the same four nodes in a
-    192-byte row, rows stored z fastest) hold the same values as node rows,
+    in one 128-byte row) and column rows (those rows stored z fastest; in
+    FP64 192-byte rows) hold the same values as node rows,
     and the group kernel reads a right corner from the left corner's row
     only where the right index is the left one's periodic successor:
     snapshots and counters identical to node rows with the group kernel
@@ -882,7 +883,7 @@ def test_pair_rows_match_node_rows(method, form):
 
 
 @pytest.mark.parametrize("form", ["fp64 nodes", "fp64 pairs", "fp64 columns", "nodes", "pairs",
-                                  "quads"])
+                                  "quads", "columns"])
 @pytest.mark.parametrize("method", ["rk5", "dop853"])
 def test_trio_kernel_matches_group_kernel(method, form):
     """The trio kernel (three lanes per particle, group_lanes 3) against
@@ -933,13 +934,13 @@ def test_trio_kernel_matches_group_kernel(method, form):
         P.set_option("group_lanes", "auto")
 
 
-@pytest.mark.parametrize("form", ["pairs", "quads", "fp64 pairs", "fp64 columns"])
+@pytest.mark.parametrize("form", ["pairs", "quads", "columns", "fp64 pairs", "fp64 columns"])
 def test_pair_rows_refused_where_node_rows_are_read(form):
     """Probes, record mode and snapshot series read node rows: they refuse
     pair, quad and column rows with an error; the learned controller runs
-    on FP32 pair and quad rows (group kernel) with the results it gives on
-    node rows, and refuses FP64 pair and column rows (it runs FP64 fields
-    one lane per particle)."""
+    on FP32 pair, quad and column rows (group kernel) with the results it
+    gives on node rows, and refuses FP64 pair and column rows (it runs FP64
+    fields one lane per particle)."""
     from oracle import mlp_oracle as M
     uu, bb = W.turbulent_vb(16, seed=57, nmodes=32)
     p = W.params_vector(tmax=0.5 * W.T0, dtmax=1.0, atol=1e-8, rtol=1e-8)
