@@ -62,6 +62,26 @@ def test_traffic_matches_only_its_configuration():
     assert bench.traffic_from_profile(_cfg(["--snapshots", "8", "--group-lanes", "3"])) == series
 
 
+def test_default_workload_traffic_follows_the_field_storage():
+    """The 1024^3 figures: column rows (the default storage) and pair /
+    quad rows each get their own profile; the FP64 storage does not split
+    FP32 runs, nor the FP32 storage FP64 runs."""
+    base = ["--workload", "reconnection1024", "--order", "auto", "--group-lanes", "auto"]
+    cols64 = bench.traffic_from_profile(_cfg(base + ["--layout", "columns",
+                                                     "--f32-layout", "columns"]))
+    pairs64 = bench.traffic_from_profile(_cfg(base + ["--layout", "pairs"]))
+    cols32 = bench.traffic_from_profile(_cfg(base + ["--prec", "fp32", "--f32-layout", "columns"]))
+    quads32 = bench.traffic_from_profile(_cfg(base + ["--prec", "fp32", "--f32-layout", "quads"]))
+    assert None not in (cols64, pairs64, cols32, quads32)
+    assert cols64 < pairs64 and cols32 < pairs64
+    assert bench.traffic_from_profile(_cfg(base + ["--layout", "columns",
+                                                   "--f32-layout", "nodes"])) == cols64
+    assert bench.traffic_from_profile(_cfg(base + ["--prec", "fp32", "--layout", "aos",
+                                                   "--f32-layout", "columns"])) == cols32
+    assert bench.traffic_from_profile(_cfg(base + ["--layout", "columns",
+                                                   "--group-lanes", "6"])) is None
+
+
 def test_more_than_one_gpu_is_refused():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
                        capture_output=True, text=True, timeout=120)
