@@ -352,20 +352,27 @@ int ptc_particles_use_mlp(ptc_particles *p, const ptc_mlp *net, double shift);
  *                 "soa" (the reference's six planes), "aos" (six
  *                 components interleaved per node), "brick" (aos plus
  *                 each cell's eight corners in one 384-byte block; 9x the
- *                 memory of "soa") or "pairs" (128-byte rows holding a
+ *                 memory of "soa"), "pairs" (128-byte rows holding a
  *                 node and its +x neighbour, so a gather reads four
- *                 aligned cache lines; 2.67x the memory of "aos"); FP64
+ *                 aligned cache lines; 2.67x the memory of "aos") or
+ *                 "columns" (192-byte rows holding a node and its +x, +y
+ *                 and +x+y neighbours, stored z fastest, so a gather
+ *                 reads one 384-byte span, three or four cache lines; 4x
+ *                 the memory of "aos", 192 GiB at 1024^3; the fastest
+ *                 FP64 form measured on 256^3 to 1024^3 fields); FP64
  *                 fields only
  *   "f32_layout"  FP32 fields: "nodes" (default; 32-byte node rows),
  *                 "pairs" (64-byte rows holding a node and its +x
  *                 neighbour, so a gather reads four aligned 64-byte rows;
- *                 twice the memory) or "quads" (128-byte rows holding a
+ *                 twice the memory), "quads" (128-byte rows holding a
  *                 node and its +x, +y and +x+y neighbours, so a gather
- *                 reads two aligned rows; four times the memory).  Pair
- *                 and quad rows (FP64 and FP32) are read by the group
- *                 kernel only: every "tail" setting then runs it, and
- *                 probes, record mode and snapshot series refuse them.
- *                 Results are identical to node rows with the group kernel.
+ *                 reads two aligned rows; four times the memory) or
+ *                 "columns" (those rows stored z fastest, so the two rows
+ *                 are adjacent).  Pair, quad and column rows (FP64 and
+ *                 FP32) are read by the group or trio kernel only: every
+ *                 "tail" setting then runs it, and probes, record mode and
+ *                 snapshot series refuse them.  Results are identical to
+ *                 node rows with the group kernel.
  *   "max_steps"   attempted-step cap per particle (0 = none)
  *   "cell_cache"  1 (default) keeps, per particle, the corner values of the
  *                 last cell it sampled (aos layout) in registers, so stage
@@ -405,14 +412,15 @@ int ptc_particles_use_mlp(ptc_particles *p, const ptc_mlp *net, double shift);
  *                 fields, and in record mode, it runs one lane per
  *                 particle.
  *   "group_lanes" static fields with the PI controller, wherever a group
- *                 kernel runs ("tail" group or a hand-over, and pair and
- *                 quad rows): "6" the group kernel (one component per
+ *                 kernel runs ("tail" group or a hand-over, and pair, quad
+ *                 and column rows): "6" the group kernel (one component per
  *                 lane, ten particles per wave); "3" the trio kernel
  *                 (lane a holds position a and momentum a, 21 particles
  *                 per wave: less than half the instructions per particle,
  *                 more particles in flight, each trajectory slower);
- *                 "auto" (default): three for FP32 DOP853 batches of at
- *                 least 2,000,000 particles, six elsewhere.  Results are
+ *                 "auto" (default): three on column rows and for FP32
+ *                 DOP853 batches of at least 2,000,000 particles, six
+ *                 elsewhere.  Results are
  *                 identical for both (FP64 and FP32); snapshot series run
  *                 six lanes; the learned controller (FP32 fields) follows
  *                 the option as the PI controller does.
@@ -438,9 +446,11 @@ double ptc_pow_host(double x, double y);
  * (allocated and freed here): groups of eight lanes each read one line per
  * load (16 bytes a lane), eight independent loads in flight per lane, for
  * `rounds` rounds of loads, timed with HIP events after one untimed launch.
- * This is the access pattern of the integrator's cell loads from fields far
- * larger than the caches (pair and quad rows are one line each), so its
- * rate bounds a gather-bound launch.  result = GB/s, ms, bytes read. */
+ * This is the access pattern of the integrator's cell loads from pair and
+ * quad rows of fields far larger than the caches (rows of one line each at
+ * unrelated addresses), so its rate bounds a gather-bound launch on them;
+ * column rows put a cell's lines side by side and are served faster.
+ * result = GB/s, ms, bytes read. */
 int ptc_diag_line_gather(int64_t bytes, int rounds, double result[3]);
 
 #ifdef __cplusplus

@@ -653,7 +653,7 @@ static int resolve_tail(int tail, const ptc_particles *p)
 // Lanes per particle of the kernel that runs a static field with the PI
 // controller where particles share a wave's lanes ("group_lanes" auto,
 // measured on the MI355X, both giving the same results): three (the trio
-// kernel) for column rows (FP64: 1.24-1.30x the six-lane kernel on 256^3
+// kernel) for column rows (FP64: 1.25-1.44x the six-lane kernel on 256^3
 // DOP853 with 100,000 particles, 512^3 RK5 with 1,000,000, 1024^3 DOP853
 // and RK5 with 1,000,000 and 4,000,000; FP32: 1.32x on 256^3 DOP853, 1.25x
 // on 512^3 RK5, 1.36x on 1024^3 DOP853) and for FP32 DOP853 batches of at
