@@ -480,18 +480,21 @@ int ptc_fields_series(const ptc_fields *const *snaps, const double *times, int n
         if (!snaps[i]) return fail(PTC_EARG, "ptc_fields_series: snapshot %d is null", i);
         const DeviceFields &fi = snaps[i]->d;
         if (fi.nsnap > 0) return fail(PTC_EARG, "ptc_fields_series: snapshot %d is a series", i);
-        if (fi.layout != LAYOUT_AOS)
-            return fail(PTC_EARG, "ptc_fields_series: snapshot %d is not in the aos layout", i);
+        if (fi.layout != LAYOUT_AOS && fi.layout != LAYOUT_COLUMNS)
+            return fail(PTC_EARG, "ptc_fields_series: snapshot %d is not in node rows (layout "
+                                  "aos, f32_layout nodes) or column rows", i);
         if (fi.dm[0] != f0.dm[0] || fi.dm[1] != f0.dm[1] || fi.dm[2] != f0.dm[2] ||
-            fi.mode != f0.mode || fi.prec != f0.prec)
+            fi.mode != f0.mode || fi.prec != f0.prec || fi.layout != f0.layout)
             return fail(PTC_EARG, "ptc_fields_series: snapshot %d differs from snapshot 0 in "
-                                  "dimensions, mode or precision", i);
+                                  "dimensions, mode, precision or layout", i);
         if (!std::isfinite(times[i]))
             return fail(PTC_EARG, "ptc_fields_series: time %d is not finite", i);
         if (i > 0 && !(times[i] > times[i - 1]))
             return fail(PTC_EARG, "ptc_fields_series: times must increase strictly (%d)", i);
     }
-    const int64_t stride = (f0.prec == PTC_FP32 ? 8 : 6) * f0.nodes;
+    // node rows (6 doubles or 8 floats) or column rows (24 doubles or 32
+    // floats) per node
+    const int64_t stride = (int64_t)row_values(f0.layout, f0.prec) * f0.nodes;
     const size_t bytes = (size_t)stride * (f0.prec == PTC_FP32 ? sizeof(float) : sizeof(double));
     void *data = nullptr;
     double *st = nullptr;

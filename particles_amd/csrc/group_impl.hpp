@@ -208,15 +208,15 @@ __device__ __forceinline__ void group_corners(const StepParams &P, double pos, i
 // own rows).  ROWS = ROWS_QUADS (static FP32 field in LAYOUT_QUADS): the two
 // (x-left, y-left) corner rows hold all four corners of their z plane, read
 // from them where the right x and y indices are the left ones' successors.
-// ROWS = ROWS_COLUMNS (static field in LAYOUT_COLUMNS): as quad rows, the
-// offsets rows of the column storage (group_corners), so the two rows read
-// are adjacent whenever the right z index is the left one's successor.
+// ROWS = ROWS_COLUMNS (a static field or a snapshot series in
+// LAYOUT_COLUMNS): as quad rows, the offsets rows of the column storage
+// (group_corners), so the two rows read are adjacent whenever the right z
+// index is the left one's successor.
 template <int MODE, int GEOM, int PREC, int TIME = 0, int FEAT = 0, int ROWS = ROWS_NODES>
 struct GroupForce {
-    static_assert(ROWS == ROWS_NODES ||
-                      (TIME == 0 && (ROWS == ROWS_PAIRS || ROWS == ROWS_COLUMNS ||
-                                     (ROWS == ROWS_QUADS && PREC == PTC_FP32))),
-                  "pair and column rows hold a static field, quad rows a static FP32 field");
+    static_assert(ROWS == ROWS_NODES || ROWS == ROWS_COLUMNS ||
+                      (TIME == 0 && (ROWS == ROWS_PAIRS || (ROWS == ROWS_QUADS && PREC == PTC_FP32))),
+                  "pair rows hold a static field, quad rows a static FP32 field");
     using K = std::conditional_t<PREC == PTC_FP32, float, double>;
     const DeviceFields &F;
     const StepParams &P;
@@ -316,12 +316,12 @@ struct GroupForce {
     }
 
     // the same from FP32 quad rows (ROWS_QUADS, or ROWS_COLUMNS in FP32: the
-    // rows in column storage): the (x-left, y-left) corner row of each z
-    // plane holds the lane's component of its four corners side by side (one
-    // 16-byte load per plane), corner (x, y) at x + 2 y, when the right x and
-    // y indices are the left ones' periodic successors; else every corner
-    // comes from its own row
-    __device__ __forceinline__ void load_quads(const uint32_t *o, K *dst) const
+    // rows in column storage, of snapshot k): the (x-left, y-left) corner row
+    // of each z plane holds the lane's component of its four corners side by
+    // side (one 16-byte load per plane), corner (x, y) at x + 2 y, when the
+    // right x and y indices are the left ones' periodic successors; else
+    // every corner comes from its own row
+    __device__ __forceinline__ void load_quads(const uint32_t *o, K *dst, int k = 0) const
     {
         // row strides along x and y (node order, or column storage)
         const uint32_t sx = ROWS == ROWS_COLUMNS ? (uint32_t)P.dm[2] : 1u;
@@ -329,13 +329,14 @@ struct GroupForce {
         const uint32_t dx = o[4] - o[0], dy = o[2] - o[0];
         const bool xs = dx == sx || dx == (uint32_t)((int64_t)(1 - P.dm[0]) * sx);
         const bool ys = dy == sy || dy == (uint32_t)((int64_t)(1 - P.dm[1]) * sy);
-        const float4 *__restrict__ d4 = (const float4 *)F.data + comp;
+        const float *__restrict__ snap = (const float *)F.data + (size_t)k * F.sstride;
+        const float4 *__restrict__ d4 = (const float4 *)snap + comp;
         // corner j: x = j >> 2, y = (j >> 1) & 1, z = j & 1
         const float4 q0 = d4[(size_t)o[0] * 8], q1 = d4[(size_t)o[1] * 8];
         dst[0] = q0.x; dst[1] = q1.x; dst[2] = q0.z; dst[3] = q1.z;
         dst[4] = q0.y; dst[5] = q1.y; dst[6] = q0.w; dst[7] = q1.w;
         if (!(xs && ys)) {
-            const float *__restrict__ d = (const float *)F.data + 4 * comp;
+            const float *__restrict__ d = snap + 4 * comp;
 #pragma unroll
             for (int j = 2; j < 8; j++) dst[j] = d[(size_t)o[j] * 32];
         }
@@ -345,24 +346,34 @@ struct GroupForce {
     // row of each z plane holds the lane's component of its four corners
     // side by side (two 16-byte loads per plane), corner (x, y) at x + 2 y,
     // when the right x and y indices are the left ones' periodic successors;
-    // else every corner comes from its own row
-    __device__ __forceinline__ void load_columns(const uint32_t *o, K *dst) const
+    // else every corner comes from its own row (snapshot k of a series)
+    __device__ __forceinline__ void load_columns(const uint32_t *o, K *dst, int k = 0) const
     {
         const uint32_t sx = (uint32_t)P.dm[2], sy = (uint32_t)P.dm[0] * (uint32_t)P.dm[2];
         const uint32_t dx = o[4] - o[0], dy = o[2] - o[0];
         const bool xs = dx == sx || dx == (uint32_t)((int64_t)(1 - P.dm[0]) * P.dm[2]);
         const bool ys = dy == sy || dy == (uint32_t)((int64_t)(1 - P.dm[1]) * sy);
-        const double2 *__restrict__ d2 = (const double2 *)F.data + 2 * comp;
+        const double *__restrict__ snap = (const double *)F.data + (size_t)k * F.sstride;
+        const double2 *__restrict__ d2 = (const double2 *)snap + 2 * comp;
         // corner j: x = j >> 2, y = (j >> 1) & 1, z = j & 1
         const double2 a0 = d2[(size_t)o[0] * 12], a1 = d2[(size_t)o[0] * 12 + 1];
         const double2 b0 = d2[(size_t)o[1] * 12], b1 = d2[(size_t)o[1] * 12 + 1];
         dst[0] = a0.x; dst[1] = b0.x; dst[2] = a1.x; dst[3] = b1.x;
         dst[4] = a0.y; dst[5] = b0.y; dst[6] = a1.y; dst[7] = b1.y;
         if (!(xs && ys)) {
-            const double *__restrict__ d = (const double *)F.data + 4 * comp;
+            const double *__restrict__ d = snap + 4 * comp;
 #pragma unroll
             for (int j = 2; j < 8; j++) dst[j] = d[(size_t)o[j] * 24];
         }
+    }
+
+    // component comp of the cell's eight corner rows of snapshot k, in the
+    // kernel's row form
+    __device__ __forceinline__ void load_snapshot(const uint32_t *o, int k, K *dst) const
+    {
+        if constexpr (ROWS == ROWS_COLUMNS && PREC == PTC_FP32) load_quads(o, dst, k);
+        else if constexpr (ROWS == ROWS_COLUMNS) load_columns(o, dst, k);
+        else load(o, k, dst);
     }
 
     // component comp of f(s), s given by its component comp on each lane
@@ -383,8 +394,8 @@ struct GroupForce {
             const int key = 2 * k + (two ? 1 : 0);
             if (!(c.o[0] == key0 && c.o[7] == key7 && (key == kc || (!two && key + 1 == kc)))) {
                 loads += two ? 2u : 1u;
-                load(c.o, k, v);
-                if (two) load(c.o, k + 1, v1);
+                load_snapshot(c.o, k, v);
+                if (two) load_snapshot(c.o, k + 1, v1);
                 key0 = c.o[0];
                 key7 = c.o[7];
                 kc = key;

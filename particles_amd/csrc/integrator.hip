@@ -361,11 +361,22 @@ hipError_t launch_integrate(int method, const DeviceFields &f, const StepParams 
 {
     if (f.prec == PTC_FP32) return launch_integrate_mixed(method, f, sp, p, t_stop, lc, stream);
     if (f.layout == LAYOUT_PAIRS || f.layout == LAYOUT_COLUMNS) {
-        // FP64 pair or column rows: static field, PI controller, the trio or
-        // group kernel only
-        if (f.nsnap > 0 || lc.ctrl != CTRL_PI || lc.tail != kTailGroupOnly)
-            return hipErrorInvalidValue;
+        // FP64 pair or column rows: PI controller, the trio or group kernel
+        // only; a snapshot series (column rows) the group kernel
+        if (lc.ctrl != CTRL_PI || lc.tail != kTailGroupOnly) return hipErrorInvalidValue;
         const int g = sp.pow2 ? 1 : 0;
+        if (f.nsnap > 0) {
+            if (f.layout != LAYOUT_COLUMNS) return hipErrorInvalidValue;
+#define GROUP_TC(M, MO, G)                                                     \
+    if (method == M && f.mode == MO && g == G)                                 \
+        return launch_group_t<M, MO, G, PTC_FP64, 1, CTRL_PI, ROWS_COLUMNS>(f, sp, p, t_stop, kBlock, stream);
+            GROUP_TC(PTC_RK5, PTC_MODE_VB, 0) GROUP_TC(PTC_RK5, PTC_MODE_EB, 0)
+            GROUP_TC(PTC_DOP853, PTC_MODE_VB, 0) GROUP_TC(PTC_DOP853, PTC_MODE_EB, 0)
+            GROUP_TC(PTC_RK5, PTC_MODE_VB, 1) GROUP_TC(PTC_RK5, PTC_MODE_EB, 1)
+            GROUP_TC(PTC_DOP853, PTC_MODE_VB, 1) GROUP_TC(PTC_DOP853, PTC_MODE_EB, 1)
+#undef GROUP_TC
+            return hipErrorInvalidValue;
+        }
         const int rows = f.layout == LAYOUT_COLUMNS ? ROWS_COLUMNS : ROWS_PAIRS;
 #define GROUP_R(M, MO, G, R)                                                   \
     if (method == M && f.mode == MO && g == G && rows == R)                    \

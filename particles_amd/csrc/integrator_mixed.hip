@@ -19,8 +19,21 @@ hipError_t launch_integrate_mixed(int method, const DeviceFields &f, const StepP
         return hipErrorInvalidValue;
     const int geom = sp.pow2 ? 1 : 0;
     if (rows_layout(f.layout)) {
-        // pair, quad or column rows: static field, the group kernel only
-        if (f.nsnap > 0 || lc.tail != kTailGroupOnly) return hipErrorInvalidValue;
+        // pair, quad or column rows: the trio or group kernel only; a
+        // snapshot series (column rows, PI controller) the group kernel
+        if (lc.tail != kTailGroupOnly) return hipErrorInvalidValue;
+        if (f.nsnap > 0) {
+            if (f.layout != LAYOUT_COLUMNS || lc.ctrl != CTRL_PI) return hipErrorInvalidValue;
+#define GROUP_TC(M, MO, G)                                                     \
+    if (method == M && f.mode == MO && geom == G)                              \
+        return launch_group_t<M, MO, G, PTC_FP32, 1, CTRL_PI, ROWS_COLUMNS>(f, sp, p, t_stop, kBlock, stream);
+            GROUP_TC(PTC_RK5, PTC_MODE_VB, 0) GROUP_TC(PTC_RK5, PTC_MODE_EB, 0)
+            GROUP_TC(PTC_DOP853, PTC_MODE_VB, 0) GROUP_TC(PTC_DOP853, PTC_MODE_EB, 0)
+            GROUP_TC(PTC_RK5, PTC_MODE_VB, 1) GROUP_TC(PTC_RK5, PTC_MODE_EB, 1)
+            GROUP_TC(PTC_DOP853, PTC_MODE_VB, 1) GROUP_TC(PTC_DOP853, PTC_MODE_EB, 1)
+#undef GROUP_TC
+            return hipErrorInvalidValue;
+        }
         const int rows = f.layout == LAYOUT_QUADS     ? ROWS_QUADS
                          : f.layout == LAYOUT_COLUMNS ? ROWS_COLUMNS
                                                       : ROWS_PAIRS;

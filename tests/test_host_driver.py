@@ -159,6 +159,12 @@ def test_dry_run_kernel_choice(tmp_path):
     assert r.returncode == 0 and re.search(r"kernels\s*=\s*group", r.stdout)
     r = _run(tmp, "--kernels", "fastest", "--dry-run")
     assert r.returncode != 0 and "--kernels" in r.stdout and "tail" in r.stdout
+    r = _run(tmp, "--layout", "columns", "--dry-run")
+    assert r.returncode == 0 and re.search(r"field storage\s*=\s*columns", r.stdout)
+    r = _run(tmp, "--mixed", "--f32-layout", "columns", "--dry-run")
+    assert r.returncode == 0 and re.search(r"field storage\s*=\s*columns", r.stdout)
+    r = _run(tmp, "--layout", "column", "--dry-run")
+    assert r.returncode != 0 and "--layout" in r.stdout
 
 
 @pytest.mark.gpu
@@ -176,6 +182,24 @@ def test_full_run_kernel_choices_agree(tmp_path):
                   fits.read(os.path.join(tmp, "counter.fits")))
     assert np.array_equal(out["lanes"][0], out["group"][0])
     assert np.array_equal(out["lanes"][1], out["group"][1])
+
+
+@pytest.mark.gpu
+def test_full_run_field_storage_choices_agree(tmp_path):
+    """--layout columns (the fields in column rows, the trio kernel) writes
+    the same state and counter files as the default node rows, bit for
+    bit."""
+    out = {}
+    for lay in ("aos", "columns"):
+        tmp = os.path.join(str(tmp_path), lay)
+        os.makedirs(tmp)
+        _setup(tmp, PARAMS.replace("nparticles  = 9", "nparticles  = 40"))
+        r = _run(tmp, "--layout", lay)
+        assert r.returncode == 0, r.stdout + r.stderr
+        out[lay] = (fits.read(os.path.join(tmp, "state.fits")),
+                    fits.read(os.path.join(tmp, "counter.fits")))
+    assert np.array_equal(out["aos"][0], out["columns"][0])
+    assert np.array_equal(out["aos"][1], out["columns"][1])
 
 
 @pytest.mark.gpu

@@ -283,6 +283,86 @@ def test_group_kernel_series_matches_oracle(oracle, method, mode):
     assert (status32 == 0).all() and np.median(dev) < 1e-6 and dev.max() < 1e-3
 
 
+def device_series_columns(P, uu, bb, times, mode="vb", prec="fp64"):
+    """device_series with every snapshot in column rows (FP64 layout or FP32
+    f32_layout "columns")."""
+    opt = "layout" if prec == "fp64" else "f32_layout"
+    old = P.get_option(opt)
+    P.set_option(opt, "columns")
+    try:
+        return device_series(P, uu, bb, times, mode=mode, prec=prec)
+    finally:
+        P.set_option(opt, old)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("method", ["rk5", "dop853"])
+@pytest.mark.parametrize("mode", ["vb", "eb"])
+def test_column_row_series_matches_node_rows(oracle, method, mode):
+    """A snapshot series in column rows (each snapshot's quad rows stored z
+    fastest; the six-lane group kernel reads both bracketing snapshots with
+    the column loads): FP64 bit-identical to the oracle's restatement, in
+    the unit box with particles on x, y and z cell faces and in the last
+    cells, and in a shifted box of non-power-of-two size; FP32 identical to
+    the FP32 series in node rows with the group kernel; the download gives
+    each snapshot's values back."""
+    import particles_amd as P
+    tt = TIMES[:3]
+    uu, bb = W.turbulent_vb_series(24, seed=11, times=tt, speed=0.02, nmodes=128)
+    if mode == "eb":
+        uu = np.asfortranarray(0.05 * uu)
+    p = W.params_vector(tmax=4 * W.T0, dtmax=1.0, atol=1e-8, rtol=1e-8)
+    tg = W.time_grid(4, p[2])
+    st0 = W.initial_state(400, len(tg), seed=12, gamma_range=(1.0, 1e3))
+    rng = np.random.default_rng(27)
+    for a in range(3):
+        st0[a, 0, 100 * a:100 * a + 60] = rng.integers(0, 24, 60) / 24.0  # on faces
+        st0[a, 0, 100 * a + 60:100 * a + 80] = 1.0 - 1e-12  # last cell
+    st0[:3, 0, 300:320] = 1.0 - 1e-12
+    xl, xs = (-0.5, 0.2, 0.1), (1.5, 2.0, 0.75)
+    g = W.params_vector(qom=1.0, vun=0.9, tmax=2 * W.T0, dtmax=1.0, atol=1e-8, rtol=1e-8,
+                        xl=xl, xs=xs)
+    tgg = W.time_grid(3, g[2])
+    sg = W.initial_state(256, len(tgg), seed=26, gamma_range=(1.0, 1e3), xl=xl, xs=xs)
+    q = W.params_vector(tmax=0.2 * W.T0, dtmax=1.0, atol=1e-8, rtol=1e-8)
+    tgq = W.time_grid(2, q[2])
+    say(f"[column series] {method} {mode}")
+    so, co, _ = oracle.integrate_series(method, uu, bb, tt, p, tg, st0, mode=mode)
+    sgo, cgo, _ = oracle.integrate_series(method, uu, bb, tt, g, tgg, sg, mode=mode)
+    with device_series_columns(P, uu, bb, tt, mode=mode) as f:
+        st, cnt, status = P.integrate(f, method, p, tg, st0)
+        stg, cntg, statusg = P.integrate(f, method, g, tgg, sg)
+        u1, b1 = f.download(1)
+    assert np.array_equal(u1, uu[..., 1]) and np.array_equal(b1, bb[..., 1])
+    assert (status == 0).all() and np.array_equal(st, so) and np.array_equal(cnt, co)
+    assert (statusg == 0).all() and np.array_equal(stg, sgo) and np.array_equal(cntg, cgo)
+    try:
+        P.set_option("tail", "group")
+        with device_series(P, uu, bb, tt, mode=mode, prec="fp32") as f:
+            ref32 = P.integrate(f, method, q, tgq, st0[:, :len(tgq), :])
+    finally:
+        P.set_option("tail", "auto")
+    with device_series_columns(P, uu, bb, tt, mode=mode, prec="fp32") as f:
+        got32 = P.integrate(f, method, q, tgq, st0[:, :len(tgq), :])
+    assert (got32[2] == 0).all()
+    assert np.array_equal(got32[0], ref32[0]) and np.array_equal(got32[1], ref32[1])
+
+
+@pytest.mark.gpu
+def test_column_row_series_refuses_mixed_layouts():
+    import particles_amd as P
+    uu, bb = W.turbulent_vb(8, seed=22, nmodes=16)
+    old = P.get_option("layout")
+    try:
+        with P.Fields(uu, bb) as f:
+            P.set_option("layout", "columns")
+            with P.Fields(uu, bb) as c:
+                with pytest.raises(P.PtcError, match="differs"):
+                    P.Fields.series([c, f], [0.0, 1.0])
+    finally:
+        P.set_option("layout", old)
+
+
 @pytest.mark.gpu
 def test_device_series_many_snapshots_matches_oracle(oracle):
     """33 snapshots over the run: lanes that pick up a new particle start
