@@ -487,7 +487,9 @@ def variants(args, fields64, make_fields, p, tg, st0, method, plain, cfg):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed launches of 1 T0 each (default 10, fewer where the workload's "
+                         "horizon ends sooner: reconnection512 runs 10 T0)")
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="reconnection1024")
     ap.add_argument("--particles", type=int, default=0)
@@ -621,6 +623,14 @@ def main():
 
     t_setup = time.perf_counter()
     make_fields, uu, bb, p, tg, st0, method, cfg = build_workload(args)
+    # every launch advances the batch by 1 T0 up to the workload's horizon;
+    # launches past it would find every particle done
+    horizon = int(round(p[2] / W.T0))
+    if args.steps is None:
+        args.steps = max(1, min(10, horizon - args.warmup))
+    if args.warmup + args.steps > horizon:
+        raise SystemExit(f"--warmup {args.warmup} + --steps {args.steps} launches of 1 T0 pass "
+                         f"the workload's horizon of {horizon} T0")
     if args.train_gamma is None:
         args.train_gamma = (1.0, 1.0e4) if cfg["recipe"] == "reconnection" else (1.0, 1.0e3)
     P.set_option("layout", args.layout)

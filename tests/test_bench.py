@@ -82,6 +82,15 @@ def test_default_workload_traffic_follows_the_field_storage():
                                                    "--group-lanes", "6"])) is None
 
 
+def test_launches_past_the_horizon_are_refused():
+    """Every launch advances 1 T0; the 512^3 workload's horizon is 10 T0, so
+    2 warm-up and 10 timed launches are refused before any device work."""
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--workload",
+                        "reconnection512", "--warmup", "2", "--steps", "10"],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode != 0 and "horizon of 10 T0" in r.stderr
+
+
 def test_more_than_one_gpu_is_refused():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
                        capture_output=True, text=True, timeout=120)
