@@ -6,10 +6,11 @@ compared is compared:
 
 * 256^3 turbulent VB field, 100,000 protons, DOP853 at tol 1e-8 (the
   turbulent-box DOP853 configuration): every particle of a sample of 1,024
-  bit-identical to the oracle on the same host field;
+  bit-identical to the oracle on the same host field, with the field in
+  node rows and in column rows;
 * 512^3 EB reconnection field built on the device, 1,000,000 protons,
   Cash-Karp at tol 1e-8: a sample bit-identical to the oracle on the field
-  downloaded from the device;
+  downloaded from the device, in node rows and in column rows;
 * 1024^3 EB reconnection field, 4,000,000 protons, DOP853 at tol 1e-8: the
   FP64 run (column rows, as the benchmark stores it) bit-identical to the
   oracle on a sample (51.5 GB field downloaded), the mixed-precision run
@@ -55,7 +56,16 @@ def _sample(np_):
     return np.linspace(0, np_ - 1, SAMPLE).astype(np.int64)
 
 
-def test_turbulent256_dop853_100k(oracle):
+@pytest.fixture(params=["aos", "columns"])
+def layout(request):
+    """The FP64 field storage: node rows, and column rows (the benchmark's)."""
+    old = P.get_option("layout")
+    P.set_option("layout", request.param)
+    yield request.param
+    P.set_option("layout", old)
+
+
+def test_turbulent256_dop853_100k(oracle, layout):
     dm, npart = 256, 100_000
     say("[c1] 256^3 turbulent field on the host")
     uu, bb = W.turbulent_vb(dm, seed=1)
@@ -63,7 +73,7 @@ def test_turbulent256_dop853_100k(oracle):
     tg = W.time_grid(2, p[2])
     st0 = W.initial_state(npart, len(tg), seed=2, gamma_range=(1.0, 1.0e3))
     with P.Fields(uu, bb) as f:
-        say("[c1] device, 100,000 particles, 0.25 T0")
+        say(f"[c1] device, 100,000 particles, 0.25 T0, layout {layout}")
         st, cnt, status = P.integrate(f, "dop853", p, tg, st0)
     assert (status == 0).all()
     idx = _sample(npart)
@@ -74,14 +84,14 @@ def test_turbulent256_dop853_100k(oracle):
     assert ok.all() and (sto == 0).all()
 
 
-def test_reconnection512_rk5_1m(oracle):
+def test_reconnection512_rk5_1m(oracle, layout):
     dm, npart = 512, 1_000_000
     say("[c2] 512^3 reconnection field on the device")
     with W.reconnection_eb_device(dm, seed=1) as f:
         p = W.params_vector(tmax=0.1 * W.T0, dtmax=W.T0, atol=1e-8, rtol=1e-8)
         tg = W.time_grid(2, p[2])
         st0 = W.initial_state(npart, len(tg), seed=2, gamma_range=(1.0, 1.0e4))
-        say("[c2] device, 1,000,000 particles, 0.1 T0")
+        say(f"[c2] device, 1,000,000 particles, 0.1 T0, layout {layout}")
         st, cnt, status = P.integrate(f, "rk5", p, tg, st0)
         say("[c2] download the field")
         ee, bb = f.download()
