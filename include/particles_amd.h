@@ -112,8 +112,9 @@ int ptc_fields_synthesize(const int dm[3], int recipe, const double *scalars, in
 /* Time-dependent fields (no counterpart in the reference, whose field is
  * static: sources/fields.F90:51-78 loads one snapshot and the RHS has no
  * time argument, sources/accelerations.F90:59).  Builds a series from n
- * snapshot handles of equal dimensions, mode and precision in the aos layout
- * (ptc_fields_create or ptc_fields_synthesize), at strictly increasing
+ * snapshot handles of equal dimensions, mode, precision and layout, in node
+ * rows (layout aos, f32_layout nodes) or column rows (ptc_fields_create or
+ * ptc_fields_synthesize), at strictly increasing
  * finite times in the integrator's time unit (that of params(3)).  Their
  * device arrays are copied into one block, so the snapshot handles may be
  * destroyed afterwards; destroy the series with ptc_fields_destroy.
@@ -126,8 +127,9 @@ int ptc_fields_synthesize(const int dm[3], int recipe, const double *scalars, in
  * series integrates with the PI step-size controller; with the cell cache
  * (option cell_cache 1) each lane keeps the two snapshots' corner values of
  * its last cell, the second in LDS (48 values per thread), where the device
- * gives a block that much LDS.  n = 1 is a static field taken through the
- * series kernels. */
+ * gives a block that much LDS.  A series in column rows runs the six-lane
+ * group kernel, which keeps both snapshots' corner values in registers.
+ * n = 1 is a static field taken through the series kernels. */
 int ptc_fields_series(const ptc_fields *const *snaps, const double *times, int n,
                       ptc_fields **out);
 
