@@ -372,8 +372,9 @@ int ptc_particles_use_mlp(ptc_particles *p, const ptc_mlp *net, double shift);
  *                 "columns" (those rows stored z fastest, so the two rows
  *                 are adjacent).  Pair, quad and column rows (FP64 and
  *                 FP32) are read by the group or trio kernel only: every
- *                 "tail" setting then runs it, and probes, record mode and
- *                 snapshot series refuse them.  Results are identical to
+ *                 "tail" setting then runs it, and probes and record mode
+ *                 refuse them; snapshot series take node rows or column
+ *                 rows, pair and quad rows not.  Results are identical to
  *                 node rows with the group kernel.
  *   "max_steps"   attempted-step cap per particle (0 = none)
  *   "cell_cache"  1 (default) keeps, per particle, the corner values of the
@@ -423,9 +424,11 @@ int ptc_particles_use_mlp(ptc_particles *p, const ptc_mlp *net, double shift);
  *                 "auto" (default): three on column rows and for FP32
  *                 DOP853 batches of at least 2,000,000 particles, six
  *                 elsewhere.  Results are
- *                 identical for both (FP64 and FP32); snapshot series run
- *                 six lanes; the learned controller (FP32 fields) follows
- *                 the option as the PI controller does.
+ *                 identical for both (FP64 and FP32); snapshot series
+ *                 follow the option too ("auto": six in FP64, three in
+ *                 FP32); the
+ *                 learned controller (FP32 fields) follows the option as
+ *                 the PI controller does.
  * Takes effect for the next create/advance. */
 int ptc_set_option(const char *key, const char *value);
 /* Reads an option back as text into buf (size len). */

@@ -664,10 +664,14 @@ static int resolve_tail(int tail, const ptc_particles *p)
 // quad rows: 408 against 472 ms per gyro-period); six elsewhere (the trio
 // kernel measured 6-13 % slower on FP32 quad rows with 512^3 RK5 and with
 // DOP853 batches of 1,000,000 particles, and 4-7 % slower on FP64 pair rows
-// at 1024^3).
+// at 1024^3).  Snapshot series (8 snapshots of 256^3, DOP853, 100,000
+// particles): six in FP64, where the trio's series form spills 268-292 B
+// per lane and measured 1.17-1.46x slower (node and column rows); three in
+// FP32 (36-156 B of spills), 1.11-1.38x faster than six lanes.
 static int choose_lanes(int lanes, const DeviceFields &f, int method, int64_t np)
 {
     if (lanes != 0) return lanes;
+    if (f.nsnap > 0) return f.prec == PTC_FP32 ? 3 : 6;
     if (f.layout == LAYOUT_COLUMNS) return 3;
     return f.prec == PTC_FP32 && method == PTC_DOP853 && np >= 2000000 ? 3 : 6;
 }

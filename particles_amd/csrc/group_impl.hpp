@@ -196,6 +196,41 @@ __device__ __forceinline__ void group_corners(const StepParams &P, double pos, i
     cr.w[7] = d0 * d1 * d2;
 }
 
+// The snapshot interval of stage time ts in a series, as Force::interval:
+// k with stime[k] <= ts < stime[k + 1], a = (ts - stime[k]) / (stime[k + 1] -
+// stime[k]) and two = true; before or at the first snapshot time k = 0, at
+// or after the last k = nsnap - 1, a = 0 and two = false.  kh holds the
+// lane's last interval: the search walks from it (0 or 1 steps nearly
+// always) and bisects otherwise.
+__device__ __forceinline__ int series_interval(const DeviceFields &F, double ts, int &kh,
+                                               double &a, bool &two)
+{
+    const int ns = F.nsnap;
+    const double *__restrict__ st = F.stime;
+    a = 0.0;
+    two = false;
+    if (ns <= 1 || !(ts > st[0])) return 0;
+    if (ts >= st[ns - 1]) return ns - 1;
+    int k = kh;
+    if (!(st[k] <= ts && ts < st[k + 1])) {
+        if (k + 2 < ns && st[k + 1] <= ts && ts < st[k + 2]) {
+            k = k + 1;
+        } else {
+            int lo = 0, hi = ns - 1;  // st[lo] <= ts < st[hi]
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (st[mid] <= ts) lo = mid;
+                else hi = mid;
+            }
+            k = lo;
+        }
+    }
+    kh = k;
+    a = (ts - st[k]) / (st[k + 1] - st[k]);
+    two = true;
+    return k;
+}
+
 // The RHS as the group evaluates it: each lane keeps component comp of the
 // eight corner rows of the cell it last gathered from (the lane-per-particle
 // kernel's cell cache, split over the group).  PREC = PTC_FP32: FP32 node
@@ -245,33 +280,10 @@ struct GroupForce {
         if constexpr (TIME == 1) ts = t;
     }
 
-    // the snapshot interval of the stage time, as Force::interval
+    // the snapshot interval of the stage time (series_interval)
     __device__ __forceinline__ int interval(double &a, bool &two)
     {
-        const int ns = F.nsnap;
-        const double *__restrict__ st = F.stime;
-        a = 0.0;
-        two = false;
-        if (ns <= 1 || !(ts > st[0])) return 0;
-        if (ts >= st[ns - 1]) return ns - 1;
-        int k = kh;
-        if (!(st[k] <= ts && ts < st[k + 1])) {
-            if (k + 2 < ns && st[k + 1] <= ts && ts < st[k + 2]) {
-                k = k + 1;
-            } else {
-                int lo = 0, hi = ns - 1;  // st[lo] <= ts < st[hi]
-                while (hi - lo > 1) {
-                    const int mid = (lo + hi) >> 1;
-                    if (st[mid] <= ts) lo = mid;
-                    else hi = mid;
-                }
-                k = lo;
-            }
-        }
-        kh = k;
-        a = (ts - st[k]) / (st[k + 1] - st[k]);
-        two = true;
-        return k;
+        return series_interval(F, ts, kh, a, two);
     }
 
     // component comp of the eight corner rows of snapshot k at offsets o

@@ -362,14 +362,16 @@ hipError_t launch_integrate(int method, const DeviceFields &f, const StepParams 
     if (f.prec == PTC_FP32) return launch_integrate_mixed(method, f, sp, p, t_stop, lc, stream);
     if (f.layout == LAYOUT_PAIRS || f.layout == LAYOUT_COLUMNS) {
         // FP64 pair or column rows: PI controller, the trio or group kernel
-        // only; a snapshot series (column rows) the group kernel
+        // only (a snapshot series in column rows as well)
         if (lc.ctrl != CTRL_PI || lc.tail != kTailGroupOnly) return hipErrorInvalidValue;
         const int g = sp.pow2 ? 1 : 0;
         if (f.nsnap > 0) {
             if (f.layout != LAYOUT_COLUMNS) return hipErrorInvalidValue;
 #define GROUP_TC(M, MO, G)                                                     \
     if (method == M && f.mode == MO && g == G)                                 \
-        return launch_group_t<M, MO, G, PTC_FP64, 1, CTRL_PI, ROWS_COLUMNS>(f, sp, p, t_stop, kBlock, stream);
+        return lc.lanes == 3                                                   \
+                   ? launch_trio_t<M, MO, G, PTC_FP64, ROWS_COLUMNS, CTRL_PI, 1>(f, sp, p, t_stop, kBlock, stream) \
+                   : launch_group_t<M, MO, G, PTC_FP64, 1, CTRL_PI, ROWS_COLUMNS>(f, sp, p, t_stop, kBlock, stream);
             GROUP_TC(PTC_RK5, PTC_MODE_VB, 0) GROUP_TC(PTC_RK5, PTC_MODE_EB, 0)
             GROUP_TC(PTC_DOP853, PTC_MODE_VB, 0) GROUP_TC(PTC_DOP853, PTC_MODE_EB, 0)
             GROUP_TC(PTC_RK5, PTC_MODE_VB, 1) GROUP_TC(PTC_RK5, PTC_MODE_EB, 1)
@@ -400,7 +402,9 @@ hipError_t launch_integrate(int method, const DeviceFields &f, const StepParams 
             const int g = sp.pow2 ? 1 : 0;
 #define GROUP_T(M, MO, G)                                                      \
     if (method == M && f.mode == MO && g == G)                                 \
-        return launch_group_t<M, MO, G, PTC_FP64, 1>(f, sp, p, t_stop, kBlock, stream);
+        return lc.lanes == 3                                                   \
+                   ? launch_trio_t<M, MO, G, PTC_FP64, ROWS_NODES, CTRL_PI, 1>(f, sp, p, t_stop, kBlock, stream) \
+                   : launch_group_t<M, MO, G, PTC_FP64, 1>(f, sp, p, t_stop, kBlock, stream);
             GROUP_T(PTC_RK5, PTC_MODE_VB, 0) GROUP_T(PTC_RK5, PTC_MODE_EB, 0)
             GROUP_T(PTC_DOP853, PTC_MODE_VB, 0) GROUP_T(PTC_DOP853, PTC_MODE_EB, 0)
             GROUP_T(PTC_RK5, PTC_MODE_VB, 1) GROUP_T(PTC_RK5, PTC_MODE_EB, 1)

@@ -19,14 +19,16 @@ hipError_t launch_integrate_mixed(int method, const DeviceFields &f, const StepP
         return hipErrorInvalidValue;
     const int geom = sp.pow2 ? 1 : 0;
     if (rows_layout(f.layout)) {
-        // pair, quad or column rows: the trio or group kernel only; a
-        // snapshot series (column rows, PI controller) the group kernel
+        // pair, quad or column rows: the trio or group kernel only (a
+        // snapshot series in column rows with the PI controller as well)
         if (lc.tail != kTailGroupOnly) return hipErrorInvalidValue;
         if (f.nsnap > 0) {
             if (f.layout != LAYOUT_COLUMNS || lc.ctrl != CTRL_PI) return hipErrorInvalidValue;
 #define GROUP_TC(M, MO, G)                                                     \
     if (method == M && f.mode == MO && geom == G)                              \
-        return launch_group_t<M, MO, G, PTC_FP32, 1, CTRL_PI, ROWS_COLUMNS>(f, sp, p, t_stop, kBlock, stream);
+        return lc.lanes == 3                                                   \
+                   ? launch_trio_t<M, MO, G, PTC_FP32, ROWS_COLUMNS, CTRL_PI, 1>(f, sp, p, t_stop, kBlock, stream) \
+                   : launch_group_t<M, MO, G, PTC_FP32, 1, CTRL_PI, ROWS_COLUMNS>(f, sp, p, t_stop, kBlock, stream);
             GROUP_TC(PTC_RK5, PTC_MODE_VB, 0) GROUP_TC(PTC_RK5, PTC_MODE_EB, 0)
             GROUP_TC(PTC_DOP853, PTC_MODE_VB, 0) GROUP_TC(PTC_DOP853, PTC_MODE_EB, 0)
             GROUP_TC(PTC_RK5, PTC_MODE_VB, 1) GROUP_TC(PTC_RK5, PTC_MODE_EB, 1)
@@ -63,7 +65,9 @@ hipError_t launch_integrate_mixed(int method, const DeviceFields &f, const StepP
         if (lc.tail == kTailGroupOnly) {
 #define GROUP_T(M, MO, G)                                                      \
     if (method == M && f.mode == MO && geom == G)                              \
-        return launch_group_t<M, MO, G, PTC_FP32, 1>(f, sp, p, t_stop, kBlock, stream);
+        return lc.lanes == 3                                                   \
+                   ? launch_trio_t<M, MO, G, PTC_FP32, ROWS_NODES, CTRL_PI, 1>(f, sp, p, t_stop, kBlock, stream) \
+                   : launch_group_t<M, MO, G, PTC_FP32, 1>(f, sp, p, t_stop, kBlock, stream);
             GROUP_T(PTC_RK5, PTC_MODE_VB, 0) GROUP_T(PTC_RK5, PTC_MODE_EB, 0)
             GROUP_T(PTC_DOP853, PTC_MODE_VB, 0) GROUP_T(PTC_DOP853, PTC_MODE_EB, 0)
             GROUP_T(PTC_RK5, PTC_MODE_VB, 1) GROUP_T(PTC_RK5, PTC_MODE_EB, 1)

@@ -37,10 +37,15 @@ constexpr int kTriosPerWave = 64 / kTrio;
 // over the trio).  The corner lookup is group_corners with the trio's three
 // lanes as the three axes.
 // FEAT = 1 also keeps, from the last evaluation, the lane's two interpolated
-// field components and the in-cell fraction (step-size features).
-template <int MODE, int GEOM, int PREC, int ROWS, int FEAT = 0>
+// field components and the in-cell fraction (step-size features).  TIME = 1
+// (a snapshot series in node or column rows): the lane's two components of
+// the cell's corner rows in snapshot k + 1 as well, the stage time and the
+// cached interval key, as GroupForce.
+template <int MODE, int GEOM, int PREC, int ROWS, int FEAT = 0, int TIME = 0>
 struct TrioForce {
     static_assert(ROWS != ROWS_QUADS || PREC == PTC_FP32, "quad rows hold FP32 fields");
+    static_assert(TIME == 0 || (FEAT == 0 && (ROWS == ROWS_NODES || ROWS == ROWS_COLUMNS)),
+                  "a series is read from node or column rows, with the PI controller");
     using K = std::conditional_t<PREC == PTC_FP32, float, double>;
     const DeviceFields &F;
     const StepParams &P;
@@ -49,6 +54,9 @@ struct TrioForce {
     uint32_t key0 = 0xffffffffu, key7 = 0xffffffffu;
     uint32_t loads = 0;  // cell loads of the trio's particle (the same on each lane)
     K v[2][8];           // components a and a + 3 of the cached cell's corner rows
+    K v1[TIME ? 2 : 1][8];  // TIME: the same in snapshot k + 1
+    double ts = 0.0;        // TIME: the stage time
+    int kh = 0, kc = -1;    // TIME: the last interval, the cached interval key
     K fint0 = K(0), fint1 = K(0);          // FEAT: U or E component a, B component a
     float xdl[3] = {0.0f, 0.0f, 0.0f};     // FEAT: the in-cell fraction
 
@@ -58,15 +66,21 @@ struct TrioForce {
     {
     }
 
+    __device__ __forceinline__ void at(double t)
+    {
+        if constexpr (TIME == 1) ts = t;
+    }
+
     // node rows (six FP64 values per node, or eight FP32: six and |B|, pad)
-    __device__ __forceinline__ void load_nodes(const uint32_t *o)
+    // of snapshot k into dst
+    __device__ __forceinline__ void load_nodes(const uint32_t *o, K (*dst)[8], int k = 0) const
     {
         constexpr int kRow = PREC == PTC_FP32 ? 8 : 6;
-        const K *__restrict__ d = (const K *)F.data;
+        const K *__restrict__ d = (const K *)F.data + (size_t)k * F.sstride;
 #pragma unroll
         for (int h = 0; h < 2; h++)
 #pragma unroll
-            for (int j = 0; j < 8; j++) v[h][j] = d[(size_t)o[j] * kRow + a + 3 * h];
+            for (int j = 0; j < 8; j++) dst[h][j] = d[(size_t)o[j] * kRow + a + 3 * h];
     }
 
     // pair rows: as GroupForce::load_pairs, for both components
@@ -98,57 +112,66 @@ struct TrioForce {
         }
     }
 
-    // FP32 quad rows (node order, or column storage): as
-    // GroupForce::load_quads, for both components
-    __device__ __forceinline__ void load_quads(const uint32_t *o)
+    // FP32 quad rows (node order, or column storage; snapshot k of a
+    // series) into dst: as GroupForce::load_quads, for both components
+    __device__ __forceinline__ void load_quads(const uint32_t *o, K (*dst)[8], int k = 0) const
     {
         const uint32_t sx = ROWS == ROWS_COLUMNS ? (uint32_t)P.dm[2] : 1u;
         const uint32_t sy = (uint32_t)P.dm[0] * sx;
         const uint32_t dx = o[4] - o[0], dy = o[2] - o[0];
         const bool xs = dx == sx || dx == (uint32_t)((int64_t)(1 - P.dm[0]) * sx);
         const bool ys = dy == sy || dy == (uint32_t)((int64_t)(1 - P.dm[1]) * sy);
-        const float4 *__restrict__ d4 = (const float4 *)F.data;
+        const float *__restrict__ d = (const float *)F.data + (size_t)k * F.sstride;
+        const float4 *__restrict__ d4 = (const float4 *)d;
 #pragma unroll
         for (int h = 0; h < 2; h++) {
             const int c = a + 3 * h;
             const float4 q0 = d4[(size_t)o[0] * 8 + c], q1 = d4[(size_t)o[1] * 8 + c];
-            v[h][0] = q0.x; v[h][1] = q1.x; v[h][2] = q0.z; v[h][3] = q1.z;
-            v[h][4] = q0.y; v[h][5] = q1.y; v[h][6] = q0.w; v[h][7] = q1.w;
+            dst[h][0] = q0.x; dst[h][1] = q1.x; dst[h][2] = q0.z; dst[h][3] = q1.z;
+            dst[h][4] = q0.y; dst[h][5] = q1.y; dst[h][6] = q0.w; dst[h][7] = q1.w;
         }
         if (!(xs && ys)) {
-            const float *__restrict__ d = (const float *)F.data;
 #pragma unroll
             for (int h = 0; h < 2; h++)
 #pragma unroll
                 for (int j = 2; j < 8; j++)
-                    v[h][j] = d[(size_t)o[j] * 32 + 4 * (a + 3 * h)];
+                    dst[h][j] = d[(size_t)o[j] * 32 + 4 * (a + 3 * h)];
         }
     }
 
-    // column rows: as GroupForce::load_columns, for both components
-    __device__ __forceinline__ void load_columns(const uint32_t *o)
+    // column rows (snapshot k of a series) into dst: as
+    // GroupForce::load_columns, for both components
+    __device__ __forceinline__ void load_columns(const uint32_t *o, K (*dst)[8], int k = 0) const
     {
         const uint32_t sx = (uint32_t)P.dm[2], sy = (uint32_t)P.dm[0] * (uint32_t)P.dm[2];
         const uint32_t dx = o[4] - o[0], dy = o[2] - o[0];
         const bool xs = dx == sx || dx == (uint32_t)((int64_t)(1 - P.dm[0]) * P.dm[2]);
         const bool ys = dy == sy || dy == (uint32_t)((int64_t)(1 - P.dm[1]) * sy);
-        const double2 *__restrict__ d2 = (const double2 *)F.data;
+        const double *__restrict__ d = (const double *)F.data + (size_t)k * F.sstride;
+        const double2 *__restrict__ d2 = (const double2 *)d;
 #pragma unroll
         for (int h = 0; h < 2; h++) {
             const int c = a + 3 * h;
             const double2 a0 = d2[(size_t)o[0] * 12 + 2 * c], a1 = d2[(size_t)o[0] * 12 + 2 * c + 1];
             const double2 b0 = d2[(size_t)o[1] * 12 + 2 * c], b1 = d2[(size_t)o[1] * 12 + 2 * c + 1];
-            v[h][0] = a0.x; v[h][1] = b0.x; v[h][2] = a1.x; v[h][3] = b1.x;
-            v[h][4] = a0.y; v[h][5] = b0.y; v[h][6] = a1.y; v[h][7] = b1.y;
+            dst[h][0] = a0.x; dst[h][1] = b0.x; dst[h][2] = a1.x; dst[h][3] = b1.x;
+            dst[h][4] = a0.y; dst[h][5] = b0.y; dst[h][6] = a1.y; dst[h][7] = b1.y;
         }
         if (!(xs && ys)) {
-            const double *__restrict__ d = (const double *)F.data;
 #pragma unroll
             for (int h = 0; h < 2; h++)
 #pragma unroll
                 for (int j = 2; j < 8; j++)
-                    v[h][j] = d[(size_t)o[j] * 24 + 4 * (a + 3 * h)];
+                    dst[h][j] = d[(size_t)o[j] * 24 + 4 * (a + 3 * h)];
         }
+    }
+
+    // snapshot k's rows into dst, in the kernel's row form (a series)
+    __device__ __forceinline__ void load_snapshot(const uint32_t *o, K (*dst)[8], int k) const
+    {
+        if constexpr (ROWS == ROWS_COLUMNS && PREC == PTC_FP32) load_quads(o, dst, k);
+        else if constexpr (ROWS == ROWS_COLUMNS) load_columns(o, dst, k);
+        else load_nodes(o, dst, k);
     }
 
     // components a and a + 3 of f(s), s given by its components a and a + 3
@@ -157,23 +180,60 @@ struct TrioForce {
     {
         CornersT<K> c;
         group_corners<K, GEOM, ROWS>(P, s[0], base, a, c, FEAT ? xdl : nullptr);
-        if (c.o[0] != key0 || c.o[7] != key7) {
-            loads++;
-            if constexpr (ROWS == ROWS_PAIRS) load_pairs(c.o);
-            else if constexpr (ROWS == ROWS_QUADS) load_quads(c.o);
-            else if constexpr (ROWS == ROWS_COLUMNS && PREC == PTC_FP32) load_quads(c.o);
-            else if constexpr (ROWS == ROWS_COLUMNS) load_columns(c.o);
-            else load_nodes(c.o);
-            key0 = c.o[0];
-            key7 = c.o[7];
-        }
-        // interpolations.F90:212-220, corners summed left to right: U or E
-        // component a in f0, B component a in f1
-        K f0 = c.w[0] * v[0][0], f1 = c.w[0] * v[1][0];
+        K f0, f1;
+        if constexpr (TIME == 1) {
+            // the two bracketing snapshots' interpolants blended in time, as
+            // GroupForce
+            double al;
+            bool two;
+            const int k = series_interval(F, ts, kh, al, two);
+            const int key = 2 * k + (two ? 1 : 0);
+            if (!(c.o[0] == key0 && c.o[7] == key7 && (key == kc || (!two && key + 1 == kc)))) {
+                loads += two ? 2u : 1u;
+                load_snapshot(c.o, v, k);
+                if (two) load_snapshot(c.o, v1, k + 1);
+                key0 = c.o[0];
+                key7 = c.o[7];
+                kc = key;
+            }
+            f0 = c.w[0] * v[0][0];
+            f1 = c.w[0] * v[1][0];
 #pragma unroll
-        for (int j = 1; j < 8; j++) {
-            f0 = f0 + c.w[j] * v[0][j];
-            f1 = f1 + c.w[j] * v[1][j];
+            for (int j = 1; j < 8; j++) {
+                f0 = f0 + c.w[j] * v[0][j];
+                f1 = f1 + c.w[j] * v[1][j];
+            }
+            if (two) {
+                K g0 = c.w[0] * v1[0][0], g1 = c.w[0] * v1[1][0];
+#pragma unroll
+                for (int j = 1; j < 8; j++) {
+                    g0 = g0 + c.w[j] * v1[0][j];
+                    g1 = g1 + c.w[j] * v1[1][j];
+                }
+                const K ak = K(al);
+                f0 = f0 + ak * (g0 - f0);
+                f1 = f1 + ak * (g1 - f1);
+            }
+        } else {
+            if (c.o[0] != key0 || c.o[7] != key7) {
+                loads++;
+                if constexpr (ROWS == ROWS_PAIRS) load_pairs(c.o);
+                else if constexpr (ROWS == ROWS_QUADS) load_quads(c.o, v);
+                else if constexpr (ROWS == ROWS_COLUMNS && PREC == PTC_FP32) load_quads(c.o, v);
+                else if constexpr (ROWS == ROWS_COLUMNS) load_columns(c.o, v);
+                else load_nodes(c.o, v);
+                key0 = c.o[0];
+                key7 = c.o[7];
+            }
+            // interpolations.F90:212-220, corners summed left to right: U or E
+            // component a in f0, B component a in f1
+            f0 = c.w[0] * v[0][0];
+            f1 = c.w[0] * v[1][0];
+#pragma unroll
+            for (int j = 1; j < 8; j++) {
+                f0 = f0 + c.w[j] * v[0][j];
+                f1 = f1 + c.w[j] * v[1][j];
+            }
         }
         if constexpr (FEAT == 1) {
             fint0 = f0;
@@ -307,7 +367,7 @@ __device__ __forceinline__ double trio_dti_guess(const StepParams &P, int iord, 
 template <class TF>
 __device__ __forceinline__ double trio_attempt_rk5(TF &rhsf, const StepParams &P, int base,
                                                    const double *si, const typename TF::K *f1,
-                                                   double dt, double *sf, double *k6d)
+                                                   double t, double dt, double *sf, double *k6d)
 {
     using K = typename TF::K;
     double k1[2], k2[2], k3[2], k4[2], k5[2], k6[2], s[2];
@@ -317,24 +377,28 @@ __device__ __forceinline__ double trio_attempt_rk5(TF &rhsf, const StepParams &P
         k1[h] = double(K(dt * double(f1[h])));
         s[h] = si[h] + CK_A21 * k1[h];
     }
+    rhsf.at(t + CK_N2 * dt);
     rhsf(s, r);
 #pragma unroll
     for (int h = 0; h < 2; h++) {
         k2[h] = double(K(dt * double(r[h])));
         s[h] = si[h] + CK_A31 * k1[h] + CK_A32 * k2[h];
     }
+    rhsf.at(t + CK_N3 * dt);
     rhsf(s, r);
 #pragma unroll
     for (int h = 0; h < 2; h++) {
         k3[h] = double(K(dt * double(r[h])));
         s[h] = si[h] + CK_A41 * k1[h] + CK_A42 * k2[h] + CK_A43 * k3[h];
     }
+    rhsf.at(t + CK_N4 * dt);
     rhsf(s, r);
 #pragma unroll
     for (int h = 0; h < 2; h++) {
         k4[h] = double(K(dt * double(r[h])));
         s[h] = si[h] + CK_A51 * k1[h] + CK_A52 * k2[h] + CK_A53 * k3[h] + CK_A54 * k4[h];
     }
+    rhsf.at(t + dt);
     rhsf(s, r);
 #pragma unroll
     for (int h = 0; h < 2; h++) {
@@ -342,6 +406,7 @@ __device__ __forceinline__ double trio_attempt_rk5(TF &rhsf, const StepParams &P
         s[h] = si[h] + CK_A61 * k1[h] + CK_A62 * k2[h] + CK_A63 * k3[h] + CK_A64 * k4[h] +
                CK_A65 * k5[h];
     }
+    rhsf.at(t + CK_N6 * dt);
     rhsf(s, r);
     const double eps = 2.2250738585072014e-308;  // tiny(1d0), integrations.F90:99
     double qc[2];
@@ -380,8 +445,8 @@ __device__ __forceinline__ double trio_attempt_rk5(TF &rhsf, const StepParams &P
 template <class TF>
 __device__ __forceinline__ double trio_attempt_dp853(TF &rhsf, const StepParams &P, int base,
                                                      const double *si,
-                                                     const typename TF::K *k1k, double dt,
-                                                     double *sf)
+                                                     const typename TF::K *k1k, double t,
+                                                     double dt, double *sf)
 {
     using K = typename TF::K;
     double k1[2], k2[2], k3[2], k4[2], k5[2], k6[2], k7[2], k8[2], k9[2], k10[2], k11[2],
@@ -392,36 +457,42 @@ __device__ __forceinline__ double trio_attempt_dp853(TF &rhsf, const StepParams 
         k1[h] = double(k1k[h]);
         ss[h] = si[h] + dt * A0201 * k1[h];
     }
+    rhsf.at(t + DN02 * dt);
     rhsf(ss, r);
 #pragma unroll
     for (int h = 0; h < 2; h++) {
         k2[h] = double(r[h]);
         ss[h] = si[h] + dt * (A0301 * k1[h] + A0302 * k2[h]);
     }
+    rhsf.at(t + DN03 * dt);
     rhsf(ss, r);
 #pragma unroll
     for (int h = 0; h < 2; h++) {
         k3[h] = double(r[h]);
         ss[h] = si[h] + dt * (A0401 * k1[h] + A0403 * k3[h]);
     }
+    rhsf.at(t + DN04 * dt);
     rhsf(ss, r);
 #pragma unroll
     for (int h = 0; h < 2; h++) {
         k4[h] = double(r[h]);
         ss[h] = si[h] + dt * (A0501 * k1[h] + A0503 * k3[h] + A0504 * k4[h]);
     }
+    rhsf.at(t + DN05 * dt);
     rhsf(ss, r);
 #pragma unroll
     for (int h = 0; h < 2; h++) {
         k5[h] = double(r[h]);
         ss[h] = si[h] + dt * (A0601 * k1[h] + A0604 * k4[h] + A0605 * k5[h]);
     }
+    rhsf.at(t + DN06 * dt);
     rhsf(ss, r);
 #pragma unroll
     for (int h = 0; h < 2; h++) {
         k6[h] = double(r[h]);
         ss[h] = si[h] + dt * (A0701 * k1[h] + A0704 * k4[h] + A0705 * k5[h] + A0706 * k6[h]);
     }
+    rhsf.at(t + DN07 * dt);
     rhsf(ss, r);
 #pragma unroll
     for (int h = 0; h < 2; h++) {
@@ -429,6 +500,7 @@ __device__ __forceinline__ double trio_attempt_dp853(TF &rhsf, const StepParams 
         ss[h] = si[h] + dt * (A0801 * k1[h] + A0804 * k4[h] + A0805 * k5[h] + A0806 * k6[h] +
                               A0807 * k7[h]);
     }
+    rhsf.at(t + DN08 * dt);
     rhsf(ss, r);
 #pragma unroll
     for (int h = 0; h < 2; h++) {
@@ -436,6 +508,7 @@ __device__ __forceinline__ double trio_attempt_dp853(TF &rhsf, const StepParams 
         ss[h] = si[h] + dt * (A0901 * k1[h] + A0904 * k4[h] + A0905 * k5[h] + A0906 * k6[h] +
                               A0907 * k7[h] + A0908 * k8[h]);
     }
+    rhsf.at(t + DN09 * dt);
     rhsf(ss, r);
 #pragma unroll
     for (int h = 0; h < 2; h++) {
@@ -443,6 +516,7 @@ __device__ __forceinline__ double trio_attempt_dp853(TF &rhsf, const StepParams 
         ss[h] = si[h] + dt * (A1001 * k1[h] + A1004 * k4[h] + A1005 * k5[h] + A1006 * k6[h] +
                               A1007 * k7[h] + A1008 * k8[h] + A1009 * k9[h]);
     }
+    rhsf.at(t + DN10 * dt);
     rhsf(ss, r);
 #pragma unroll
     for (int h = 0; h < 2; h++) {
@@ -450,6 +524,7 @@ __device__ __forceinline__ double trio_attempt_dp853(TF &rhsf, const StepParams 
         ss[h] = si[h] + dt * (A1101 * k1[h] + A1104 * k4[h] + A1105 * k5[h] + A1106 * k6[h] +
                               A1107 * k7[h] + A1108 * k8[h] + A1109 * k9[h] + A1110 * k10[h]);
     }
+    rhsf.at(t + DN11 * dt);
     rhsf(ss, r);
 #pragma unroll
     for (int h = 0; h < 2; h++) {
@@ -458,6 +533,7 @@ __device__ __forceinline__ double trio_attempt_dp853(TF &rhsf, const StepParams 
                               A1207 * k7[h] + A1208 * k8[h] + A1209 * k9[h] + A1210 * k10[h] +
                               A1211 * k11[h]);
     }
+    rhsf.at(t + dt);
     rhsf(ss, r);
     double e3[2], e5[2];
 #pragma unroll
@@ -526,14 +602,16 @@ constexpr int kTrioWaves = 2;
 // reference's controller; CTRL_LEARNED (FP32 fields): after an accepted step
 // the network proposes the next step, as in the group kernel, evaluated once
 // per wave for its 21 particles (mlp_forward_groups<3>; the network and a
-// 64x72 BF16 buffer per wave in dynamic LDS).
-template <int METHOD, int MODE, int GEOM, int PREC, int ROWS, int CTRL = CTRL_PI>
+// 64x72 BF16 buffer per wave in dynamic LDS).  TIME = 1: a snapshot series
+// in node or column rows (PI controller), every RHS at its stage time as in
+// the group kernel.
+template <int METHOD, int MODE, int GEOM, int PREC, int ROWS, int CTRL = CTRL_PI, int TIME = 0>
 __global__ void __launch_bounds__(kBlock, kTrioWaves)
 integrate_trio_kernel(const DeviceFields F, const StepParams P, const DeviceParticles S,
                       double t_stop)
 {
-    static_assert(CTRL == CTRL_PI || (CTRL == CTRL_LEARNED && PREC == PTC_FP32),
-                  "the trio kernel's learned controller takes FP32 fields");
+    static_assert(CTRL == CTRL_PI || (CTRL == CTRL_LEARNED && PREC == PTC_FP32 && TIME == 0),
+                  "the trio kernel's learned controller takes static FP32 fields");
     __shared__ double s_logt[128][3];
     __shared__ unsigned long long s_expt[256];
     for (int i = threadIdx.x; i < 128 * 3; i += blockDim.x)
@@ -551,7 +629,7 @@ integrate_trio_kernel(const DeviceFields F, const StepParams P, const DevicePart
     const int base = (lane / kTrio) * kTrio, a = lane - base;
     const double t_end = t_stop < P.tmx ? t_stop : P.tmx;
     const int64_t N = S.np;
-    TrioForce<MODE, GEOM, PREC, ROWS, CTRL == CTRL_LEARNED ? 1 : 0> force(F, P, base, a);
+    TrioForce<MODE, GEOM, PREC, ROWS, CTRL == CTRL_LEARNED ? 1 : 0, TIME> force(F, P, base, a);
     using K = typename decltype(force)::K;
     // this lane's two state planes, chosen once (indexing the kernel
     // argument's pointer array by a per-lane value would copy it to scratch)
@@ -616,6 +694,7 @@ integrate_trio_kernel(const DeviceFields F, const StepParams P, const DevicePart
 
         // ---- f(si) for new particles and after accepted steps ----
         if (need_rhs) {
+            force.at(t);
             force(si, f1);
             c_rhs++;
             need_rhs = false;
@@ -691,7 +770,7 @@ integrate_trio_kernel(const DeviceFields F, const StepParams P, const DevicePart
                 double sf[2], dtn = 0.0;
                 if constexpr (METHOD == PTC_RK5) {
                     double k6[2];
-                    const double del = trio_attempt_rk5(force, P, base, si, f1, dt, sf, k6);
+                    const double del = trio_attempt_rk5(force, P, base, si, f1, t, dt, sf, k6);
                     c_rhs += 5;
                     if (!isfinite(del) || !trio_finite(sf, base)) {
                         status = PTC_STATUS_NONFINITE;
@@ -730,7 +809,7 @@ integrate_trio_kernel(const DeviceFields F, const StepParams P, const DevicePart
                         }
                     }
                 } else {
-                    const double err = trio_attempt_dp853(force, P, base, si, f1, dt, sf);
+                    const double err = trio_attempt_dp853(force, P, base, si, f1, t, dt, sf);
                     c_rhs += 11;
                     if (!isfinite(err) || !trio_finite(sf, base)) {
                         status = PTC_STATUS_NONFINITE;
@@ -742,6 +821,7 @@ integrate_trio_kernel(const DeviceFields F, const StepParams P, const DevicePart
                             t = t + dt;
                             if (n <= P.m && t >= tnext) {
                                 K fe[2];
+                                force.at(t);
                                 force(sf, fe);
                                 c_rhs++;
                                 double ka[2], kb[2];
@@ -822,11 +902,11 @@ integrate_trio_kernel(const DeviceFields F, const StepParams P, const DevicePart
     add_cell_loads(S, a == 0 && lane < kTrio * kTriosPerWave ? force.loads : 0u);
 }
 
-template <int METHOD, int MODE, int GEOM, int PREC, int ROWS, int CTRL = CTRL_PI>
+template <int METHOD, int MODE, int GEOM, int PREC, int ROWS, int CTRL = CTRL_PI, int TIME = 0>
 hipError_t launch_trio_t(const DeviceFields &f, const StepParams &sp, const DeviceParticles &p,
                          double t_stop, int block, hipStream_t stream)
 {
-    auto kern = integrate_trio_kernel<METHOD, MODE, GEOM, PREC, ROWS, CTRL>;
+    auto kern = integrate_trio_kernel<METHOD, MODE, GEOM, PREC, ROWS, CTRL, TIME>;
     const size_t lds =
         CTRL == CTRL_LEARNED ? kNetLds + (size_t)(block / 64) * sizeof(MlpWaveBuf) : 0;
     if (CTRL == CTRL_LEARNED && !p.net) return hipErrorInvalidValue;

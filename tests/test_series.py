@@ -349,6 +349,55 @@ def test_column_row_series_matches_node_rows(oracle, method, mode):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("method", ["rk5", "dop853"])
+@pytest.mark.parametrize("mode", ["vb", "eb"])
+@pytest.mark.parametrize("rows", ["nodes", "columns"])
+def test_trio_series_matches_group_series(oracle, method, mode, rows):
+    """A snapshot series in the trio kernel (three lanes per particle,
+    group_lanes 3) against the six-lane group kernel, with the group or trio
+    kernel running the whole launch: FP64 bit-identical to each other and to
+    the oracle's restatement, FP32 identical to each other, on node and
+    column rows, with particles on cell faces and past the last snapshot
+    time."""
+    import particles_amd as P
+    tt = TIMES[:3]
+    uu, bb = W.turbulent_vb_series(24, seed=11, times=tt, speed=0.02, nmodes=128)
+    if mode == "eb":
+        uu = np.asfortranarray(0.05 * uu)
+    p = W.params_vector(tmax=4 * W.T0, dtmax=1.0, atol=1e-8, rtol=1e-8)
+    tg = W.time_grid(4, p[2])
+    st0 = W.initial_state(400, len(tg), seed=12, gamma_range=(1.0, 1e3))
+    rng = np.random.default_rng(28)
+    for a in range(3):
+        st0[a, 0, 100 * a:100 * a + 60] = rng.integers(0, 24, 60) / 24.0
+    st0[:3, 0, 300:320] = 1.0 - 1e-12
+    q = W.params_vector(tmax=0.2 * W.T0, dtmax=1.0, atol=1e-8, rtol=1e-8)
+    tgq = W.time_grid(2, q[2])
+    make = device_series_columns if rows == "columns" else device_series
+    so, co, _ = oracle.integrate_series(method, uu, bb, tt, p, tg, st0, mode=mode)
+    got = {}
+    try:
+        P.set_option("tail", "group")
+        for lanes in ("6", "3"):
+            say(f"[trio series] {method} {mode} {rows} group_lanes {lanes}")
+            P.set_option("group_lanes", lanes)
+            with make(P, uu, bb, tt, mode=mode) as f:
+                r64 = P.integrate(f, method, p, tg, st0)
+            with make(P, uu, bb, tt, mode=mode, prec="fp32") as f:
+                r32 = P.integrate(f, method, q, tgq, st0[:, :len(tgq), :])
+            got[lanes] = (r64, r32)
+    finally:
+        P.set_option("tail", "auto")
+        P.set_option("group_lanes", "auto")
+    for lanes in ("6", "3"):
+        (st, cnt, status), (s32, c32, st32) = got[lanes]
+        assert (status == 0).all() and (st32 == 0).all(), lanes
+        assert np.array_equal(st, so) and np.array_equal(cnt, co), lanes
+    assert np.array_equal(got["3"][1][0], got["6"][1][0])
+    assert np.array_equal(got["3"][1][1], got["6"][1][1])
+
+
+@pytest.mark.gpu
 def test_column_row_series_refuses_mixed_layouts():
     import particles_amd as P
     uu, bb = W.turbulent_vb(8, seed=22, nmodes=16)
