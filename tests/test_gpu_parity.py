@@ -936,11 +936,21 @@ def test_trio_kernel_matches_group_kernel(method, form):
 
 @pytest.mark.parametrize("form", ["pairs", "quads", "columns", "fp64 pairs", "fp64 columns"])
 def test_pair_rows_refused_where_node_rows_are_read(form):
-    """Probes, record mode and snapshot series read node rows: they refuse
-    pair, quad and column rows with an error; the learned controller runs
-    on FP32 pair, quad and column rows (group kernel) with the results it
-    gives on node rows, and refuses FP64 pair and column rows (it runs FP64
-    fields one lane per particle)."""
+    """Probes and record mode read node rows: they refuse pair, quad and
+    column rows with an error; snapshot series take column rows and refuse
+    pair and quad rows; the learned controller runs on FP32 pair, quad and
+    column rows (group kernel) with the results it gives on node rows, and
+    refuses FP64 pair and column rows (it runs FP64 fields one lane per
+    particle)."""
+
+    def series_check(f):
+        if form.endswith("columns"):
+            with P.Fields.series([f, f], [0.0, 1.0]) as s:
+                assert s.download(1)[0].shape == uu.shape
+        else:
+            with pytest.raises(P.PtcError):
+                P.Fields.series([f, f], [0.0, 1.0])
+
     from oracle import mlp_oracle as M
     uu, bb = W.turbulent_vb(16, seed=57, nmodes=32)
     p = W.params_vector(tmax=0.5 * W.T0, dtmax=1.0, atol=1e-8, rtol=1e-8)
@@ -952,8 +962,7 @@ def test_pair_rows_refused_where_node_rows_are_read(form):
         with _fields_rows(uu, bb, form) as f, P.Mlp(zero) as net:
             with pytest.raises(P.PtcError):
                 P.probe(f, "gather", np.full((4, 3), 0.3), p)
-            with pytest.raises(P.PtcError):
-                P.Fields.series([f, f], [0.0, 1.0])
+            series_check(f)
             with P.Particles(f, "dop853", p, tg, st0) as b:
                 with pytest.raises(P.PtcError):
                     b.record(1.0, 1000)
@@ -966,8 +975,7 @@ def test_pair_rows_refused_where_node_rows_are_read(form):
         with P.Particles(f, "dop853", p, tg, st0) as b:
             with pytest.raises(P.PtcError):
                 b.record(1.0, 1000)
-        with pytest.raises(P.PtcError):
-            P.Fields.series([f, f], [0.0, 1.0])
+        series_check(f)
         P.set_option("tail", "group")
         try:
             with P.Particles(f, "dop853", p, tg, st0) as b:
