@@ -504,15 +504,23 @@ def test_autotune_keeps_results():
     assert np.array_equal(s, s_ref) and np.array_equal(c, c_ref) and (status == 0).all()
 
 
-def test_graph_advances_equal_eager_advances():
+@pytest.mark.parametrize("layout", ["aos", "columns"])
+def test_graph_advances_equal_eager_advances(layout):
     """Snapshot intervals submitted as one captured HIP graph give the same
     snapshots and counters as eager advances; two graphs in a row, with
-    eager work and synchronisation between them."""
+    eager work and synchronisation between them; node rows (the lane
+    kernel with its hand-over) and column rows (the trio kernel)."""
     uu, bb = W.turbulent_vb(16, seed=2, nmodes=32)
     p = W.params_vector(tmax=2.0 * W.T0, dtmax=1.0, atol=1e-8, rtol=1e-8)
     tg = W.time_grid(8, p[2])
     st0 = W.initial_state(3000, len(tg), seed=7, gamma_range=(1.0, 100.0))
-    with P.Fields(uu, bb) as f:
+    old = P.get_option("layout")
+    P.set_option("layout", layout)
+    try:
+        f = P.Fields(uu, bb)
+    finally:
+        P.set_option("layout", old)
+    with f:
         with P.Particles(f, "dop853", p, tg, st0) as b:
             for t in tg[1:]:
                 b.advance(t)

@@ -78,16 +78,25 @@ def _run_noalloc(fields, method, p, tg, st0, with_status=True):
             b.free()
 
 
-def test_noalloc_matches_integrate_twice_then_smaller():
+@pytest.mark.parametrize("layout", ["aos", "columns"])
+def test_noalloc_matches_integrate_twice_then_smaller(layout):
+    """Node rows (one lane per particle with the hand-over) and column rows
+    (the trio kernel)."""
     uu, bb = W.turbulent_vb(16, seed=21, nmodes=64)
-    with P.Fields(uu, bb) as f:
+    old = P.get_option("layout")
+    P.set_option("layout", layout)
+    try:
+        f = P.Fields(uu, bb)
+    finally:
+        P.set_option("layout", old)
+    with f:
         # first call: DOP853, 3,000 particles, four snapshots
         p = W.params_vector(tmax=0.5 * W.T0, dtmax=1.0, atol=1e-8, rtol=1e-8)
         tg = W.time_grid(3, p[2])
         st0 = W.initial_state(3000, len(tg), seed=22, gamma_range=(1.0, 1000.0))
         s_ref, c_ref, t_ref = P.integrate(f, "dop853", p, tg, st0)
         s, c, t = _run_noalloc(f, "dop853", p, tg, st0)
-        print(f"noalloc dop853 3000: {int(c[1].sum())} accepted steps", flush=True)
+        print(f"noalloc {layout} dop853 3000: {int(c[1].sum())} accepted steps", flush=True)
         assert (t_ref == 0).all()
         assert np.array_equal(s, s_ref) and np.array_equal(c, c_ref) and np.array_equal(t, t_ref)
         # second call, smaller: Cash-Karp, 700 other particles, another box,
@@ -99,7 +108,7 @@ def test_noalloc_matches_integrate_twice_then_smaller():
                               xl=(-1.0, 0.5, 2.0), xs=(2.5, 3.0, 1.75))
         s_ref, c_ref, t_ref = P.integrate(f, "rk5", g, tg2, st2)
         s, c, t = _run_noalloc(f, "rk5", g, tg2, st2, with_status=False)
-        print(f"noalloc rk5 700: {int(c[1].sum())} accepted steps", flush=True)
+        print(f"noalloc {layout} rk5 700: {int(c[1].sum())} accepted steps", flush=True)
         assert np.array_equal(s, s_ref) and np.array_equal(c, c_ref)
         assert (t == -7).all()  # status not written when not requested
 
