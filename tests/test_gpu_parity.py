@@ -890,6 +890,37 @@ def test_pair_rows_match_node_rows(method, form):
         P.set_option("tail", "auto")
 
 
+@pytest.mark.parametrize("dims", [(2, 3, 2), (5, 2, 3), (3, 4, 2)])
+def test_row_forms_on_the_smallest_grids(oracle, dims):
+    """Grids of two to five nodes per axis, where the periodic +x and +y
+    neighbours wrap within one or two cells: FP64 pair and column rows with
+    the six-lane and trio kernels bit-identical to the oracle, FP32 pair,
+    quad and column rows identical to FP32 node rows (group kernel)."""
+    uu, bb = W.turbulent_vb(8, seed=91, nmodes=16)
+    uu = np.asfortranarray(uu[:dims[0], :dims[1], :dims[2]])
+    bb = np.asfortranarray(bb[:dims[0], :dims[1], :dims[2]])
+    p = W.params_vector(tmax=0.5 * W.T0, dtmax=1.0, atol=1e-8, rtol=1e-8)
+    tg = W.time_grid(2, p[2])
+    st0 = W.initial_state(300, len(tg), seed=92, gamma_range=(1.0, 30.0))
+    so, co, _ = oracle.integrate("dop853", uu, bb, p, tg, st0)
+    try:
+        P.set_option("tail", "group")
+        with _fields_rows(uu, bb, "nodes") as f:
+            ref32 = P.integrate(f, "dop853", p, tg, st0)
+        for form in ("fp64 pairs", "fp64 columns", "pairs", "quads", "columns"):
+            for lanes in ("6", "3"):
+                P.set_option("group_lanes", lanes)
+                say(f"[small grid] {dims} {form} group_lanes {lanes}")
+                with _fields_rows(uu, bb, form) as f:
+                    st, cnt, status = P.integrate(f, "dop853", p, tg, st0)
+                assert (status == 0).all(), (form, lanes)
+                want = (so, co) if form.startswith("fp64") else ref32[:2]
+                assert np.array_equal(st, want[0]) and np.array_equal(cnt, want[1]), (form, lanes)
+    finally:
+        P.set_option("tail", "auto")
+        P.set_option("group_lanes", "auto")
+
+
 @pytest.mark.parametrize("form", ["fp64 nodes", "fp64 pairs", "fp64 columns", "nodes", "pairs",
                                   "quads", "columns"])
 @pytest.mark.parametrize("method", ["rk5", "dop853"])
