@@ -379,6 +379,35 @@ def host_buffer_rate(fields, method, p, st0, n_t0):
                     "download) against the integrator kernel time of the same span"}
 
 
+def equal_error_comparison(args, f32, method, p, tg, st0, q, tga, sa, err, mixed, net):
+    """The learned controller against PI at equal accuracy: PI once more at
+    half the tolerance (the same launches, and its error on the accuracy
+    sample), PI's time per T0 interpolated log-linearly in its error between
+    the two tolerances (extrapolated outside them) at the network's median
+    error, and the network's time per T0 against it."""
+    import particles_amd as P
+    tol = float(p[5])
+    ph = np.asarray(p, dtype=np.float64).copy()
+    ph[5] = ph[6] = 0.5 * tol
+    r = run_batch(f32, method, ph, tg, st0, args.steps, args.warmup, label="mixed tol/2")
+    qh = np.asarray(q, dtype=np.float64).copy()
+    qh[5] = qh[6] = 0.5 * tol
+    e_half = err(P.integrate(f32, method, qh, tga, sa)[0])["median"]
+    e1, e_net = mixed["error_vs_fp64_tol1e-12"]["median"], net["error_vs_fp64_tol1e-12"]["median"]
+    t1 = mixed["wall_s"] / args.steps * 1e3
+    t2 = r["wall"] / args.steps * 1e3
+    t_net = net["wall_s"] / args.steps * 1e3
+    out = {"pi_tolerances": [tol, 0.5 * tol], "pi_error_median": [e1, e_half],
+           "pi_ms_per_T0": [t1, t2], "network_error_median": e_net, "network_ms_per_T0": t_net}
+    if e1 > 0 and e_half > 0 and e_net > 0 and e1 != e_half:
+        slope = (np.log(t2) - np.log(t1)) / (np.log(e_half) - np.log(e1))
+        t_pi = float(np.exp(np.log(t1) + slope * (np.log(e_net) - np.log(e1))))
+        out["pi_ms_per_T0_at_network_error"] = t_pi
+        out["network_vs_mixed_pi_at_equal_error"] = t_pi / t_net
+    print(f"equal error: {out}", file=sys.stderr, flush=True)
+    return out
+
+
 def variants(args, fields64, make_fields, p, tg, st0, method, plain, cfg):
     """Mixed precision and learned-h (mixed) on the same workload, then the
     accuracy of each against a tight FP64 run on the first particles.  The
@@ -468,6 +497,8 @@ def variants(args, fields64, make_fields, p, tg, st0, method, plain, cfg):
     out["mixed"]["deviation_vs_plain_fp64"] = dev_plain(s_mixed)
     net_line["error_vs_fp64_tol1e-12"] = err(s_learned)
     net_line["deviation_vs_plain_fp64"] = dev_plain(s_learned)
+    net_line["equal_error"] = equal_error_comparison(
+        args, f32, method, p, tg, st0, q, tga, sa, err, out["mixed"], net_line)
     out["learned_h_network"] = net_line
     if ctl.beats_pi:
         out["learned_h"] = dict(net_line, selected="network (ran faster than PI at no larger "
