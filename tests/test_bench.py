@@ -82,6 +82,19 @@ def test_default_workload_traffic_follows_the_field_storage():
                                                    "--group-lanes", "6"])) is None
 
 
+def test_secondary_workloads_have_traffic_figures():
+    """The 256^3 and 512^3 workloads' default configurations (FP64 column
+    rows; FP32 node rows at 256^3, column rows at 512^3) each get their
+    profile's traffic, in both precisions."""
+    for wl, f32 in (("turbulent256", "nodes"), ("reconnection512", "columns")):
+        for prec in ("fp64", "fp32"):
+            cfg = _cfg(["--workload", wl, "--layout", "columns", "--f32-layout", f32,
+                        "--order", "auto", "--group-lanes", "auto", "--prec", prec])
+            if wl == "turbulent256":
+                cfg["f32_layout"] = f32
+            assert bench.traffic_from_profile(cfg) is not None, (wl, prec)
+
+
 def test_launches_past_the_horizon_are_refused():
     """Every launch advances 1 T0; the 512^3 workload's horizon is 10 T0, so
     2 warm-up and 10 timed launches are refused before any device work."""
