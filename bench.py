@@ -624,15 +624,29 @@ def main():
     if args.gather_bandwidth is not None:
         import particles_amd as P
         sizes = args.gather_bandwidth or [1.0, 16.0, 128.0]
-        rows = []
+        rows, cells = [], []
         for gib in sizes:
             r = P.diag_line_gather(int(gib * (1 << 30)), 4096)
             rows.append({"buffer_GiB": gib, **r})
             print(f"{gib:g} GiB: {r['GBps']:.0f} GB/s ({r['ms']:.1f} ms)", file=sys.stderr,
                   flush=True)
+            # the cell loads of pair rows (four unrelated lines) and of
+            # column rows (one 384-byte span) from a buffer of the same size
+            for pat in ("lines", "span"):
+                c = P.diag_cell_gather(int(gib * (1 << 30)), pat)
+                cells.append({"buffer_GiB": gib, "pattern": pat, **c})
+                print(f"{gib:g} GiB cell loads, {pat}: {c['cell_loads_per_s']:.3e}/s",
+                      file=sys.stderr, flush=True)
         print(json.dumps({"metric": "random aligned 128-byte line reads", "unit": "GB/s",
                           "lines_in_flight_per_lane": 8, "bytes_per_lane_load": 16,
-                          "rows": rows}), flush=True)
+                          "rows": rows,
+                          "cell_loads": cells,
+                          "cell_loads_note": "ten groups of six lanes per wave, three waves "
+                                             "per SIMD, one cell load in flight per group: "
+                                             "'lines' four random 128-byte lines per load "
+                                             "(FP64 pair rows), 'span' one 384-byte span at "
+                                             "a random 192-byte-aligned offset (FP64 column "
+                                             "rows)"}), flush=True)
         return
     if args.calibrate_copy:
         import torch

@@ -457,6 +457,15 @@ double ptc_pow_host(double x, double y);
  * column rows put a cell's lines side by side and are served faster.
  * result = GB/s, ms, bytes read. */
 int ptc_diag_line_gather(int64_t bytes, int rounds, double result[3]);
+/* Diagnostic, no counterpart in the reference: the rate of cell loads as the
+ * integrator's group kernels make them (ten groups of six lanes per wave,
+ * three waves per SIMD, 16-byte loads, one cell load at a time per group)
+ * from a buffer of `bytes` (allocated and freed here), `cells` loads per
+ * group, timed with HIP events after one untimed launch.  pattern 0: four
+ * random aligned 128-byte lines per load (FP64 pair rows); pattern 1: one
+ * 384-byte span at a random 192-byte-aligned offset (FP64 column rows:
+ * three or four lines).  result = cell loads per second, ms, cell loads. */
+int ptc_diag_cell_gather(int64_t bytes, int pattern, int cells, double result[3]);
 
 #ifdef __cplusplus
 }

@@ -99,6 +99,7 @@ def _load():
         "ptc_last_error": ([], C.c_char_p),
         "ptc_version": ([], C.c_char_p),
         "ptc_diag_line_gather": ([i64, i32, vp], i32),
+        "ptc_diag_cell_gather": ([i64, i32, i32, vp], i32),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -130,6 +131,17 @@ def diag_line_gather(nbytes: int, rounds: int = 4096) -> dict:
     r = np.zeros(3, dtype=np.float64)
     _check(_lib.ptc_diag_line_gather(int(nbytes), int(rounds), _ptr(r)))
     return {"GBps": float(r[0]), "ms": float(r[1]), "bytes": float(r[2])}
+
+
+def diag_cell_gather(nbytes: int, pattern: str, cells: int = 8192) -> dict:
+    """Cell loads as the group kernels make them from a device buffer of
+    nbytes (ptc_diag_cell_gather): pattern "lines", four random 128-byte
+    lines per load (pair rows), or "span", one 384-byte span at a random
+    192-byte-aligned offset (column rows)."""
+    pat = {"lines": 0, "span": 1}[pattern]
+    r = np.zeros(3, dtype=np.float64)
+    _check(_lib.ptc_diag_cell_gather(int(nbytes), pat, int(cells), _ptr(r)))
+    return {"cell_loads_per_s": float(r[0]), "ms": float(r[1]), "cell_loads": float(r[2])}
 
 
 def set_option(key: str, value) -> None:

@@ -175,6 +175,51 @@ int ptc_diag_line_gather(int64_t bytes, int rounds, double result[3])
     return PTC_OK;
 }
 
+int ptc_diag_cell_gather(int64_t bytes, int pattern, int cells, double result[3])
+{
+    if (!result || bytes < 1024 * 1024 || cells < 1 || (pattern != 0 && pattern != 1))
+        return fail(PTC_EARG, "ptc_diag_cell_gather: needs bytes >= 1 MiB, pattern 0 or 1 and "
+                              "cells >= 1");
+    const uint64_t unit = pattern == 0 ? 128 : 192;
+    const uint64_t nunits = (uint64_t)bytes / unit;
+    void *buf = nullptr;
+    uint32_t *sink = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int dev = 0, cus = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    // the buffer's contents do not matter: it is read, never used
+    if (e == hipSuccess) e = hipMalloc(&buf, nunits * unit);
+    if (e == hipSuccess) e = hipMalloc((void **)&sink, sizeof(uint32_t));
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    // three 256-thread blocks per CU: three waves of ten groups per SIMD, as
+    // the six-lane group kernel runs
+    const unsigned blocks = (unsigned)(cus > 0 ? cus : 1) * 3u;
+    const double loads = (double)blocks * 4.0 * 10.0 * cells;
+    float ms = 0.f;
+    for (int pass = 0; pass < 2 && e == hipSuccess; pass++) {
+        // pass 0 warms the code and the address translation; pass 1 is timed
+        e = hipEventRecord(e0, nullptr);
+        if (e == hipSuccess)
+            e = launch_cell_gather(buf, nunits, pattern, (uint64_t)(pass + 1), cells, sink, blocks,
+                                   nullptr);
+        if (e == hipSuccess) e = hipEventRecord(e1, nullptr);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (sink) (void)hipFree(sink);
+    if (buf) (void)hipFree(buf);
+    if (e != hipSuccess) return hip_fail(e, "ptc_diag_cell_gather");
+    result[0] = loads / (ms * 1e-3);
+    result[1] = ms;
+    result[2] = loads;
+    ok();
+    return PTC_OK;
+}
+
 int ptc_set_option(const char *key, const char *value)
 {
     if (!key || !value) return fail(PTC_EARG, "ptc_set_option: null key or value");

@@ -150,6 +150,11 @@ constexpr int kDiagLineLanes = 8;
 constexpr int kDiagInFlight = 8;
 hipError_t launch_line_gather(const void *buf, uint64_t nlines, uint64_t seed, int rounds,
                               uint32_t *sink, unsigned blocks, hipStream_t stream);
+// Cell-load diagnostic: pattern 0 four random 128-byte lines per cell load
+// (nunits lines), pattern 1 one 384-byte span at a random 192-byte-aligned
+// offset (nunits 192-byte rows, at least 2).
+hipError_t launch_cell_gather(const void *buf, uint64_t nunits, int pattern, uint64_t seed,
+                              int cells, uint32_t *sink, unsigned blocks, hipStream_t stream);
 // Zeroes n int32 values (graph-safe replacement for hipMemsetAsync).
 hipError_t launch_zero_i32(int32_t *ptr, int64_t n, hipStream_t stream);
 hipError_t launch_zero_i64(int64_t *ptr, int64_t n, hipStream_t stream);

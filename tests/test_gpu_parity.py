@@ -1032,6 +1032,21 @@ def test_pair_rows_refused_where_node_rows_are_read(form):
     assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1])
 
 
+def test_cell_gather_diagnostic():
+    """The cell-load diagnostic counts the loads it makes (ten groups per
+    wave, four waves per block, three blocks per CU) and reports a positive
+    rate; it refuses an unknown pattern and a buffer under 1 MiB."""
+    for pat in ("lines", "span"):
+        r = P.diag_cell_gather(256 << 20, pat, cells=64)
+        say(f"[diag] cell loads, {pat}: {r['cell_loads_per_s']:.3e}/s")
+        assert r["cell_loads"] > 0 and r["cell_loads"] % (3 * 4 * 10 * 64) == 0
+        assert r["cell_loads_per_s"] > 0 and r["ms"] > 0
+    with pytest.raises(KeyError):
+        P.diag_cell_gather(256 << 20, "rows")
+    with pytest.raises(P.PtcError):
+        P.diag_cell_gather(1024, "span")
+
+
 def test_line_gather_diagnostic():
     """The line-gather diagnostic reads what it says it reads and reports a
     positive rate below the nominal HBM peak."""
