@@ -654,7 +654,8 @@ int ptc_fields_destroy(ptc_fields *f)
 // kernel idle, and for fields beyond 2 GB (the gathers then miss the caches
 // and the group kernel's two waves per SIMD hide their latency better);
 // otherwise one lane per particle, in FP64 handing the last 32768 over (4 %
-// faster on the 256^3 DOP853 workload; in FP32 the hand-over costs 4 %).
+// faster on the 256^3 DOP853 workload; in FP32 the hand-over costs 4 %, and
+// 10 % with the trio kernel taking the tail: 34.0 against 30.7 ms per T0).
 // A snapshot series (two snapshots per gather; no hand-over, the lane
 // kernel's series form keeps its particles to the end) takes the group
 // kernel in FP64 as well (measured on 8 snapshots of 256^3 with 100,000
