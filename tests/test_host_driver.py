@@ -185,21 +185,23 @@ def test_full_run_kernel_choices_agree(tmp_path):
 
 
 @pytest.mark.gpu
-def test_full_run_field_storage_choices_agree(tmp_path):
-    """--layout columns (the fields in column rows, the trio kernel) writes
-    the same state and counter files as the default node rows, bit for
-    bit."""
+@pytest.mark.parametrize("mixed", [False, True])
+def test_full_run_field_storage_choices_agree(tmp_path, mixed):
+    """--layout columns (FP64) and --mixed --f32-layout columns (FP32) store
+    the fields in column rows, read by the trio kernel; the state and counter
+    files equal those of the default node rows bit for bit."""
     out = {}
-    for lay in ("aos", "columns"):
+    for lay in (("nodes", "columns") if mixed else ("aos", "columns")):
         tmp = os.path.join(str(tmp_path), lay)
         os.makedirs(tmp)
         _setup(tmp, PARAMS.replace("nparticles  = 9", "nparticles  = 40"))
-        r = _run(tmp, "--layout", lay)
+        args = ("--mixed", "--f32-layout", lay) if mixed else ("--layout", lay)
+        r = _run(tmp, *args)
         assert r.returncode == 0, r.stdout + r.stderr
         out[lay] = (fits.read(os.path.join(tmp, "state.fits")),
                     fits.read(os.path.join(tmp, "counter.fits")))
-    assert np.array_equal(out["aos"][0], out["columns"][0])
-    assert np.array_equal(out["aos"][1], out["columns"][1])
+    a, b = out.values()
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
 
 
 @pytest.mark.gpu
